@@ -1,0 +1,141 @@
+/*
+ * tarang_gpu.h -- C ABI of the MI355X (gfx950) pseudo-spectral hot path.
+ *
+ * The entry points replace, on one GPU, the transform library and the
+ * solver-level hot path of Tarang (anandogc/tarang).  Each declaration names
+ * the reference interface it replaces (paths relative to the reference
+ * checkout).
+ *
+ * Conventions
+ *   - Every pointer argument named *_dev is a device pointer (HIP global
+ *     memory).  Pointers to small result arrays (energies, spectra, fluxes)
+ *     are host pointers; those calls synchronise the plan's stream.
+ *   - Spectral fields: interleaved re/im FP64, shape (Nx, Ny, Nz/2+1) with
+ *     kz fastest (Blitz Array<Complex,3> layout, FFF_pencil.cc:209-213).
+ *   - Real fields: FP64, shape (Nx, Ny, Nz+2) with z fastest, last two
+ *     entries of each z-line padding (in-place r2c layout).
+ *   - A vector field is an array of 3 device pointers (V1, V2, V3).
+ *   - Return value: TG_OK (0) or a negative TG_ERR_* code; the message of
+ *     the last error on the calling thread is tg_last_error_string().
+ *     Nothing throws across this boundary.
+ *   - All work is enqueued on the plan's stream (tg_set_stream); one plan
+ *     per host thread; plans are independent of each other.
+ */
+#ifndef TARANG_GPU_H
+#define TARANG_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TG_OK 0
+#define TG_ERR_ARG (-1)
+#define TG_ERR_SHAPE (-2)
+#define TG_ERR_OOM (-3)
+#define TG_ERR_HIP (-4)
+#define TG_ERR_NAN (-5)
+#define TG_ERR_STATE (-6)
+
+enum { TG_BASIS_FFF = 0 };
+enum { TG_PREC_FP64 = 0, TG_PREC_FP32 = 1 };
+/* Dealias modes: reference corner box (FFF_pencil_basic.cc:87-90) or the
+ * two-thirds rule |kx|<=Nx/3, |ky|<=Ny/3, kz<=Nz/3. */
+enum { TG_DEALIAS_NONE = 0, TG_DEALIAS_REF_CORNER = 1, TG_DEALIAS_TWOTHIRDS = 2 };
+enum { TG_STEP_CONTINUE = 1 };
+
+typedef struct tg_config {
+    int basis;          /* TG_BASIS_FFF */
+    int nx, ny, nz;     /* grid; powers of two, nx,ny in [8,1024], nz in [16,1024] */
+    double kfactor[3];  /* 2 pi / L per axis (global.field.kfactor[1..3]) */
+    int precision;      /* TG_PREC_FP64 or TG_PREC_FP32 (transform/product precision) */
+    int dealias;        /* TG_DEALIAS_* (global.program.alias_option) */
+    int strong_reality; /* apply_strong_realitycond_alltime_switch */
+    int device;         /* HIP device ordinal */
+} tg_config;
+
+/* Fluid parameters (FluidVF dissipation_coefficient, fixed dt, forcing). */
+typedef struct tg_fluid {
+    double nu;
+    double dt;
+    int force_kind;     /* 0 none, 1 constant energy supply alpha-forcing */
+    double force_eps;   /* energy injection rate */
+    double force_r_in;  /* forcing band (r_in, r_out] */
+    double force_r_out;
+} tg_fluid;
+
+typedef struct tg_plan tg_plan;
+
+const char *tg_last_error_string(void);
+const char *tg_version(void);
+
+/* Replaces fftk.Init("FFF", Nx, Ny, Nz, num_p_rows, comm) (FFF_pencil.cc:91)
+ * and the FFF_PENCIL constructor's shape queries (:93-94). */
+int tg_plan_create(const tg_config *cfg, tg_plan **plan);
+int tg_plan_destroy(tg_plan *plan);
+int tg_set_stream(tg_plan *plan, void *hip_stream);
+int tg_synchronize(tg_plan *plan);
+/* complex_shape = (Nx, Ny, Nz/2+1), real_shape = (Nx, Ny, Nz+2)
+ * (fftk.Get_FA_shape / Get_RA_shape, FFF_pencil.cc:93-94). */
+int tg_get_shapes(const tg_plan *plan, int64_t complex_shape[3], int64_t real_shape[3]);
+/* Bytes of device memory held by the plan. */
+int64_t tg_plan_device_bytes(const tg_plan *plan);
+
+/* FFF_PENCIL::Forward_transform -> fftk.Forward_transform_3d("FFF", Ar, A)
+ * (FFF_pencil_tr.cc:43-51): r2c, scaled by 1/(Nx Ny Nz).  sincos_code is
+ * accepted for signature parity with the basis interface and must be 0. */
+int tg_forward(tg_plan *plan, int sincos_code, const double *real_dev, double *spec_dev);
+/* FFF_PENCIL::Inverse_transform (FFF_pencil_tr.cc:57-66): c2r, unscaled,
+ * input preserved. */
+int tg_inverse(tg_plan *plan, int sincos_code, const double *spec_dev, double *real_dev);
+
+/* FFF_PENCIL::Dealias (FFF_pencil_basic.cc:87-90) in place, per the plan's mode. */
+int tg_dealias(tg_plan *plan, double *const V_dev[], int ncomp);
+/* Satisfy_strong_reality_condition_in_Array (FFF_pencil_basic.cc:139-179). */
+int tg_strong_reality(tg_plan *plan, double *const V_dev[], int ncomp);
+
+/* Nlin_incompress::Compute_nlin(FluidVF&) (compute_nlin.cc:58-109):
+ * dealiases V in place, nlin_i = FT[d_j(u_j u_i)]. */
+int tg_compute_nlin(tg_plan *plan, double *const V_dev[3], double *const nlin_dev[3]);
+/* Time_advance_incompress::Compute_force_TO_rhs (Tadvance_functions.cc:441-453):
+ * nlin_i = -(N_i - F_i + i K_i p), p = i K.(N - F)/K^2. */
+int tg_compute_rhs(tg_plan *plan, double *const V_dev[3], const tg_fluid *fluid, double *const rhs_dev[3]);
+/* Nlin_incompress::Compute_nlin(FluidVF&, PlainFluidVF&) (compute_nlin_ET_VF.cc:71-87):
+ * dealiases V and W in place, nlin_i = FT[d_j(u_j w_i)]. */
+int tg_compute_nlin_giver(tg_plan *plan, double *const V_dev[3], double *const W_dev[3],
+                          double *const nlin_dev[3]);
+
+/* Time_advance_incompress::Time_advance_step with integration_scheme RK4
+ * (Time_advance.cc:43-65, RK.cc:91-170): nsteps Lawson RK4 steps of V in
+ * place.  flags & TG_STEP_CONTINUE reuses the stage-1 transform prepared by
+ * the previous call on the same V (valid unless V was modified outside
+ * this library since). */
+int tg_rk4_step(tg_plan *plan, double *const V_dev[3], const tg_fluid *fluid, int nsteps, int flags);
+
+/* Universal::Get_total_energy (universal_energy.cc:63-71) of one component:
+ * sum |A|^2 - sum_{kz=0} |A|^2 / 2. */
+int tg_energy(tg_plan *plan, const double *A_dev, double *energy);
+/* Universal::Get_total_Sn (universal_energy.cc:106-141): sum m K^n |A|^2. */
+int tg_Sn(tg_plan *plan, const double *A_dev, int n, double *Sn);
+/* Universal::Compute_shell_spectrum (universal_energy.cc:279-360), Stepanov
+ * normalisation; nbins >= Min_radius_outside()+1. */
+int tg_shell_spectrum(tg_plan *plan, const double *A_dev, int n, double *Sk, int nbins);
+/* EnergyTr::Compute_flux(FluidVF&) (compute_flux.cc:51-65): flux[0..nspheres],
+ * radii[0..nspheres].  Dealiases V in place. */
+int tg_flux(tg_plan *plan, double *const V_dev[3], const double *radii, int nspheres, double *flux);
+/* EnergyTr::Compute_shell_tr(FluidVF&) (compute_shell_tr.cc:57-75):
+ * S[(nshells+1)*(nshells+1)], row = giver shell.  Dealiases V in place. */
+int tg_shell_to_shell(tg_plan *plan, double *const V_dev[3], const double *radii, int nshells, double *S);
+
+/* Seeded solenoidal initial field (this library's own generator; the
+ * reference's Init_cond_energy_helicity_spectrum, init_cond_energy.cc:67-176,
+ * is seeded by time(0)). kind 0: E(K) ~ K^4 exp(-2 (K/k0)^2) with Gaussian
+ * amplitudes; kind 1: E(K) ~ K^(-5/3) on [kmin, kmax] with random phases. */
+int tg_init_field(tg_plan *plan, int kind, double k0, double kmin, double kmax, double energy, uint64_t seed,
+                  double *const V_dev[3]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

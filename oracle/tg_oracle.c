@@ -1,0 +1,935 @@
+/*
+ * tg_oracle.c -- FP64 CPU restatement of Tarang's pseudo-spectral hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the parity checker).  Loaded by tests/, by
+ * __graft_entry__.smoke() and by bench.py's cpu_baseline leg; never by the
+ * product library.
+ *
+ * Every function cites the reference file:line whose algorithm it restates.
+ * Paths are relative to the reference checkout (anandogc/tarang).
+ *
+ * Pinning of this restatement (checks live in tests/test_oracle.py):
+ *   or_fft1d, or_forward, or_inverse  -- numpy.fft golden fixtures
+ *       (tests/golden/fft_*.npz), brute-force DFT, single-mode inputs,
+ *       Parseval, round trip.
+ *   or_dealias                         -- mode-count closed forms for the
+ *       reference index ranges (FFF_pencil_basic.cc:87-90) and the 2/3 rule.
+ *   or_strong_reality                  -- Hermitian plane after fill; the
+ *       inverse of a filled field equals the inverse of its Hermitian part.
+ *   or_compute_nlin                    -- triad closed form, Taylor-Green
+ *       (u.grad)u at t=0, Beltrami P.N = 0, energy conservation (2/3 rule).
+ *   or_compute_nlin_giver              -- linearity in the giver; giver = u
+ *       reproduces or_compute_nlin.
+ *   or_compute_rhs                     -- K.RHS = 0, closed-form projection.
+ *   or_rk4_step / or_euler_step        -- N=0 gives exp(-nu K^2 t) exactly,
+ *       4th-order convergence in dt, stage formulas on a single mode.
+ *   or_energy, or_Sn                   -- Parseval against real space,
+ *       single-mode values.
+ *   or_shell_spectrum                  -- hand-computed 8^3 sums.
+ *   or_flux, or_shell_to_shell         -- antisymmetry, zero sums,
+ *       flux = sum of shell-to-shell blocks (2/3 rule, nu=0).
+ *   or_init_field                      -- divergence free, Hermitian,
+ *       exact total energy (this build's own seeded definition).
+ * The reference itself cannot be built here (MPI, FFTW, Blitz++, yaml-cpp,
+ * HDF5 and the external fftk transform library are all absent), so no
+ * vector from a running reference pins any of these.
+ */
+#include "tg_oracle.h"
+
+#include <complex.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef double complex cplx;
+
+/* ------------------------------------------------------------------------ */
+/* 1D mixed-radix FFT (recursive decimation in time, own implementation)     */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    int n, nf;
+    int f[64];
+    cplx *w; /* w[k] = exp(-2 pi i k / n) */
+} fftplan;
+
+static int plan_init(fftplan *p, int n)
+{
+    p->n = n;
+    p->nf = 0;
+    int m = n;
+    while (m % 4 == 0) { p->f[p->nf++] = 4; m /= 4; }
+    while (m % 2 == 0) { p->f[p->nf++] = 2; m /= 2; }
+    for (int q = 3; q * q <= m; q += 2)
+        while (m % q == 0) { p->f[p->nf++] = q; m /= q; }
+    if (m > 1) p->f[p->nf++] = m;
+    p->w = (cplx *)malloc(sizeof(cplx) * (size_t)n);
+    if (!p->w) return -1;
+    for (int k = 0; k < n; k++) {
+        double a = -2.0 * M_PI * (double)k / (double)n;
+        p->w[k] = cos(a) + I * sin(a);
+    }
+    return 0;
+}
+
+static void plan_free(fftplan *p) { free(p->w); p->w = NULL; }
+
+static void fft_rec(const fftplan *p, const cplx *in, int is, cplx *out, int n, int fi,
+                    int wstride, int sign)
+{
+    if (n == 1) { out[0] = in[0]; return; }
+    int r = p->f[fi];
+    int m = n / r;
+    for (int j = 0; j < r; j++)
+        fft_rec(p, in + (size_t)j * is, is * r, out + (size_t)j * m, m, fi + 1, wstride * r, sign);
+    cplx t[64];
+    cplx *tt = r <= 64 ? t : (cplx *)malloc(sizeof(cplx) * (size_t)r);
+    const int N = p->n;
+    for (int k = 0; k < m; k++) {
+        for (int j = 0; j < r; j++) {
+            long e = ((long)j * k * wstride) % N;
+            cplx w = p->w[e];
+            if (sign > 0) w = conj(w);
+            tt[j] = out[(size_t)j * m + k] * w;
+        }
+        for (int q = 0; q < r; q++) {
+            cplx s = 0;
+            for (int j = 0; j < r; j++) {
+                long e = (long)((j * q) % r) * (N / r);
+                cplx w = p->w[e];
+                if (sign > 0) w = conj(w);
+                s += tt[j] * w;
+            }
+            out[(size_t)q * m + k] = s;
+        }
+    }
+    if (tt != t) free(tt);
+}
+
+/* x <- DFT(x) with exponent sign*2*pi*i*j*k/n; work has n entries. */
+static void fft_exec(const fftplan *p, cplx *x, cplx *work, int sign)
+{
+    memcpy(work, x, sizeof(cplx) * (size_t)p->n);
+    fft_rec(p, work, 1, x, p->n, 0, 1, sign);
+}
+
+int or_fft1d(double *data, int n, int sign)
+{
+    if (n <= 0) return -1;
+    fftplan p;
+    if (plan_init(&p, n)) return -1;
+    cplx *work = (cplx *)malloc(sizeof(cplx) * (size_t)n);
+    fft_exec(&p, (cplx *)data, work, sign);
+    free(work);
+    plan_free(&p);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* 3D FFF transforms (fftk "FFF": FFF_pencil_tr.cc:43-66)                    */
+/* ------------------------------------------------------------------------ */
+
+static inline size_t cidx(const or_grid *g, int lx, int ly, int lz)
+{
+    return ((size_t)lx * g->ny + ly) * (size_t)(g->nz / 2 + 1) + lz;
+}
+
+/* complex-to-complex transforms along x and y of a (Nx,Ny,Nzc) array */
+static void fft_xy(const or_grid *g, cplx *A, int sign)
+{
+    const int nx = g->nx, ny = g->ny, nzc = g->nz / 2 + 1;
+    if (ny > 1) {
+#pragma omp parallel
+        {
+            fftplan p;
+            plan_init(&p, ny);
+            cplx *line = (cplx *)malloc(sizeof(cplx) * (size_t)ny);
+            cplx *work = (cplx *)malloc(sizeof(cplx) * (size_t)ny);
+#pragma omp for collapse(2) schedule(static)
+            for (int lx = 0; lx < nx; lx++)
+                for (int lz = 0; lz < nzc; lz++) {
+                    for (int ly = 0; ly < ny; ly++) line[ly] = A[cidx(g, lx, ly, lz)];
+                    fft_exec(&p, line, work, sign);
+                    for (int ly = 0; ly < ny; ly++) A[cidx(g, lx, ly, lz)] = line[ly];
+                }
+            free(line); free(work); plan_free(&p);
+        }
+    }
+    if (nx > 1) {
+#pragma omp parallel
+        {
+            fftplan p;
+            plan_init(&p, nx);
+            cplx *line = (cplx *)malloc(sizeof(cplx) * (size_t)nx);
+            cplx *work = (cplx *)malloc(sizeof(cplx) * (size_t)nx);
+#pragma omp for collapse(2) schedule(static)
+            for (int ly = 0; ly < ny; ly++)
+                for (int lz = 0; lz < nzc; lz++) {
+                    for (int lx = 0; lx < nx; lx++) line[lx] = A[cidx(g, lx, ly, lz)];
+                    fft_exec(&p, line, work, sign);
+                    for (int lx = 0; lx < nx; lx++) A[cidx(g, lx, ly, lz)] = line[lx];
+                }
+            free(line); free(work); plan_free(&p);
+        }
+    }
+}
+
+/* Forward_transform (FFF_pencil_tr.cc:43-51): r2c along z, then y, x; 1/N^3. */
+void or_forward(const or_grid *g, const double *real_in, double *spec_out)
+{
+    const int nx = g->nx, ny = g->ny, nz = g->nz, nzc = nz / 2 + 1;
+    cplx *A = (cplx *)spec_out;
+#pragma omp parallel
+    {
+        fftplan p;
+        plan_init(&p, nz);
+        cplx *line = (cplx *)malloc(sizeof(cplx) * (size_t)nz);
+        cplx *work = (cplx *)malloc(sizeof(cplx) * (size_t)nz);
+#pragma omp for collapse(2) schedule(static)
+        for (int lx = 0; lx < nx; lx++)
+            for (int ly = 0; ly < ny; ly++) {
+                const double *r = real_in + ((size_t)lx * ny + ly) * (size_t)(nz + 2);
+                for (int z = 0; z < nz; z++) line[z] = r[z];
+                fft_exec(&p, line, work, -1);
+                for (int kz = 0; kz < nzc; kz++) A[cidx(g, lx, ly, kz)] = line[kz];
+            }
+        free(line); free(work); plan_free(&p);
+    }
+    fft_xy(g, A, -1);
+    const double s = 1.0 / ((double)nx * (double)ny * (double)nz);
+    const size_t tot = (size_t)nx * ny * nzc;
+#pragma omp parallel for schedule(static)
+    for (size_t i = 0; i < tot; i++) A[i] *= s;
+}
+
+/* Inverse_transform (FFF_pencil_tr.cc:57-66): input preserved (the reference
+ * copies to X_transform first, :59); x, y inverse, then c2r along z. */
+void or_inverse(const or_grid *g, const double *spec_in, double *real_out)
+{
+    const int nx = g->nx, ny = g->ny, nz = g->nz, nzc = nz / 2 + 1;
+    const size_t tot = (size_t)nx * ny * nzc;
+    cplx *A = (cplx *)malloc(sizeof(cplx) * tot);
+    memcpy(A, spec_in, sizeof(cplx) * tot);
+    fft_xy(g, A, +1);
+#pragma omp parallel
+    {
+        fftplan p;
+        plan_init(&p, nz);
+        cplx *line = (cplx *)malloc(sizeof(cplx) * (size_t)nz);
+        cplx *work = (cplx *)malloc(sizeof(cplx) * (size_t)nz);
+#pragma omp for collapse(2) schedule(static)
+        for (int lx = 0; lx < nx; lx++)
+            for (int ly = 0; ly < ny; ly++) {
+                const cplx *c = A + cidx(g, lx, ly, 0);
+                line[0] = creal(c[0]);
+                for (int k = 1; k < nz / 2; k++) {
+                    line[k] = c[k];
+                    line[nz - k] = conj(c[k]);
+                }
+                line[nz / 2] = creal(c[nz / 2]);
+                fft_exec(&p, line, work, +1);
+                double *r = real_out + ((size_t)lx * ny + ly) * (size_t)(nz + 2);
+                for (int z = 0; z < nz; z++) r[z] = creal(line[z]);
+                r[nz] = 0.0;
+                r[nz + 1] = 0.0;
+            }
+        free(line); free(work); plan_free(&p);
+    }
+    free(A);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Basis helpers                                                            */
+/* ------------------------------------------------------------------------ */
+
+static inline int get_kx(const or_grid *g, int lx) { return lx <= g->nx / 2 ? lx : lx - g->nx; }
+static inline int get_ky(const or_grid *g, int ly) { return ly <= g->ny / 2 ? ly : ly - g->ny; }
+
+/* Kmagnitude, waveno_switch = true (FFF_pencil_inline.h:486-493) */
+double or_kmag(const or_grid *g, int lx, int ly, int lz)
+{
+    double a = get_kx(g, lx) * g->kfactor[0];
+    double b = get_ky(g, ly) * g->kfactor[1];
+    double c = lz * g->kfactor[2];
+    return sqrt(a * a + b * b + c * c);
+}
+
+static inline double ksqr(const or_grid *g, int lx, int ly, int lz)
+{
+    /* Array_divide_ksqr / Array_exp_ksqr order (universal_basic.cc:208-215) */
+    double kx = get_kx(g, lx) * g->kfactor[0];
+    double ky = get_ky(g, ly) * g->kfactor[1];
+    double kz = lz * g->kfactor[2];
+    double kxs = kx * kx;
+    double kxys = kxs + ky * ky;
+    return kxys + kz * kz;
+}
+
+/* Min_radius_outside (FFF_pencil_inline.h:497-504) */
+int or_min_radius_outside(const or_grid *g)
+{
+    double a = (g->nx / 2) * g->kfactor[0], b = (g->ny / 2) * g->kfactor[1], c = (g->nz / 2) * g->kfactor[2];
+    return (int)ceil(sqrt(a * a + b * b + c * c));
+}
+
+/* Max_radius_inside (FFF_pencil_inline.h:509-540), 3D branch */
+int or_max_radius_inside(const or_grid *g)
+{
+    double k = fmin((g->nx / 2) * g->kfactor[0], (g->ny / 2) * g->kfactor[1]);
+    k = fmin(k, (g->nz / 2) * g->kfactor[2]);
+    return (int)k;
+}
+
+/* Multiplicity_factor (FFF_pencil_inline.h:583-602) */
+double or_multiplicity(const or_grid *g, int lx, int ly, int lz)
+{
+    double f = lz > 0 ? 1.0 : 0.5;
+    int kx = get_kx(g, lx), ky = get_ky(g, ly);
+    if (kx == g->nx / 2 || (ky == g->ny / 2 && g->ny > 1)) return 2 * f;
+    return f;
+}
+
+static int dealias_zero(const or_grid *g, int lx, int ly, int lz)
+{
+    const int nx = g->nx, ny = g->ny, nz = g->nz;
+    if (g->dealias == OR_DEALIAS_REF_CORNER) {
+        /* FFF_PENCIL::Dealias (FFF_pencil_basic.cc:87-90): intersection of the
+         * index ranges lx in [Nx/3+1, 2Nx/3-1], ly in [Ny/3+1, 2Ny/3-1],
+         * lz in [Nz/3+1, Nz/2]. Assign_sub_array (:228-259) clamps an empty
+         * y/z range to its first index. */
+        int ylo = ny / 3 + 1, yhi = 2 * ny / 3 - 1;
+        if (yhi < ylo) yhi = ylo;
+        int zlo = nz / 3 + 1, zhi = nz / 2;
+        if (zhi < zlo) zhi = zlo;
+        return (lx >= nx / 3 + 1 && lx <= 2 * nx / 3 - 1) && (ly >= ylo && ly <= yhi) &&
+               (lz >= zlo && lz <= zhi);
+    }
+    if (g->dealias == OR_DEALIAS_TWOTHIRDS) {
+        int kx = abs(get_kx(g, lx)), ky = abs(get_ky(g, ly));
+        return kx > nx / 3 || ky > ny / 3 || lz > nz / 3;
+    }
+    return 0;
+}
+
+void or_dealias(const or_grid *g, double *spec)
+{
+    if (g->dealias == OR_DEALIAS_NONE) return;
+    cplx *A = (cplx *)spec;
+    const int nzc = g->nz / 2 + 1;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int lx = 0; lx < g->nx; lx++)
+        for (int ly = 0; ly < g->ny; ly++)
+            for (int lz = 0; lz < nzc; lz++)
+                if (dealias_zero(g, lx, ly, lz)) A[cidx(g, lx, ly, lz)] = 0;
+}
+
+/* Satisfy_strong_reality_condition_in_Array (FFF_pencil_basic.cc:139-179),
+ * single-process form (num_y_procs == 1). */
+void or_strong_reality(const or_grid *g, double *spec)
+{
+    cplx *A = (cplx *)spec;
+    const int nx = g->nx, ny = g->ny, nzc = g->nz / 2 + 1;
+    cplx *plane = (cplx *)malloc(sizeof(cplx) * (size_t)nx * ny);
+    for (int lx = 0; lx < nx; lx++)
+        for (int ly = 0; ly < ny; ly++) plane[(size_t)lx * ny + ly] = A[cidx(g, lx, ly, 0)];
+    for (int lx = nx / 2 + 1; lx < nx; lx++) {
+        int mkx = -get_kx(g, lx);
+        for (int ly = 0; ly < ny; ly++) {
+            if (get_ky(g, ly) != ny / 2) {
+                int mky = -get_ky(g, ly);
+                int iy = mky >= 0 ? mky : mky + ny;
+                A[cidx(g, lx, ly, 0)] = conj(plane[(size_t)mkx * ny + iy]);
+            }
+        }
+    }
+    for (int ly = 0; ly < ny; ly++) {
+        if (get_ky(g, ly) < 0) {
+            int mky = -get_ky(g, ly);
+            A[cidx(g, 0, ly, 0)] = conj(plane[(size_t)0 * ny + mky]);
+        }
+    }
+    free(plane);
+    for (int lx = 0; lx < nx; lx++)
+        for (int ly = 0; ly < ny; ly++) A[cidx(g, lx, ly, nzc - 1)] = 0;
+}
+
+/* Satisfy_weak_reality_condition_in_Array (FFF_pencil_basic.cc:181-186) */
+void or_weak_reality(const or_grid *g, double *spec)
+{
+    cplx *A = (cplx *)spec;
+    const int nzc = g->nz / 2 + 1;
+    for (int lx = 0; lx < g->nx; lx++)
+        for (int ly = 0; ly < g->ny; ly++) A[cidx(g, lx, ly, nzc - 1)] = 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Nonlinear term                                                           */
+/* ------------------------------------------------------------------------ */
+
+static size_t nreal(const or_grid *g) { return (size_t)g->nx * g->ny * (size_t)(g->nz + 2); }
+static size_t ncplx(const or_grid *g) { return (size_t)g->nx * g->ny * (size_t)(g->nz / 2 + 1); }
+
+/* ArrayOps::Real_space_multiply (ArrayOps.cc:49-63): C = A*B */
+static void real_mult(const or_grid *g, const double *a, const double *b, double *c)
+{
+    const size_t n = nreal(g);
+#pragma omp parallel for schedule(static)
+    for (size_t i = 0; i < n; i++) c[i] = a[i] * b[i];
+}
+
+/* B (op)= i K_d A for axis d (FFF_pencil_tr.cc:70-145); add != 0 adds. */
+static void deriv(const or_grid *g, int d, const cplx *A, cplx *B, int add)
+{
+    const int nx = g->nx, ny = g->ny, nzc = g->nz / 2 + 1;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int lx = 0; lx < nx; lx++)
+        for (int ly = 0; ly < ny; ly++)
+            for (int lz = 0; lz < nzc; lz++) {
+                double K = d == 0 ? get_kx(g, lx) * g->kfactor[0]
+                         : d == 1 ? get_ky(g, ly) * g->kfactor[1]
+                                  : lz * g->kfactor[2];
+                size_t i = cidx(g, lx, ly, lz);
+                cplx v = (I * K) * A[i];
+                if (add) B[i] += v; else B[i] = v;
+            }
+}
+
+/* Nlin_incompress::Compute_nlin(FluidVF&) (compute_nlin.cc:58-109) with
+ * Compute_nlin_diag (compute_diag_offdiag.cc:47-65) and Compute_nlin_offdiag
+ * (:68-112).  N_i = sum_j i K_j FT[u_j u_i].  Dealiases V in place (:61-62). */
+void or_compute_nlin(const or_grid *g, double *V[3], double *N[3])
+{
+    const size_t nr = nreal(g);
+    for (int c = 0; c < 3; c++) or_dealias(g, V[c]);
+    double *u[3], *p = (double *)malloc(sizeof(double) * nr);
+    for (int c = 0; c < 3; c++) {
+        u[c] = (double *)malloc(sizeof(double) * nr);
+        or_inverse(g, V[c], u[c]);
+    }
+    cplx *X = (cplx *)malloc(sizeof(cplx) * ncplx(g));
+    cplx *n1 = (cplx *)N[0], *n2 = (cplx *)N[1], *n3 = (cplx *)N[2];
+    /* diag */
+    real_mult(g, u[0], u[0], p); or_forward(g, p, N[0]); deriv(g, 0, n1, n1, 0);
+    real_mult(g, u[1], u[1], p); or_forward(g, p, N[1]); deriv(g, 1, n2, n2, 0);
+    real_mult(g, u[2], u[2], p); or_forward(g, p, N[2]); deriv(g, 2, n3, n3, 0);
+    /* offdiag */
+    real_mult(g, u[0], u[1], p); or_forward(g, p, (double *)X);
+    deriv(g, 1, X, n1, 1); deriv(g, 0, X, n2, 1);
+    real_mult(g, u[1], u[2], p); or_forward(g, p, (double *)X);
+    deriv(g, 2, X, n2, 1); deriv(g, 1, X, n3, 1);
+    real_mult(g, u[2], u[0], p); or_forward(g, p, (double *)X);
+    deriv(g, 2, X, n1, 1); deriv(g, 0, X, n3, 1);
+    free(X); free(p);
+    for (int c = 0; c < 3; c++) free(u[c]);
+}
+
+/* Compute_nlin(FluidVF& U, PlainFluidVF& W) (compute_nlin_ET_VF.cc:71-87) with
+ * the (U,Plain) diag/offdiag (compute_diag_offdiag.cc:329-414):
+ * N_i = sum_j i K_j FT[u_j w_i].  Dealiases V and W in place. */
+void or_compute_nlin_giver(const or_grid *g, double *V[3], double *W[3], double *N[3])
+{
+    const size_t nr = nreal(g);
+    for (int c = 0; c < 3; c++) { or_dealias(g, V[c]); or_dealias(g, W[c]); }
+    double *u[3], *w[3], *p = (double *)malloc(sizeof(double) * nr);
+    for (int c = 0; c < 3; c++) {
+        u[c] = (double *)malloc(sizeof(double) * nr);
+        w[c] = (double *)malloc(sizeof(double) * nr);
+        or_inverse(g, V[c], u[c]);
+        or_inverse(g, W[c], w[c]);
+    }
+    cplx *X = (cplx *)malloc(sizeof(cplx) * ncplx(g));
+    cplx *n1 = (cplx *)N[0], *n2 = (cplx *)N[1], *n3 = (cplx *)N[2];
+    real_mult(g, u[0], w[0], p); or_forward(g, p, N[0]); deriv(g, 0, n1, n1, 0);
+    real_mult(g, u[1], w[1], p); or_forward(g, p, N[1]); deriv(g, 1, n2, n2, 0);
+    real_mult(g, u[2], w[2], p); or_forward(g, p, N[2]); deriv(g, 2, n3, n3, 0);
+    real_mult(g, u[0], w[1], p); or_forward(g, p, (double *)X); deriv(g, 0, X, n2, 1);
+    real_mult(g, w[0], u[1], p); or_forward(g, p, (double *)X); deriv(g, 1, X, n1, 1);
+    real_mult(g, u[1], w[2], p); or_forward(g, p, (double *)X); deriv(g, 1, X, n3, 1);
+    real_mult(g, w[1], u[2], p); or_forward(g, p, (double *)X); deriv(g, 2, X, n2, 1);
+    real_mult(g, u[2], w[0], p); or_forward(g, p, (double *)X); deriv(g, 2, X, n1, 1);
+    real_mult(g, w[2], u[0], p); or_forward(g, p, (double *)X); deriv(g, 0, X, n3, 1);
+    free(X); free(p);
+    for (int c = 0; c < 3; c++) { free(u[c]); free(w[c]); }
+}
+
+/* Constant energy supply alpha-forcing F = alpha V on the band (r_in, r_out]
+ * with alpha = eps / (2 E_band) -- the helicity-free limit of
+ * FORCE::Compute_force_Carati_scheme_energy_supply (compute_force_Carati.cc:
+ * 62-149) with one supply shell and Const_energy_supply_alpha_beta
+ * (force_alpha_beta.cc:56-94, beta = 0).  E_band uses the energy weights of
+ * Get_local_energy (FFF_pencil_energy.cc:59-67). */
+void or_compute_force(const or_grid *g, const or_fluid *f, double *const V[3], double *F[3])
+{
+    const size_t nc = ncplx(g);
+    for (int c = 0; c < 3; c++) memset(F[c], 0, sizeof(cplx) * nc);
+    if (f->force_kind == 0) return;
+    const int nzc = g->nz / 2 + 1;
+    double eband = 0.0;
+    for (int lx = 0; lx < g->nx; lx++)
+        for (int ly = 0; ly < g->ny; ly++)
+            for (int lz = 0; lz < nzc; lz++) {
+                double K = or_kmag(g, lx, ly, lz);
+                if (K > f->force_r_in && K <= f->force_r_out) {
+                    double w = lz == 0 ? 0.5 : 1.0;
+                    size_t i = cidx(g, lx, ly, lz);
+                    for (int c = 0; c < 3; c++) {
+                        cplx v = ((cplx *)V[c])[i];
+                        eband += w * (creal(v) * creal(v) + cimag(v) * cimag(v));
+                    }
+                }
+            }
+    if (!(eband > 0.0)) return;
+    double alpha = f->force_eps / (2.0 * eband);
+    for (int lx = 0; lx < g->nx; lx++)
+        for (int ly = 0; ly < g->ny; ly++)
+            for (int lz = 0; lz < nzc; lz++) {
+                double K = or_kmag(g, lx, ly, lz);
+                if (K > f->force_r_in && K <= f->force_r_out) {
+                    size_t i = cidx(g, lx, ly, lz);
+                    for (int c = 0; c < 3; c++) ((cplx *)F[c])[i] = alpha * ((cplx *)V[c])[i];
+                }
+            }
+}
+
+/* Add_force (Tadvance_functions.cc:45-53), Pressure::Compute_pressure
+ * (Pressure.cc:56-72: divergence universal_basic.cc:343-363, divide_ksqr
+ * :202-225), Compute_rhs (Tadvance_functions.cc:148-191):
+ * N <- -(N - F + i K p), p = i K.(N - F) / K^2, p(0) = 0. */
+void or_compute_rhs(const or_grid *g, double *N[3], double *const F[3])
+{
+    const int nx = g->nx, ny = g->ny, nzc = g->nz / 2 + 1;
+    cplx *n1 = (cplx *)N[0], *n2 = (cplx *)N[1], *n3 = (cplx *)N[2];
+    const cplx *f1 = F ? (const cplx *)F[0] : NULL, *f2 = F ? (const cplx *)F[1] : NULL,
+               *f3 = F ? (const cplx *)F[2] : NULL;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int lx = 0; lx < nx; lx++)
+        for (int ly = 0; ly < ny; ly++)
+            for (int lz = 0; lz < nzc; lz++) {
+                size_t i = cidx(g, lx, ly, lz);
+                double kx = get_kx(g, lx) * g->kfactor[0];
+                double ky = get_ky(g, ly) * g->kfactor[1];
+                double kz = lz * g->kfactor[2];
+                cplx a = n1[i], b = n2[i], c = n3[i];
+                if (f1) { a -= f1[i]; b -= f2[i]; c -= f3[i]; }
+                cplx div = (I * kx) * a;
+                div += (I * ky) * b;
+                div += (I * kz) * c;
+                double ks = ksqr(g, lx, ly, lz);
+                cplx p = (lx == 0 && ly == 0 && lz == 0) ? 0 : div / ks;
+                a += (I * kx) * p;
+                b += (I * ky) * p;
+                c += (I * kz) * p;
+                n1[i] = -a; n2[i] = -b; n3[i] = -c;
+            }
+}
+
+/* Array_exp_ksqr (universal_basic.cc:235-255): A *= exp(factor*K^2) */
+static void exp_ksqr(const or_grid *g, double *spec, double factor)
+{
+    cplx *A = (cplx *)spec;
+    const int nx = g->nx, ny = g->ny, nzc = g->nz / 2 + 1;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int lx = 0; lx < nx; lx++)
+        for (int ly = 0; ly < ny; ly++)
+            for (int lz = 0; lz < nzc; lz++) A[cidx(g, lx, ly, lz)] *= exp(factor * ksqr(g, lx, ly, lz));
+}
+
+/* FluidVF::Mult_field_exp_ksqr_dt / Mult_nlin_exp_ksqr_dt (FluidVF.cc:339-389) */
+static void mult_exp(const or_grid *g, const or_fluid *f, double *A[3], double a)
+{
+    if (fabs(a) > 1e-15)
+        for (int c = 0; c < 3; c++) exp_ksqr(g, A[c], -f->nu * a * f->dt);
+}
+
+/* FluidVF::Add_nlin_factor_dt (FluidVF.cc:308-316): Y += (factor*dt)*N */
+static void add_factor(const or_grid *g, double *Y[3], double *const N[3], double factor, double dt)
+{
+    if (fabs(factor) <= 1e-5) return;
+    const size_t nc = ncplx(g);
+    const double s = factor * dt;
+    for (int c = 0; c < 3; c++) {
+        cplx *y = (cplx *)Y[c];
+        const cplx *n = (const cplx *)N[c];
+#pragma omp parallel for schedule(static)
+        for (size_t i = 0; i < nc; i++) y[i] += s * n[i];
+    }
+}
+
+/* Compute_force_TO_rhs (Tadvance_functions.cc:441-453) */
+static void force_to_rhs(const or_grid *g, const or_fluid *f, double *V[3], double *N[3], double *F[3])
+{
+    or_compute_force(g, f, V, F);
+    or_compute_nlin(g, V, N);
+    or_compute_rhs(g, N, F);
+}
+
+/* Single_time_step (Tadvance_functions.cc:298-324) */
+static void single_time_step(const or_grid *g, const or_fluid *f, double *V[3], double *N[3],
+                             double a, double b, double c)
+{
+    const double eps = 1e-15;
+    if (fabs(a) < eps && fabs(b) < eps) {
+        add_factor(g, V, N, c, f->dt);
+    } else if (fabs(a) < eps && fabs(b) > eps) {
+        mult_exp(g, f, N, b);
+        add_factor(g, V, N, c, f->dt);
+    } else if (fabs(b) < eps && fabs(a) > eps) {
+        mult_exp(g, f, V, a);
+        add_factor(g, V, N, c, f->dt);
+    } else if (fabs(a - b) < eps && fabs(a) > eps) {
+        add_factor(g, V, N, c, f->dt);
+        mult_exp(g, f, V, a);
+    } else if (fabs(a) > eps && fabs(b) > eps) {
+        mult_exp(g, f, V, a - b);
+        add_factor(g, V, N, c, f->dt);
+        mult_exp(g, f, V, b);
+    }
+}
+
+/* Compute_RK4_parts (Tadvance_functions.cc:566-570) */
+static void rk4_parts(const or_grid *g, const or_fluid *f, double *N[3], double *acc[3], double b,
+                      double factor)
+{
+    mult_exp(g, f, N, b);
+    add_factor(g, acc, N, factor, f->dt);
+}
+
+/* Time_advance_incompress::RK4 (RK.cc:91-170, LES branch omitted) followed
+ * by the strong reality condition (Time_advance.cc:60-61). */
+void or_rk4_step(const or_grid *g, const or_fluid *f, double *V[3])
+{
+    const size_t nc = ncplx(g);
+    double *V0[3], *acc[3], *N[3], *F[3];
+    for (int c = 0; c < 3; c++) {
+        V0[c] = (double *)malloc(sizeof(cplx) * nc);
+        acc[c] = (double *)calloc(nc, sizeof(cplx));
+        N[c] = (double *)malloc(sizeof(cplx) * nc);
+        F[c] = (double *)malloc(sizeof(cplx) * nc);
+        memcpy(V0[c], V[c], sizeof(cplx) * nc);
+    }
+    /* step 1 */
+    force_to_rhs(g, f, V, N, F);
+    single_time_step(g, f, V, N, 0.5, 0.5, 0.5);
+    rk4_parts(g, f, N, acc, 1.0, 1.0 / 6);
+    /* step 2 */
+    force_to_rhs(g, f, V, N, F);
+    for (int c = 0; c < 3; c++) memcpy(V[c], V0[c], sizeof(cplx) * nc);
+    single_time_step(g, f, V, N, 0.5, 0, 0.5);
+    rk4_parts(g, f, N, acc, 0.5, 1.0 / 3);
+    /* step 3 */
+    force_to_rhs(g, f, V, N, F);
+    for (int c = 0; c < 3; c++) memcpy(V[c], V0[c], sizeof(cplx) * nc);
+    single_time_step(g, f, V, N, 1, 0.5, 1);
+    rk4_parts(g, f, N, acc, 0.5, 1.0 / 3);
+    /* step 4 */
+    force_to_rhs(g, f, V, N, F);
+    rk4_parts(g, f, N, acc, 0, 1.0 / 6);
+    /* final */
+    for (int c = 0; c < 3; c++) memcpy(V[c], V0[c], sizeof(cplx) * nc);
+    mult_exp(g, f, V, 1.0);
+    for (int c = 0; c < 3; c++) {
+        cplx *v = (cplx *)V[c];
+        const cplx *a = (const cplx *)acc[c];
+#pragma omp parallel for schedule(static)
+        for (size_t i = 0; i < nc; i++) v[i] = v[i] + a[i];
+    }
+    if (g->strong_reality)
+        for (int c = 0; c < 3; c++) or_strong_reality(g, V[c]);
+    for (int c = 0; c < 3; c++) { free(V0[c]); free(acc[c]); free(N[c]); free(F[c]); }
+}
+
+/* Euler (RK.cc:61-66) */
+void or_euler_step(const or_grid *g, const or_fluid *f, double *V[3])
+{
+    const size_t nc = ncplx(g);
+    double *N[3], *F[3];
+    for (int c = 0; c < 3; c++) {
+        N[c] = (double *)malloc(sizeof(cplx) * nc);
+        F[c] = (double *)malloc(sizeof(cplx) * nc);
+    }
+    force_to_rhs(g, f, V, N, F);
+    single_time_step(g, f, V, N, 1, 1, 1);
+    if (g->strong_reality)
+        for (int c = 0; c < 3; c++) or_strong_reality(g, V[c]);
+    for (int c = 0; c < 3; c++) { free(N[c]); free(F[c]); }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Diagnostics                                                              */
+/* ------------------------------------------------------------------------ */
+
+/* Get_local_energy (FFF_pencil_energy.cc:59-67): sum|A|^2 - sum_{kz=0}|A|^2/2 */
+double or_energy(const or_grid *g, const double *spec)
+{
+    const cplx *A = (const cplx *)spec;
+    const size_t nc = ncplx(g);
+    const int nzc = g->nz / 2 + 1;
+    double s = 0.0, s0 = 0.0;
+    for (size_t i = 0; i < nc; i++) s += creal(A[i]) * creal(A[i]) + cimag(A[i]) * cimag(A[i]);
+    for (size_t i = 0; i < nc; i += nzc) s0 += creal(A[i]) * creal(A[i]) + cimag(A[i]) * cimag(A[i]);
+    return s - s0 / 2;
+}
+
+/* Get_local_Sn (universal_energy.cc:106-130) */
+double or_Sn(const or_grid *g, const double *spec, int n)
+{
+    const cplx *A = (const cplx *)spec;
+    const int nzc = g->nz / 2 + 1;
+    const int kmax = or_min_radius_outside(g);
+    double s = 0.0;
+    for (int lx = 0; lx < g->nx; lx++)
+        for (int ly = 0; ly < g->ny; ly++)
+            for (int lz = 0; lz < nzc; lz++) {
+                double K = or_kmag(g, lx, ly, lz);
+                if (K <= kmax) {
+                    cplx a = A[cidx(g, lx, ly, lz)];
+                    s += or_multiplicity(g, lx, ly, lz) * pow(K, n) * (creal(a) * creal(a) + cimag(a) * cimag(a));
+                }
+            }
+    if (n == 0) s -= or_multiplicity(g, 0, 0, 0) * (creal(A[0]) * creal(A[0]) + cimag(A[0]) * cimag(A[0]));
+    return s;
+}
+
+/* Compute_local_shell_spectrum + Compute_shell_spectrum (universal_energy.cc:
+ * 279-360), 3D Stepanov normalisation.  Returns the number of bins written
+ * (Min_radius_outside()+1). */
+int or_shell_spectrum(const or_grid *g, const double *spec, int n, double *Sk, int nbins)
+{
+    const cplx *A = (const cplx *)spec;
+    const int nzc = g->nz / 2 + 1;
+    const int kmax = or_min_radius_outside(g);
+    if (nbins < kmax + 1) return -1;
+    double *cnt = (double *)calloc((size_t)nbins, sizeof(double));
+    for (int i = 0; i < nbins; i++) Sk[i] = 0.0;
+    for (int lx = 0; lx < g->nx; lx++)
+        for (int ly = 0; ly < g->ny; ly++)
+            for (int lz = 0; lz < nzc; lz++) {
+                double K = or_kmag(g, lx, ly, lz);
+                int idx = (int)ceil(K);
+                if (idx <= kmax) {
+                    double m = or_multiplicity(g, lx, ly, lz);
+                    cplx a = A[cidx(g, lx, ly, lz)];
+                    Sk[idx] += m * pow(K, n) * (creal(a) * creal(a) + cimag(a) * cimag(a)) * (K * K);
+                    cnt[idx] += 2 * m;
+                }
+            }
+    const double kf = g->kfactor[0] * g->kfactor[1] * g->kfactor[2];
+    for (int i = 0; i <= kmax; i++) Sk[i] = cnt[i] <= 0 ? 0.0 : Sk[i] * (4 * M_PI) / (cnt[i] * kf);
+    free(cnt);
+    return kmax + 1;
+}
+
+/* Fill_array_shell (universal_ET.cc:880-892) */
+void or_fill_array_shell(const or_grid *g, const double *spec, double *out, double r_in, double r_out)
+{
+    const cplx *A = (const cplx *)spec;
+    cplx *B = (cplx *)out;
+    const int nzc = g->nz / 2 + 1;
+    for (int lx = 0; lx < g->nx; lx++)
+        for (int ly = 0; ly < g->ny; ly++)
+            for (int lz = 0; lz < nzc; lz++) {
+                double K = or_kmag(g, lx, ly, lz);
+                size_t i = cidx(g, lx, ly, lz);
+                B[i] = (K > r_in && K <= r_out) ? A[i] : 0;
+            }
+}
+
+/* Local_shell_mult_single (universal_ET.cc:51-71) */
+double or_shell_mult_single(const or_grid *g, const double *a, const double *b, double r_in, double r_out)
+{
+    const cplx *A = (const cplx *)a, *B = (const cplx *)b;
+    const int nzc = g->nz / 2 + 1;
+    double s = 0.0;
+    for (int lx = 0; lx < g->nx; lx++)
+        for (int ly = 0; ly < g->ny; ly++)
+            for (int lz = 0; lz < nzc; lz++) {
+                double K = or_kmag(g, lx, ly, lz);
+                if (K > r_in && K <= r_out) {
+                    size_t i = cidx(g, lx, ly, lz);
+                    s += 2 * or_multiplicity(g, lx, ly, lz) * creal(A[i] * conj(B[i]));
+                }
+            }
+    return s;
+}
+
+/* Get_shell_index (shell_etc_indices.cc:49-52): first i with radii[i] >= K */
+static int shell_index(double K, const double *radii, int nradii)
+{
+    for (int i = 0; i < nradii; i++)
+        if (radii[i] >= K) return i;
+    return nradii - 1;
+}
+
+/* Local_shell_mult_all, vector form (universal_ET.cc:145-173); result has
+ * nradii entries, receivers with K > Max_radius_inside dropped (:159-166). */
+void or_shell_mult_all(const or_grid *g, double *const A[3], double *const B[3], const double *radii,
+                       int nradii, double *result)
+{
+    const int nzc = g->nz / 2 + 1;
+    const int kin = or_max_radius_inside(g);
+    for (int i = 0; i < nradii; i++) result[i] = 0.0;
+    for (int lx = 0; lx < g->nx; lx++)
+        for (int ly = 0; ly < g->ny; ly++)
+            for (int lz = 0; lz < nzc; lz++) {
+                double K = or_kmag(g, lx, ly, lz);
+                if (K <= kin) {
+                    int s = shell_index(K, radii, nradii);
+                    size_t i = cidx(g, lx, ly, lz);
+                    double d = 0.0;
+                    for (int c = 0; c < 3; c++) {
+                        cplx x = ((const cplx *)A[c])[i], y = ((const cplx *)B[c])[i];
+                        d += creal(x) * creal(y) + cimag(x) * cimag(y);
+                    }
+                    result[s] += 2 * or_multiplicity(g, lx, ly, lz) * d;
+                }
+            }
+}
+
+/* Default log-spaced radii (Global_process_advanced_vars.cc:137-175): n+1
+ * entries, radii[n] = infinity. */
+void or_default_radii(const or_grid *g, int n, double *radii)
+{
+    int mri = or_max_radius_inside(g);
+    for (int i = 0; i <= n; i++) radii[i] = 0.0;
+    radii[0] = 0.0;
+    if (n >= 1) radii[1] = 2.0;
+    if (n >= 2) radii[2] = 4.0;
+    if (n >= 3) radii[3] = 8.0;
+    if (n >= 2) radii[n - 2] = mri / 2.0;
+    if (n >= 1) radii[n - 1] = mri;
+    radii[n] = INFINITY;
+    if (n > 6) {
+        double s = log2(mri / 16.0) / (n - 5);
+        for (int i = 4; i <= n - 3; i++) radii[i] = 8 * pow(2, s * (i - 3));
+    }
+}
+
+/* EnergyTr::Compute_flux(FluidVF&) (compute_flux.cc:51-65) with Fill_in_sphere
+ * (fill_space.cc:56-64) and Prod_out_sphere_nlinV (prod_nlinV.cc:61-69).
+ * flux has nspheres+1 entries; flux[0] = 0.  radii has nspheres+1 entries. */
+void or_flux(const or_grid *g, double *V[3], const double *radii, int nspheres, double *flux)
+{
+    const size_t nc = ncplx(g);
+    double *W[3], *N[3];
+    for (int c = 0; c < 3; c++) {
+        W[c] = (double *)malloc(sizeof(cplx) * nc);
+        N[c] = (double *)malloc(sizeof(cplx) * nc);
+    }
+    flux[0] = 0.0;
+    for (int s = 1; s <= nspheres; s++) {
+        for (int c = 0; c < 3; c++) or_fill_array_shell(g, V[c], W[c], 0.0, radii[s]);
+        or_compute_nlin_giver(g, V, W, N);
+        double p = 0.0;
+        for (int c = 0; c < 3; c++) p += or_shell_mult_single(g, N[c], V[c], radii[s], INFINITY);
+        flux[s] = -p;
+    }
+    for (int c = 0; c < 3; c++) { free(W[c]); free(N[c]); }
+}
+
+/* EnergyTr::Compute_shell_tr(FluidVF&) (compute_shell_tr.cc:57-75) with
+ * Fill_shell (fill_space.cc:150).  S is (nshells+1)x(nshells+1) row-major,
+ * row = giver shell m (1..nshells-1), column = receiver shell n. */
+void or_shell_to_shell(const or_grid *g, double *V[3], const double *radii, int nshells, double *S)
+{
+    const size_t nc = ncplx(g);
+    const int nr = nshells + 1;
+    double *W[3], *N[3];
+    double *row = (double *)malloc(sizeof(double) * (size_t)nr);
+    for (int c = 0; c < 3; c++) {
+        W[c] = (double *)malloc(sizeof(cplx) * nc);
+        N[c] = (double *)malloc(sizeof(cplx) * nc);
+    }
+    for (int i = 0; i < nr * nr; i++) S[i] = 0.0;
+    for (int m = 1; m < nshells; m++) {
+        for (int c = 0; c < 3; c++) or_fill_array_shell(g, V[c], W[c], radii[m - 1], radii[m]);
+        or_compute_nlin_giver(g, V, W, N);
+        or_shell_mult_all(g, N, V, radii, nr, row);
+        for (int n = 0; n < nr; n++) S[(size_t)m * nr + n] = -row[n];
+    }
+    for (int c = 0; c < 3; c++) { free(W[c]); free(N[c]); }
+    free(row);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Seeded initial condition (this build's own definition; the reference's  */
+/* generator is seeded by time(0), init_cond_energy.cc:76)                   */
+/* ------------------------------------------------------------------------ */
+
+static inline uint64_t splitmix64(uint64_t x)
+{
+    x += 0x9E3779B97F4A7C15ULL;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBULL;
+    return x ^ (x >> 31);
+}
+
+static inline double uniform01(uint64_t seed, uint64_t ctr)
+{
+    uint64_t h = splitmix64(splitmix64(seed) ^ ctr);
+    return (double)(h >> 11) * (1.0 / 9007199254740992.0);
+}
+
+/* spectrum_kind 0: E(K) = K^4 exp(-2 (K/k0)^2), Gaussian complex amplitudes on
+ * the two Craya directions; spectrum_kind 1: E(K) = K^(-5/3) for
+ * kmin <= K <= kmax, unit-modulus random phases on the two Craya directions.
+ * |u(k)|^2 = E(K)/(2 pi K^2) before the total is rescaled to `energy`
+ * (measured with or_energy).  K=0 and the Nyquist planes kx=Nx/2, ky=Ny/2,
+ * kz=Nz/2 are zero; the kz=0 plane is made Hermitian with the strong
+ * reality fill. */
+void or_init_field(const or_grid *g, int spectrum_kind, double k0, double kmin, double kmax,
+                   double energy, uint64_t seed, double *V[3])
+{
+    const int nx = g->nx, ny = g->ny, nz = g->nz, nzc = nz / 2 + 1;
+    cplx *v1 = (cplx *)V[0], *v2 = (cplx *)V[1], *v3 = (cplx *)V[2];
+    for (int lx = 0; lx < nx; lx++)
+        for (int ly = 0; ly < ny; ly++)
+            for (int lz = 0; lz < nzc; lz++) {
+                size_t i = cidx(g, lx, ly, lz);
+                int kx = get_kx(g, lx), ky = get_ky(g, ly);
+                double Kx = kx * g->kfactor[0], Ky = ky * g->kfactor[1], Kz = lz * g->kfactor[2];
+                double K = sqrt(Kx * Kx + Ky * Ky + Kz * Kz);
+                double Ek = 0.0;
+                if (K > 0.0) {
+                    if (spectrum_kind == 0) Ek = pow(K, 4) * exp(-2.0 * (K / k0) * (K / k0));
+                    else if (K >= kmin && K <= kmax) Ek = pow(K, -5.0 / 3.0);
+                }
+                if (kx == nx / 2 || ky == ny / 2 || lz == nz / 2) Ek = 0.0;
+                if (Ek == 0.0) { v1[i] = 0; v2[i] = 0; v3[i] = 0; continue; }
+                double amp = sqrt(Ek / (2.0 * M_PI * K * K));
+                /* Craya basis: e1 = K x z / |K x z| (x if K || z), e2 = K x e1 / |K| */
+                double kp = sqrt(Kx * Kx + Ky * Ky);
+                double e1[3], e2[3];
+                if (kp > 0.0) { e1[0] = Ky / kp; e1[1] = -Kx / kp; e1[2] = 0.0; }
+                else { e1[0] = 1.0; e1[1] = 0.0; e1[2] = 0.0; }
+                e2[0] = (Ky * e1[2] - Kz * e1[1]) / K;
+                e2[1] = (Kz * e1[0] - Kx * e1[2]) / K;
+                e2[2] = (Kx * e1[1] - Ky * e1[0]) / K;
+                uint64_t base = (uint64_t)i * 4u;
+                cplx a1, a2;
+                if (spectrum_kind == 0) {
+                    double u1 = 1.0 - uniform01(seed, base + 0), u2 = uniform01(seed, base + 1);
+                    double u3 = 1.0 - uniform01(seed, base + 2), u4 = uniform01(seed, base + 3);
+                    double r1 = sqrt(-2.0 * log(u1)), r2 = sqrt(-2.0 * log(u3));
+                    a1 = r1 * (cos(2 * M_PI * u2) + I * sin(2 * M_PI * u2)) / 2.0;
+                    a2 = r2 * (cos(2 * M_PI * u4) + I * sin(2 * M_PI * u4)) / 2.0;
+                } else {
+                    double p1 = uniform01(seed, base + 0), p2 = uniform01(seed, base + 1);
+                    a1 = (cos(2 * M_PI * p1) + I * sin(2 * M_PI * p1)) / sqrt(2.0);
+                    a2 = (cos(2 * M_PI * p2) + I * sin(2 * M_PI * p2)) / sqrt(2.0);
+                }
+                v1[i] = amp * (a1 * e1[0] + a2 * e2[0]);
+                v2[i] = amp * (a1 * e1[1] + a2 * e2[1]);
+                v3[i] = amp * (a1 * e1[2] + a2 * e2[2]);
+            }
+    for (int c = 0; c < 3; c++) or_strong_reality(g, V[c]);
+    double e = or_energy(g, V[0]) + or_energy(g, V[1]) + or_energy(g, V[2]);
+    if (e > 0.0) {
+        double s = sqrt(energy / e);
+        const size_t nc = ncplx(g);
+        for (int c = 0; c < 3; c++)
+            for (size_t i = 0; i < nc; i++) ((cplx *)V[c])[i] *= s;
+    }
+}

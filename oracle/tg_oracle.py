@@ -1,0 +1,227 @@
+"""ctypes binding of the FP64 CPU oracle (oracle/libtg_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker.  The product (tarang_amd) never
+imports it.
+
+Arrays: complex spectral fields are numpy complex128 of shape (Nx, Ny, Nz/2+1);
+real fields are float64 of shape (Nx, Ny, Nz+2).  All arrays must be
+C-contiguous.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libtg_oracle.so")
+
+DEALIAS_NONE, DEALIAS_REF_CORNER, DEALIAS_TWOTHIRDS = 0, 1, 2
+
+
+class Grid(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+                ("kfactor", ctypes.c_double * 3), ("dealias", ctypes.c_int),
+                ("strong_reality", ctypes.c_int)]
+
+
+class Fluid(ctypes.Structure):
+    _fields_ = [("nu", ctypes.c_double), ("dt", ctypes.c_double), ("force_kind", ctypes.c_int),
+                ("force_eps", ctypes.c_double), ("force_r_in", ctypes.c_double),
+                ("force_r_out", ctypes.c_double)]
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(_LIB_PATH)
+        d = ctypes.c_double
+        p = ctypes.c_void_p
+        i = ctypes.c_int
+        gp = ctypes.POINTER(Grid)
+        fp = ctypes.POINTER(Fluid)
+        pp = ctypes.POINTER(ctypes.c_void_p)
+        sig = {
+            "or_fft1d": (i, [p, i, i]),
+            "or_forward": (None, [gp, p, p]),
+            "or_inverse": (None, [gp, p, p]),
+            "or_dealias": (None, [gp, p]),
+            "or_strong_reality": (None, [gp, p]),
+            "or_weak_reality": (None, [gp, p]),
+            "or_multiplicity": (d, [gp, i, i, i]),
+            "or_kmag": (d, [gp, i, i, i]),
+            "or_min_radius_outside": (i, [gp]),
+            "or_max_radius_inside": (i, [gp]),
+            "or_compute_nlin": (None, [gp, pp, pp]),
+            "or_compute_nlin_giver": (None, [gp, pp, pp, pp]),
+            "or_compute_force": (None, [gp, fp, pp, pp]),
+            "or_compute_rhs": (None, [gp, pp, pp]),
+            "or_rk4_step": (None, [gp, fp, pp]),
+            "or_euler_step": (None, [gp, fp, pp]),
+            "or_energy": (d, [gp, p]),
+            "or_Sn": (d, [gp, p, i]),
+            "or_shell_spectrum": (i, [gp, p, i, p, i]),
+            "or_fill_array_shell": (None, [gp, p, p, d, d]),
+            "or_shell_mult_single": (d, [gp, p, p, d, d]),
+            "or_shell_mult_all": (None, [gp, pp, pp, p, i, p]),
+            "or_flux": (None, [gp, pp, p, i, p]),
+            "or_shell_to_shell": (None, [gp, pp, p, i, p]),
+            "or_default_radii": (None, [gp, i, p]),
+            "or_init_field": (None, [gp, i, d, d, d, d, ctypes.c_uint64, pp]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(_lib, name)
+            fn.restype = res
+            fn.argtypes = args
+    return _lib
+
+
+def _ptr(a):
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _pp(arrs):
+    return (ctypes.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
+
+
+class Oracle:
+    """FP64 CPU restatement of the reference FFF path on one grid."""
+
+    def __init__(self, n, kfactor=(1.0, 1.0, 1.0), dealias=DEALIAS_REF_CORNER, strong_reality=True):
+        if isinstance(n, int):
+            n = (n, n, n)
+        self.nx, self.ny, self.nz = n
+        self.g = Grid(self.nx, self.ny, self.nz, (ctypes.c_double * 3)(*kfactor), dealias,
+                      1 if strong_reality else 0)
+        self.L = lib()
+
+    @property
+    def cshape(self):
+        return (self.nx, self.ny, self.nz // 2 + 1)
+
+    @property
+    def rshape(self):
+        return (self.nx, self.ny, self.nz + 2)
+
+    def czeros(self):
+        return np.zeros(self.cshape, np.complex128)
+
+    def forward(self, real):
+        real = np.ascontiguousarray(real, np.float64)
+        out = self.czeros()
+        self.L.or_forward(ctypes.byref(self.g), _ptr(real), _ptr(out))
+        return out
+
+    def inverse(self, spec):
+        spec = np.ascontiguousarray(spec, np.complex128)
+        out = np.zeros(self.rshape, np.float64)
+        self.L.or_inverse(ctypes.byref(self.g), _ptr(spec), _ptr(out))
+        return out
+
+    def dealias(self, spec):
+        self.L.or_dealias(ctypes.byref(self.g), _ptr(spec))
+
+    def strong_reality(self, spec):
+        self.L.or_strong_reality(ctypes.byref(self.g), _ptr(spec))
+
+    def compute_nlin(self, V):
+        N = [self.czeros() for _ in range(3)]
+        self.L.or_compute_nlin(ctypes.byref(self.g), _pp(V), _pp(N))
+        return N
+
+    def compute_nlin_giver(self, V, W):
+        N = [self.czeros() for _ in range(3)]
+        self.L.or_compute_nlin_giver(ctypes.byref(self.g), _pp(V), _pp(W), _pp(N))
+        return N
+
+    def compute_force(self, fluid, V):
+        F = [self.czeros() for _ in range(3)]
+        self.L.or_compute_force(ctypes.byref(self.g), ctypes.byref(fluid), _pp(V), _pp(F))
+        return F
+
+    def compute_rhs(self, N, F=None):
+        self.L.or_compute_rhs(ctypes.byref(self.g), _pp(N), _pp(F) if F is not None else None)
+
+    def rk4_step(self, fluid, V):
+        self.L.or_rk4_step(ctypes.byref(self.g), ctypes.byref(fluid), _pp(V))
+
+    def euler_step(self, fluid, V):
+        self.L.or_euler_step(ctypes.byref(self.g), ctypes.byref(fluid), _pp(V))
+
+    def energy(self, A):
+        return self.L.or_energy(ctypes.byref(self.g), _ptr(A))
+
+    def Sn(self, A, n):
+        return self.L.or_Sn(ctypes.byref(self.g), _ptr(A), n)
+
+    def multiplicity(self, lx, ly, lz):
+        return self.L.or_multiplicity(ctypes.byref(self.g), lx, ly, lz)
+
+    def kmag(self, lx, ly, lz):
+        return self.L.or_kmag(ctypes.byref(self.g), lx, ly, lz)
+
+    def min_radius_outside(self):
+        return self.L.or_min_radius_outside(ctypes.byref(self.g))
+
+    def max_radius_inside(self):
+        return self.L.or_max_radius_inside(ctypes.byref(self.g))
+
+    def shell_spectrum(self, A, n=0):
+        nb = self.min_radius_outside() + 1
+        Sk = np.zeros(nb)
+        self.L.or_shell_spectrum(ctypes.byref(self.g), _ptr(A), n, _ptr(Sk), nb)
+        return Sk
+
+    def fill_array_shell(self, A, r_in, r_out):
+        B = self.czeros()
+        self.L.or_fill_array_shell(ctypes.byref(self.g), _ptr(A), _ptr(B), r_in, r_out)
+        return B
+
+    def shell_mult_single(self, A, B, r_in, r_out):
+        return self.L.or_shell_mult_single(ctypes.byref(self.g), _ptr(A), _ptr(B), r_in, r_out)
+
+    def shell_mult_all(self, A, B, radii):
+        radii = np.ascontiguousarray(radii, np.float64)
+        out = np.zeros(len(radii))
+        self.L.or_shell_mult_all(ctypes.byref(self.g), _pp(A), _pp(B), _ptr(radii), len(radii), _ptr(out))
+        return out
+
+    def default_radii(self, n):
+        r = np.zeros(n + 1)
+        self.L.or_default_radii(ctypes.byref(self.g), n, _ptr(r))
+        return r
+
+    def flux(self, V, radii):
+        radii = np.ascontiguousarray(radii, np.float64)
+        out = np.zeros(len(radii))
+        self.L.or_flux(ctypes.byref(self.g), _pp(V), _ptr(radii), len(radii) - 1, _ptr(out))
+        return out
+
+    def shell_to_shell(self, V, radii):
+        radii = np.ascontiguousarray(radii, np.float64)
+        n = len(radii) - 1
+        out = np.zeros((n + 1, n + 1))
+        self.L.or_shell_to_shell(ctypes.byref(self.g), _pp(V), _ptr(radii), n, _ptr(out))
+        return out
+
+    def init_field(self, kind=0, k0=4.0, kmin=2.0, kmax=170.0, energy=0.5, seed=1):
+        V = [self.czeros() for _ in range(3)]
+        self.L.or_init_field(ctypes.byref(self.g), kind, k0, kmin, kmax, energy, seed, _pp(V))
+        return V
+
+
+def fft1d(x, sign):
+    x = np.array(x, dtype=np.complex128, copy=True)
+    lib().or_fft1d(_ptr(x), x.size, sign)
+    return x

@@ -1,0 +1,258 @@
+"""Python mirror of Tarang's basis / field / solver interface over the C ABI.
+
+The classes keep the reference's names and argument meaning so that a
+harness written against Tarang reads the same:
+
+  FFF_Basis               ~ Universal / FFF_PENCIL (universal.h:48-773) with
+                            fftk's transform plan (FFF_pencil.cc:91)
+  FluidVF                 ~ FluidVF (FluidVF.h:73-180): cvf V1..V3, nlin1..3,
+                            dissipation_coefficient, forcing switch
+  Nlin_incompress         ~ Nlin_incompress::Compute_nlin (Nlin.h:64-112)
+  Time_advance_incompress ~ Time_advance_incompress::Time_advance_step RK4
+                            (Time_advance.h:69-187)
+  EnergyTr                ~ EnergyTr::Compute_flux / Compute_shell_tr
+                            (EnergyTr.h:99-349)
+
+Device memory is held in torch tensors (complex128 on the GPU); torch is
+only the allocator and stream provider, every computation runs in
+libtarang_amd.so.  Errors raise TarangGpuError.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import (DEALIAS_NONE, DEALIAS_REF_CORNER, DEALIAS_TWOTHIRDS, PREC_FP32, PREC_FP64, STEP_CONTINUE,
+                   TgConfig, TgFluid, check, ptr_array)
+
+__all__ = ["FFF_Basis", "FluidVF", "Nlin_incompress", "Time_advance_incompress", "EnergyTr",
+           "DEALIAS_NONE", "DEALIAS_REF_CORNER", "DEALIAS_TWOTHIRDS", "PREC_FP64", "PREC_FP32"]
+
+
+def _vp(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+class FFF_Basis:
+    """Periodic Fourier basis on an (Nx, Ny, Nz) grid, one plan per instance."""
+
+    def __init__(self, N, kfactor=(1.0, 1.0, 1.0), precision=PREC_FP64, dealias=DEALIAS_REF_CORNER,
+                 strong_reality=True, device=0):
+        if isinstance(N, int):
+            N = (N, N, N)
+        self.Nx, self.Ny, self.Nz = (int(n) for n in N)
+        self.kfactor = tuple(float(k) for k in kfactor)
+        self.precision = precision
+        self.dealias_mode = dealias
+        self.device = torch.device("cuda", device)
+        self.L = _lib.load()
+        cfg = TgConfig(0, self.Nx, self.Ny, self.Nz, (ctypes.c_double * 3)(*self.kfactor), precision, dealias,
+                       1 if strong_reality else 0, device)
+        h = ctypes.c_void_p()
+        check("tg_plan_create", self.L.tg_plan_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self.plan = h
+        self.bind_stream()
+
+    def bind_stream(self, stream=None):
+        s = stream if stream is not None else torch.cuda.current_stream(self.device)
+        check("tg_set_stream", self.L.tg_set_stream(self.plan, ctypes.c_void_p(s.cuda_stream)))
+
+    def close(self):
+        if getattr(self, "plan", None):
+            self.L.tg_plan_destroy(self.plan)
+            self.plan = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # shapes (fftk Get_FA_shape / Get_RA_shape)
+    @property
+    def shape_complex_array(self):
+        return (self.Nx, self.Ny, self.Nz // 2 + 1)
+
+    @property
+    def shape_real_array(self):
+        return (self.Nx, self.Ny, self.Nz + 2)
+
+    def complex_array(self):
+        return torch.zeros(self.shape_complex_array, dtype=torch.complex128, device=self.device)
+
+    def real_array(self):
+        return torch.zeros(self.shape_real_array, dtype=torch.float64, device=self.device)
+
+    def device_bytes(self):
+        return int(self.L.tg_plan_device_bytes(self.plan))
+
+    def synchronize(self):
+        check("tg_synchronize", self.L.tg_synchronize(self.plan))
+
+    # transforms (FFF_pencil_tr.cc:43-66)
+    def Forward_transform(self, Ar, A):
+        check("tg_forward", self.L.tg_forward(self.plan, 0, _vp(Ar), _vp(A)))
+
+    def Inverse_transform(self, A, Ar):
+        check("tg_inverse", self.L.tg_inverse(self.plan, 0, _vp(A), _vp(Ar)))
+
+    def Dealias(self, *arrays):
+        check("tg_dealias", self.L.tg_dealias(self.plan, ptr_array([a.data_ptr() for a in arrays]), len(arrays)))
+
+    def Satisfy_strong_reality_condition_in_Array(self, *arrays):
+        check("tg_strong_reality",
+              self.L.tg_strong_reality(self.plan, ptr_array([a.data_ptr() for a in arrays]), len(arrays)))
+
+    # reductions (universal_energy.cc)
+    def Get_total_energy(self, A):
+        out = ctypes.c_double()
+        check("tg_energy", self.L.tg_energy(self.plan, _vp(A), ctypes.byref(out)))
+        return out.value
+
+    def Get_total_Sn(self, A, n):
+        out = ctypes.c_double()
+        check("tg_Sn", self.L.tg_Sn(self.plan, _vp(A), int(n), ctypes.byref(out)))
+        return out.value
+
+    def Min_radius_outside(self):
+        a, b, c = ((self.Nx // 2) * self.kfactor[0], (self.Ny // 2) * self.kfactor[1],
+                   (self.Nz // 2) * self.kfactor[2])
+        return int(np.ceil(np.sqrt(a * a + b * b + c * c)))
+
+    def Max_radius_inside(self):
+        return int(min((self.Nx // 2) * self.kfactor[0], (self.Ny // 2) * self.kfactor[1],
+                       (self.Nz // 2) * self.kfactor[2]))
+
+    def Compute_shell_spectrum(self, A, n=0):
+        nb = self.Min_radius_outside() + 1
+        out = (ctypes.c_double * nb)()
+        check("tg_shell_spectrum", self.L.tg_shell_spectrum(self.plan, _vp(A), int(n), out, nb))
+        return np.array(out[:])
+
+    def default_radii(self, n):
+        """Log-spaced radii (Global_process_advanced_vars.cc:137-175), n+1 entries."""
+        mri = self.Max_radius_inside()
+        r = np.zeros(n + 1)
+        r[0], r[min(1, n)], r[min(2, n)], r[min(3, n)] = 0.0, 2.0, 4.0, 8.0
+        r[n - 2] = mri / 2.0
+        r[n - 1] = mri
+        r[n] = np.inf
+        if n > 6:
+            s = np.log2(mri / 16.0) / (n - 5)
+            for i in range(4, n - 2):
+                r[i] = 8 * 2 ** (s * (i - 3))
+        return r
+
+
+class FluidVF:
+    """Solver-level vector field: cvf (V1, V2, V3), nlin1..3, dissipation."""
+
+    def __init__(self, basis, dissipation_coefficient=0.0, force_switch=False, name="U"):
+        self.basis = basis
+        self.name = name
+        self.dissipation_coefficient = float(dissipation_coefficient)
+        self.force_switch = force_switch
+        self.cvf = [basis.complex_array() for _ in range(3)]
+        self.nlin = [basis.complex_array() for _ in range(3)]
+
+    @property
+    def V1(self):
+        return self.cvf[0]
+
+    @property
+    def V2(self):
+        return self.cvf[1]
+
+    @property
+    def V3(self):
+        return self.cvf[2]
+
+    def ptrs(self, which="cvf"):
+        arr = self.cvf if which == "cvf" else self.nlin
+        return ptr_array([a.data_ptr() for a in arr])
+
+    def set_field(self, V):
+        for dst, src in zip(self.cvf, V):
+            dst.copy_(torch.as_tensor(src, dtype=torch.complex128))
+
+    def Init_cond_spectrum(self, kind=0, k0=4.0, kmin=2.0, kmax=170.0, energy=0.5, seed=1):
+        """Seeded solenoidal field (kind 0: K^4 exp(-2(K/k0)^2); kind 1: K^-5/3)."""
+        b = self.basis
+        check("tg_init_field", b.L.tg_init_field(b.plan, kind, k0, kmin, kmax, energy, seed, self.ptrs()))
+
+    def Compute_total_energy(self):
+        e = [self.basis.Get_total_energy(a) for a in self.cvf]
+        self.total_E1, self.total_E2, self.total_E3 = e
+        self.total_energy = sum(e)
+        return self.total_energy
+
+    def Compute_total_k2energy(self):
+        self.total_k2energy = 2 * sum(self.basis.Get_total_Sn(a, 2) for a in self.cvf)
+        return self.total_k2energy
+
+    def dissipation(self):
+        return self.dissipation_coefficient * self.Compute_total_k2energy()
+
+
+class Nlin_incompress:
+    @staticmethod
+    def Compute_nlin(U, W=None):
+        b = U.basis
+        if W is None:
+            check("tg_compute_nlin", b.L.tg_compute_nlin(b.plan, U.ptrs(), U.ptrs("nlin")))
+        else:
+            check("tg_compute_nlin_giver", b.L.tg_compute_nlin_giver(b.plan, U.ptrs(), W.ptrs(), U.ptrs("nlin")))
+
+
+class Time_advance_incompress:
+    """RK4 time advance of an incompressible fluid on one GPU."""
+
+    def __init__(self, U, dt, force_kind=0, force_eps=0.0, force_r_in=0.0, force_r_out=0.0):
+        self.U = U
+        self.fluid = TgFluid(U.dissipation_coefficient, float(dt), int(force_kind), float(force_eps),
+                             float(force_r_in), float(force_r_out))
+        self._continue = False
+
+    def Compute_force_TO_rhs(self):
+        b = self.U.basis
+        check("tg_compute_rhs", b.L.tg_compute_rhs(b.plan, self.U.ptrs(), ctypes.byref(self.fluid),
+                                                   self.U.ptrs("nlin")))
+        self._continue = False
+
+    def Time_advance_step(self, nsteps=1, chain=True):
+        b = self.U.basis
+        flags = STEP_CONTINUE if (chain and self._continue) else 0
+        check("tg_rk4_step", b.L.tg_rk4_step(b.plan, self.U.ptrs(), ctypes.byref(self.fluid), int(nsteps), flags))
+        self._continue = True
+
+    def invalidate(self):
+        self._continue = False
+
+
+class EnergyTr:
+    def __init__(self, basis, flux_radii=None, shell_radii=None):
+        self.basis = basis
+        self.flux_radii = flux_radii
+        self.shell_radii = shell_radii
+
+    def Compute_flux(self, U):
+        r = np.ascontiguousarray(self.flux_radii, dtype=np.float64)
+        n = len(r) - 1
+        out = np.zeros(n + 1)
+        b = self.basis
+        check("tg_flux", b.L.tg_flux(b.plan, U.ptrs(), r.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n,
+                                     out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        self.flux_self = out
+        return out
+
+    def Compute_shell_tr(self, U):
+        r = np.ascontiguousarray(self.shell_radii, dtype=np.float64)
+        n = len(r) - 1
+        out = np.zeros((n + 1, n + 1))
+        b = self.basis
+        check("tg_shell_to_shell",
+              b.L.tg_shell_to_shell(b.plan, U.ptrs(), r.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n,
+                                    out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        self.shelltoshell_self = out
+        return out

@@ -1,0 +1,609 @@
+// tg_api.hip -- plan object and the extern "C" boundary (include/tarang_gpu.h).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/tarang_gpu.h"
+#include "tg_diag.h"
+#include "tg_kernels.h"
+
+using namespace tg;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                                   \
+    do {                                                                                               \
+        hipError_t e_ = (expr);                                                                        \
+        if (e_ != hipSuccess) return fail(TG_ERR_HIP, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                                          __FILE__, __LINE__);                                         \
+    } while (0)
+
+bool pow2_in(int n, int lo, int hi) { return n >= lo && n <= hi && (n & (n - 1)) == 0; }
+
+}  // namespace
+
+struct tg_plan {
+    tg_config cfg{};
+    Grid g{};
+    hipStream_t stream = nullptr;
+    int prec = TG_PREC_FP64;
+    size_t elem = 16;     // bytes of one complex work element
+    size_t nc = 0;        // complex modes per field
+    int64_t dev_bytes = 0;
+    // twiddle tables in the transform precision: x (Nx), y (Ny), m (Nz/2), n (Nz, k = 0..Nz/2)
+    void *twx = nullptr, *twy = nullptr, *twm = nullptr, *twn = nullptr;
+    void *U[6] = {};      // x/y-transformed velocity (0..2) and giver (3..5) lines
+    void *P[9] = {};      // product spectra
+    double *acc[3] = {};  // RK4 accumulator (tot_Vrhs, RK.cc:96); giver nonlinear term
+    double *slots[2] = {};
+    size_t nslots = 0;
+    int slot_kb = -1;
+    double *partial = nullptr;
+    size_t partial_bytes = 0;
+    double *small = nullptr;  // device scratch for reduction outputs / radii
+    int red_blocks = 0;
+    // chaining of RK4 steps
+    bool unext_valid = false;
+    double *unext_V[3] = {};
+    // host-side band config for the current step
+    Band band{};
+};
+
+namespace {
+
+template <typename T>
+int upload_twiddles(tg_plan *p, void **dst, int L, int count)
+{
+    std::vector<T> h(2 * (size_t)count);
+    for (int k = 0; k < count; k++) {
+        const double a = -2.0 * M_PI * (double)k / (double)L;
+        h[2 * k] = (T)std::cos(a);
+        h[2 * k + 1] = (T)std::sin(a);
+    }
+    HIPCHK(hipMalloc(dst, sizeof(T) * 2 * count));
+    p->dev_bytes += sizeof(T) * 2 * count;
+    HIPCHK(hipMemcpy(*dst, h.data(), sizeof(T) * 2 * count, hipMemcpyHostToDevice));
+    return TG_OK;
+}
+
+int dmalloc(tg_plan *p, void **ptr, size_t bytes)
+{
+    hipError_t e = hipMalloc(ptr, bytes);
+    if (e != hipSuccess) {
+        *ptr = nullptr;
+        return fail(TG_ERR_OOM, "hipMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
+    }
+    p->dev_bytes += (int64_t)bytes;
+    return TG_OK;
+}
+
+void free_plan(tg_plan *p)
+{
+    if (!p) return;
+    void *ptrs[] = {p->twx, p->twy, p->twm, p->twn, p->acc[0], p->acc[1], p->acc[2], p->slots[0],
+                    p->slots[1], p->partial, p->small};
+    for (void *q : ptrs)
+        if (q) (void)hipFree(q);
+    for (void *q : p->U)
+        if (q) (void)hipFree(q);
+    for (void *q : p->P)
+        if (q) (void)hipFree(q);
+    delete p;
+}
+
+int set_device(tg_plan *p) { HIPCHK(hipSetDevice(p->cfg.device)); return TG_OK; }
+
+int check_ptrs(const void *const *v, int n, const char *what)
+{
+    if (!v) return fail(TG_ERR_ARG, "%s: null pointer array", what);
+    for (int i = 0; i < n; i++)
+        if (!v[i]) return fail(TG_ERR_ARG, "%s: component %d is null", what, i);
+    return TG_OK;
+}
+
+int ensure_slots(tg_plan *p, const tg_fluid *f)
+{
+    Band b{};
+    if (f && f->force_kind == 1) {
+        if (!(f->force_r_out > f->force_r_in) || f->force_r_in < 0)
+            return fail(TG_ERR_ARG, "forcing band must satisfy 0 <= r_in < r_out");
+        double kmin = std::fmin(p->g.kf[0], std::fmin(p->g.kf[1], p->g.kf[2]));
+        int kb = (int)std::ceil(f->force_r_out / kmin);
+        if (kb > p->g.nx / 2 - 1 || kb > p->g.ny / 2 - 1 || kb > p->g.nz / 2 - 1 || kb > 64)
+            return fail(TG_ERR_ARG, "forcing band r_out=%g too large for this grid", f->force_r_out);
+        b.kind = 1;
+        b.kb = kb;
+        b.r_in = f->force_r_in;
+        b.r_out = f->force_r_out;
+        b.eps = f->force_eps;
+    } else if (f && f->force_kind != 0) {
+        return fail(TG_ERR_ARG, "unknown force_kind %d", f->force_kind);
+    }
+    p->band = b;
+    if (b.kind == 0) return TG_OK;
+    if (p->slot_kb != b.kb) {
+        for (int i = 0; i < 2; i++)
+            if (p->slots[i]) { (void)hipFree(p->slots[i]); p->slots[i] = nullptr; }
+        p->nslots = (size_t)band_nslots(b.kb);
+        for (int i = 0; i < 2; i++) {
+            int rc = dmalloc(p, (void **)&p->slots[i], p->nslots * 3 * 16);
+            if (rc) return rc;
+            HIPCHK(hipMemsetAsync(p->slots[i], 0, p->nslots * 3 * 16, p->stream));
+        }
+        p->slot_kb = b.kb;
+    }
+    return TG_OK;
+}
+
+template <typename T>
+int run_forward(tg_plan *p, const double *real_dev, double *spec_dev)
+{
+    const double *R[1] = {real_dev};
+    void *W[1] = {p->U[0]};
+    HIPCHK(launch_z_r2c<T>(p->g, R, 1, W, p->twm, p->twn, p->stream));
+    HIPCHK(launch_colfft<T>(p->g, 1, -1, W, 1, p->twy, p->stream));
+    double *O[1] = {spec_dev};
+    const double scale = 1.0 / ((double)p->g.nx * p->g.ny * p->g.nz);
+    HIPCHK(launch_xfwd_spec<T>(p->g, W, 1, O, scale, p->twx, p->stream));
+    return TG_OK;
+}
+
+template <typename T>
+int run_inverse(tg_plan *p, const double *spec_dev, double *real_dev)
+{
+    double *V[1] = {const_cast<double *>(spec_dev)};
+    void *W[1] = {p->U[0]};
+    Band nb{};
+    HIPCHK(launch_xinv_state<T>(p->g, nb, V, 1, W, nullptr, 0, p->twx, p->stream));
+    HIPCHK(launch_colfft<T>(p->g, 1, +1, W, 1, p->twy, p->stream));
+    double *R[1] = {real_dev};
+    HIPCHK(launch_z_c2r<T>(p->g, W, 1, R, p->twm, p->twn, p->stream));
+    return TG_OK;
+}
+
+// Velocity x,y-inverse (U[0..2]) -> products -> y-forward (P[0..5])
+template <typename T>
+int ns_middle(tg_plan *p)
+{
+    HIPCHK(launch_colfft<T>(p->g, 1, +1, p->U, 3, p->twy, p->stream));
+    HIPCHK(launch_z_products<T>(p->g, p->U, p->P, 0, p->twm, p->twn, p->stream));
+    HIPCHK(launch_colfft<T>(p->g, 1, -1, p->P, 6, p->twy, p->stream));
+    return TG_OK;
+}
+
+RkParams rk_params(const tg_plan *p, const tg_fluid *f, int stage)
+{
+    RkParams rk{};
+    rk.stage = stage;
+    rk.write_rhs = 0;
+    const double nu = f ? f->nu : 0.0, dt = f ? f->dt : 0.0;
+    rk.dt = dt;
+    // Mult_field_exp_ksqr_dt factor = -dissipation_coefficient*a*dt (FluidVF.cc:342)
+    rk.fh = -nu * 0.5 * dt;
+    rk.f1 = -nu * 1.0 * dt;
+    rk.scale = 1.0 / ((double)p->g.nx * p->g.ny * p->g.nz);
+    return rk;
+}
+
+template <typename T>
+int run_nlin(tg_plan *p, double *const V[3], double *const out[3], const tg_fluid *f, int write_rhs)
+{
+    if (write_rhs) {
+        int rc = ensure_slots(p, f);
+        if (rc) return rc;
+    } else {
+        p->band = Band{};
+    }
+    // Compute_nlin dealiases the state in place (compute_nlin.cc:61-62)
+    if (p->g.dealias) HIPCHK(launch_dealias(p->g, V, 3, p->stream));
+    HIPCHK(launch_xinv_state<T>(p->g, p->band, V, 3, p->U, p->slots[0], p->band.kind ? 2 : 0, p->twx, p->stream));
+    int rc = ns_middle<T>(p);
+    if (rc) return rc;
+    RkParams rk = rk_params(p, f, 0);
+    rk.write_rhs = write_rhs;
+    HIPCHK(launch_x_nlin<T>(p->g, p->band, rk, p->P, nullptr, nullptr, nullptr, p->slots[0], nullptr, out, p->twx,
+                            p->stream));
+    p->unext_valid = false;
+    return TG_OK;
+}
+
+template <typename T>
+int run_giver_nlin(tg_plan *p, double *const V[3], double *const out[3], double r_lo, double r_hi, bool filter,
+                   double *const W[3])
+{
+    Band nb{};
+    // u = inverse of the (dealiased) state, x and y passes
+    HIPCHK(launch_xinv_state<T>(p->g, nb, V, 3, p->U, nullptr, 0, p->twx, p->stream));
+    HIPCHK(launch_colfft<T>(p->g, 1, +1, p->U, 3, p->twy, p->stream));
+    // w = giver field
+    if (filter)
+        HIPCHK(launch_xinv_state<T>(p->g, nb, V, 3, p->U + 3, nullptr, 8, p->twx, p->stream, r_lo, r_hi));
+    else
+        HIPCHK(launch_xinv_state<T>(p->g, nb, W, 3, p->U + 3, nullptr, 0, p->twx, p->stream));
+    HIPCHK(launch_colfft<T>(p->g, 1, +1, p->U + 3, 3, p->twy, p->stream));
+    HIPCHK(launch_z_products<T>(p->g, p->U, p->P, 1, p->twm, p->twn, p->stream));
+    HIPCHK(launch_colfft<T>(p->g, 1, -1, p->P, 9, p->twy, p->stream));
+    RkParams rk = rk_params(p, nullptr, -1);
+    HIPCHK(launch_x_nlin<T>(p->g, nb, rk, p->P, nullptr, nullptr, nullptr, nullptr, nullptr, out, p->twx,
+                            p->stream));
+    p->unext_valid = false;
+    return TG_OK;
+}
+
+template <typename T>
+int run_rk4(tg_plan *p, double *const V[3], const tg_fluid *f, int nsteps, int flags)
+{
+    int rc = ensure_slots(p, f);
+    if (rc) return rc;
+    const bool cont = (flags & TG_STEP_CONTINUE) && p->unext_valid && p->unext_V[0] == V[0] &&
+                      p->unext_V[1] == V[1] && p->unext_V[2] == V[2];
+    if (!cont) {
+        HIPCHK(launch_xinv_state<T>(p->g, p->band, V, 3, p->U, p->slots[0], 1 | (p->band.kind ? 2 : 0), p->twx,
+                                    p->stream));
+    }
+    for (int step = 0; step < nsteps; step++) {
+        for (int s = 1; s <= 4; s++) {
+            rc = ns_middle<T>(p);
+            if (rc) return rc;
+            RkParams rk = rk_params(p, f, s);
+            HIPCHK(launch_x_nlin<T>(p->g, p->band, rk, p->P, V, p->acc, p->U, p->slots[(s - 1) & 1],
+                                    p->slots[s & 1], nullptr, p->twx, p->stream));
+        }
+        if (p->cfg.strong_reality) {
+            HIPCHK(launch_reality(p->g, V, 3, p->stream));
+            HIPCHK(launch_xinv_state<T>(p->g, p->band, V, 3, p->U, p->slots[0], 1 | 4 | (p->band.kind ? 2 : 0),
+                                        p->twx, p->stream));
+        }
+    }
+    p->unext_valid = true;
+    for (int c = 0; c < 3; c++) p->unext_V[c] = V[c];
+    return TG_OK;
+}
+
+int reduce_to_host(tg_plan *p, int mode, const double *const *A, const double *const *B, int ncomp, int n,
+                   const double *radii_host, int nbins, int nq, int kcut, double r_lo, double r_hi, double *out)
+{
+    const int width = nq * nbins;
+    double *radii_dev = nullptr;
+    size_t need_small = sizeof(double) * (size_t)(width + nbins + 8);
+    if (need_small > 65536) return fail(TG_ERR_ARG, "too many bins (%d)", nbins);
+    double *outd = p->small;
+    if (radii_host) {
+        radii_dev = p->small + width + 4;
+        HIPCHK(hipMemcpyAsync(radii_dev, radii_host, sizeof(double) * nbins, hipMemcpyHostToDevice, p->stream));
+    }
+    const size_t need = (size_t)p->red_blocks * width * sizeof(double);
+    if (need > p->partial_bytes) return fail(TG_ERR_ARG, "reduction needs %zu bytes of partials", need);
+    HIPCHK(launch_reduce(p->g, mode, A, B, ncomp, n, radii_dev, nbins, nq, kcut, r_lo, r_hi, p->partial,
+                         p->red_blocks, outd, p->stream));
+    HIPCHK(hipMemcpyAsync(out, outd, sizeof(double) * width, hipMemcpyDeviceToHost, p->stream));
+    HIPCHK(hipStreamSynchronize(p->stream));
+    return TG_OK;
+}
+
+int max_radius_inside(const Grid &g)
+{
+    double k = std::fmin((g.nx / 2) * g.kf[0], (g.ny / 2) * g.kf[1]);
+    k = std::fmin(k, (g.nz / 2) * g.kf[2]);
+    return (int)k;
+}
+
+int min_radius_outside(const Grid &g)
+{
+    double a = (g.nx / 2) * g.kf[0], b = (g.ny / 2) * g.kf[1], c = (g.nz / 2) * g.kf[2];
+    return (int)std::ceil(std::sqrt(a * a + b * b + c * c));
+}
+
+}  // namespace
+
+#define DISPATCH_PREC(p, CALL) ((p)->prec == TG_PREC_FP32 ? CALL<float> : CALL<double>)
+
+extern "C" {
+
+const char *tg_last_error_string(void) { return g_last_error.c_str(); }
+
+const char *tg_version(void) { return "tarang_amd 0.1 (gfx950)"; }
+
+int tg_plan_create(const tg_config *cfg, tg_plan **plan)
+{
+    if (!cfg || !plan) return fail(TG_ERR_ARG, "null argument");
+    *plan = nullptr;
+    if (cfg->basis != TG_BASIS_FFF) return fail(TG_ERR_ARG, "unsupported basis %d", cfg->basis);
+    if (!pow2_in(cfg->nx, 8, 1024) || !pow2_in(cfg->ny, 8, 1024) || !pow2_in(cfg->nz, 16, 1024))
+        return fail(TG_ERR_SHAPE, "grid %dx%dx%d unsupported (powers of two, nx,ny in [8,1024], nz in [16,1024])",
+                    cfg->nx, cfg->ny, cfg->nz);
+    if (cfg->precision != TG_PREC_FP64 && cfg->precision != TG_PREC_FP32)
+        return fail(TG_ERR_ARG, "unknown precision %d", cfg->precision);
+    if (cfg->dealias < 0 || cfg->dealias > 2) return fail(TG_ERR_ARG, "unknown dealias mode %d", cfg->dealias);
+    for (int i = 0; i < 3; i++)
+        if (!(cfg->kfactor[i] > 0)) return fail(TG_ERR_ARG, "kfactor[%d] must be positive", i);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(TG_ERR_HIP, "no HIP device available");
+    if (cfg->device < 0 || cfg->device >= ndev) return fail(TG_ERR_ARG, "device %d out of range", cfg->device);
+    tg_plan *p = new (std::nothrow) tg_plan();
+    if (!p) return fail(TG_ERR_OOM, "host allocation failed");
+    p->cfg = *cfg;
+    p->g.nx = cfg->nx;
+    p->g.ny = cfg->ny;
+    p->g.nz = cfg->nz;
+    p->g.nzc = cfg->nz / 2 + 1;
+    for (int i = 0; i < 3; i++) p->g.kf[i] = cfg->kfactor[i];
+    p->g.dealias = cfg->dealias;
+    p->prec = cfg->precision;
+    p->elem = p->prec == TG_PREC_FP32 ? 8 : 16;
+    p->nc = (size_t)cfg->nx * cfg->ny * (cfg->nz / 2 + 1);
+    int rc = set_device(p);
+    if (rc) { free_plan(p); return rc; }
+    if (p->prec == TG_PREC_FP32) {
+        rc = upload_twiddles<float>(p, &p->twx, cfg->nx, cfg->nx);
+        if (!rc) rc = upload_twiddles<float>(p, &p->twy, cfg->ny, cfg->ny);
+        if (!rc) rc = upload_twiddles<float>(p, &p->twm, cfg->nz / 2, cfg->nz / 2);
+        if (!rc) rc = upload_twiddles<float>(p, &p->twn, cfg->nz, cfg->nz / 2 + 1);
+    } else {
+        rc = upload_twiddles<double>(p, &p->twx, cfg->nx, cfg->nx);
+        if (!rc) rc = upload_twiddles<double>(p, &p->twy, cfg->ny, cfg->ny);
+        if (!rc) rc = upload_twiddles<double>(p, &p->twm, cfg->nz / 2, cfg->nz / 2);
+        if (!rc) rc = upload_twiddles<double>(p, &p->twn, cfg->nz, cfg->nz / 2 + 1);
+    }
+    for (int i = 0; i < 6 && !rc; i++) rc = dmalloc(p, &p->U[i], p->nc * p->elem);
+    for (int i = 0; i < 9 && !rc; i++) rc = dmalloc(p, &p->P[i], p->nc * p->elem);
+    for (int i = 0; i < 3 && !rc; i++) rc = dmalloc(p, (void **)&p->acc[i], p->nc * 16);
+    p->red_blocks = (int)std::min<size_t>(2048, (p->nc + 1023) / 1024);
+    if (p->red_blocks < 1) p->red_blocks = 1;
+    p->partial_bytes = (size_t)p->red_blocks * 2 * 1100 * sizeof(double);
+    if (!rc) rc = dmalloc(p, (void **)&p->partial, p->partial_bytes);
+    if (!rc) rc = dmalloc(p, (void **)&p->small, 65536);
+    if (rc) { free_plan(p); return rc; }
+    *plan = p;
+    return TG_OK;
+}
+
+int tg_plan_destroy(tg_plan *plan)
+{
+    if (!plan) return TG_OK;
+    (void)hipSetDevice(plan->cfg.device);
+    (void)hipDeviceSynchronize();
+    free_plan(plan);
+    return TG_OK;
+}
+
+int tg_set_stream(tg_plan *plan, void *hip_stream)
+{
+    if (!plan) return fail(TG_ERR_ARG, "null plan");
+    plan->stream = (hipStream_t)hip_stream;
+    return TG_OK;
+}
+
+int tg_synchronize(tg_plan *plan)
+{
+    if (!plan) return fail(TG_ERR_ARG, "null plan");
+    HIPCHK(hipStreamSynchronize(plan->stream));
+    return TG_OK;
+}
+
+int tg_get_shapes(const tg_plan *plan, int64_t cshape[3], int64_t rshape[3])
+{
+    if (!plan) return fail(TG_ERR_ARG, "null plan");
+    if (cshape) { cshape[0] = plan->g.nx; cshape[1] = plan->g.ny; cshape[2] = plan->g.nzc; }
+    if (rshape) { rshape[0] = plan->g.nx; rshape[1] = plan->g.ny; rshape[2] = plan->g.nz + 2; }
+    return TG_OK;
+}
+
+int64_t tg_plan_device_bytes(const tg_plan *plan) { return plan ? plan->dev_bytes : 0; }
+
+int tg_forward(tg_plan *plan, int sincos_code, const double *real_dev, double *spec_dev)
+{
+    if (!plan || !real_dev || !spec_dev) return fail(TG_ERR_ARG, "null argument");
+    if (sincos_code != 0) return fail(TG_ERR_ARG, "FFF basis takes sincos_code 0");
+    int rc = set_device(plan);
+    if (rc) return rc;
+    plan->unext_valid = false;
+    return DISPATCH_PREC(plan, run_forward)(plan, real_dev, spec_dev);
+}
+
+int tg_inverse(tg_plan *plan, int sincos_code, const double *spec_dev, double *real_dev)
+{
+    if (!plan || !real_dev || !spec_dev) return fail(TG_ERR_ARG, "null argument");
+    if (sincos_code != 0) return fail(TG_ERR_ARG, "FFF basis takes sincos_code 0");
+    int rc = set_device(plan);
+    if (rc) return rc;
+    plan->unext_valid = false;
+    return DISPATCH_PREC(plan, run_inverse)(plan, spec_dev, real_dev);
+}
+
+int tg_dealias(tg_plan *plan, double *const V_dev[], int ncomp)
+{
+    if (!plan || ncomp < 1 || ncomp > 3) return fail(TG_ERR_ARG, "bad argument");
+    int rc = check_ptrs((const void *const *)V_dev, ncomp, "tg_dealias");
+    if (rc) return rc;
+    rc = set_device(plan);
+    if (rc) return rc;
+    if (plan->g.dealias) HIPCHK(launch_dealias(plan->g, V_dev, ncomp, plan->stream));
+    return TG_OK;
+}
+
+int tg_strong_reality(tg_plan *plan, double *const V_dev[], int ncomp)
+{
+    if (!plan || ncomp < 1 || ncomp > 3) return fail(TG_ERR_ARG, "bad argument");
+    int rc = check_ptrs((const void *const *)V_dev, ncomp, "tg_strong_reality");
+    if (rc) return rc;
+    rc = set_device(plan);
+    if (rc) return rc;
+    plan->unext_valid = false;
+    HIPCHK(launch_reality(plan->g, V_dev, ncomp, plan->stream));
+    return TG_OK;
+}
+
+int tg_compute_nlin(tg_plan *plan, double *const V_dev[3], double *const nlin_dev[3])
+{
+    if (!plan) return fail(TG_ERR_ARG, "null plan");
+    int rc = check_ptrs((const void *const *)V_dev, 3, "V");
+    if (!rc) rc = check_ptrs((const void *const *)nlin_dev, 3, "nlin");
+    if (!rc) rc = set_device(plan);
+    if (rc) return rc;
+    return DISPATCH_PREC(plan, run_nlin)(plan, V_dev, nlin_dev, nullptr, 0);
+}
+
+int tg_compute_rhs(tg_plan *plan, double *const V_dev[3], const tg_fluid *fluid, double *const rhs_dev[3])
+{
+    if (!plan || !fluid) return fail(TG_ERR_ARG, "null argument");
+    int rc = check_ptrs((const void *const *)V_dev, 3, "V");
+    if (!rc) rc = check_ptrs((const void *const *)rhs_dev, 3, "rhs");
+    if (!rc) rc = set_device(plan);
+    if (rc) return rc;
+    return DISPATCH_PREC(plan, run_nlin)(plan, V_dev, rhs_dev, fluid, 1);
+}
+
+int tg_compute_nlin_giver(tg_plan *plan, double *const V_dev[3], double *const W_dev[3], double *const nlin_dev[3])
+{
+    if (!plan) return fail(TG_ERR_ARG, "null plan");
+    int rc = check_ptrs((const void *const *)V_dev, 3, "V");
+    if (!rc) rc = check_ptrs((const void *const *)W_dev, 3, "W");
+    if (!rc) rc = check_ptrs((const void *const *)nlin_dev, 3, "nlin");
+    if (!rc) rc = set_device(plan);
+    if (rc) return rc;
+    if (plan->g.dealias) {
+        HIPCHK(launch_dealias(plan->g, V_dev, 3, plan->stream));
+        HIPCHK(launch_dealias(plan->g, W_dev, 3, plan->stream));
+    }
+    return DISPATCH_PREC(plan, run_giver_nlin)(plan, V_dev, nlin_dev, 0.0, 0.0, false, W_dev);
+}
+
+int tg_rk4_step(tg_plan *plan, double *const V_dev[3], const tg_fluid *fluid, int nsteps, int flags)
+{
+    if (!plan || !fluid) return fail(TG_ERR_ARG, "null argument");
+    if (nsteps < 0) return fail(TG_ERR_ARG, "nsteps must be >= 0");
+    if (!(fluid->dt > 0) || !(fluid->nu >= 0)) return fail(TG_ERR_ARG, "need dt > 0 and nu >= 0");
+    int rc = check_ptrs((const void *const *)V_dev, 3, "V");
+    if (!rc) rc = set_device(plan);
+    if (rc) return rc;
+    return DISPATCH_PREC(plan, run_rk4)(plan, V_dev, fluid, nsteps, flags);
+}
+
+int tg_energy(tg_plan *plan, const double *A_dev, double *energy)
+{
+    if (!plan || !A_dev || !energy) return fail(TG_ERR_ARG, "null argument");
+    int rc = set_device(plan);
+    if (rc) return rc;
+    const double *A[1] = {A_dev};
+    return reduce_to_host(plan, RED_ENERGY, A, nullptr, 1, 0, nullptr, 1, 1, 0, 0, 0, energy);
+}
+
+int tg_Sn(tg_plan *plan, const double *A_dev, int n, double *Sn)
+{
+    if (!plan || !A_dev || !Sn || n < 0) return fail(TG_ERR_ARG, "bad argument");
+    int rc = set_device(plan);
+    if (rc) return rc;
+    const double *A[1] = {A_dev};
+    return reduce_to_host(plan, RED_SN, A, nullptr, 1, n, nullptr, 1, 1, min_radius_outside(plan->g), 0, 0, Sn);
+}
+
+int tg_shell_spectrum(tg_plan *plan, const double *A_dev, int n, double *Sk, int nbins)
+{
+    if (!plan || !A_dev || !Sk || n < 0) return fail(TG_ERR_ARG, "bad argument");
+    const int kmax = min_radius_outside(plan->g);
+    if (nbins < kmax + 1) return fail(TG_ERR_ARG, "nbins must be >= %d", kmax + 1);
+    int rc = set_device(plan);
+    if (rc) return rc;
+    std::vector<double> raw(2 * (size_t)(kmax + 1));
+    const double *A[1] = {A_dev};
+    rc = reduce_to_host(plan, RED_SPECTRUM, A, nullptr, 1, n, nullptr, kmax + 1, 2, kmax, 0, 0, raw.data());
+    if (rc) return rc;
+    const double kf = plan->g.kf[0] * plan->g.kf[1] * plan->g.kf[2];
+    for (int i = 0; i < nbins; i++) Sk[i] = 0.0;
+    for (int i = 0; i <= kmax; i++) {
+        const double cnt = raw[(size_t)(kmax + 1) + i];
+        Sk[i] = cnt <= 0 ? 0.0 : raw[i] * (4 * M_PI) / (cnt * kf);
+    }
+    return TG_OK;
+}
+
+int tg_flux(tg_plan *plan, double *const V_dev[3], const double *radii, int nspheres, double *flux)
+{
+    if (!plan || !radii || !flux || nspheres < 1) return fail(TG_ERR_ARG, "bad argument");
+    int rc = check_ptrs((const void *const *)V_dev, 3, "V");
+    if (!rc) rc = set_device(plan);
+    if (rc) return rc;
+    // Compute_nlin(U, Giver) dealiases U in place (compute_nlin_ET_VF.cc:74-77)
+    if (plan->g.dealias) HIPCHK(launch_dealias(plan->g, V_dev, 3, plan->stream));
+    flux[0] = 0.0;
+    for (int s = 1; s <= nspheres; s++) {
+        rc = DISPATCH_PREC(plan, run_giver_nlin)(plan, V_dev, plan->acc, 0.0, radii[s], true, nullptr);
+        if (rc) return rc;
+        double v = 0.0;
+        const double *A[3] = {plan->acc[0], plan->acc[1], plan->acc[2]};
+        const double *B[3] = {V_dev[0], V_dev[1], V_dev[2]};
+        rc = reduce_to_host(plan, RED_SHELL_SINGLE, A, B, 3, 0, nullptr, 1, 1, 0, radii[s], INFINITY, &v);
+        if (rc) return rc;
+        flux[s] = -v;
+    }
+    return TG_OK;
+}
+
+int tg_shell_to_shell(tg_plan *plan, double *const V_dev[3], const double *radii, int nshells, double *S)
+{
+    if (!plan || !radii || !S || nshells < 1) return fail(TG_ERR_ARG, "bad argument");
+    int rc = check_ptrs((const void *const *)V_dev, 3, "V");
+    if (!rc) rc = set_device(plan);
+    if (rc) return rc;
+    const int nr = nshells + 1;
+    if (nr > 1000) return fail(TG_ERR_ARG, "too many shells");
+    if (plan->g.dealias) HIPCHK(launch_dealias(plan->g, V_dev, 3, plan->stream));
+    for (int i = 0; i < nr * nr; i++) S[i] = 0.0;
+    std::vector<double> row(nr);
+    const int kin = max_radius_inside(plan->g);
+    for (int m = 1; m < nshells; m++) {
+        rc = DISPATCH_PREC(plan, run_giver_nlin)(plan, V_dev, plan->acc, radii[m - 1], radii[m], true, nullptr);
+        if (rc) return rc;
+        const double *A[3] = {plan->acc[0], plan->acc[1], plan->acc[2]};
+        const double *B[3] = {V_dev[0], V_dev[1], V_dev[2]};
+        rc = reduce_to_host(plan, RED_SHELL_ALL, A, B, 3, 0, radii, nr, 1, kin, 0, 0, row.data());
+        if (rc) return rc;
+        for (int n = 0; n < nr; n++) S[(size_t)m * nr + n] = -row[n];
+    }
+    return TG_OK;
+}
+
+int tg_init_field(tg_plan *plan, int kind, double k0, double kmin, double kmax, double energy, uint64_t seed,
+                  double *const V_dev[3])
+{
+    if (!plan || (kind != 0 && kind != 1) || !(energy >= 0)) return fail(TG_ERR_ARG, "bad argument");
+    int rc = check_ptrs((const void *const *)V_dev, 3, "V");
+    if (!rc) rc = set_device(plan);
+    if (rc) return rc;
+    plan->unext_valid = false;
+    HIPCHK(launch_init_field(plan->g, V_dev, kind, k0, kmin, kmax, seed, plan->stream));
+    HIPCHK(launch_reality(plan->g, V_dev, 3, plan->stream));
+    // energy of each component into small[1..3], target into small[0]
+    double *tgt = plan->small + 2048;
+    HIPCHK(hipMemcpyAsync(tgt, &energy, sizeof(double), hipMemcpyHostToDevice, plan->stream));
+    for (int c = 0; c < 3; c++) {
+        const double *A[1] = {V_dev[c]};
+        HIPCHK(launch_reduce(plan->g, RED_ENERGY, A, nullptr, 1, 0, nullptr, 1, 1, 0, 0, 0, plan->partial,
+                             plan->red_blocks, tgt + 1 + c, plan->stream));
+    }
+    HIPCHK(launch_scale_fields(plan->g, V_dev, tgt, plan->stream));
+    HIPCHK(hipStreamSynchronize(plan->stream));
+    return TG_OK;
+}
+
+}  // extern "C"

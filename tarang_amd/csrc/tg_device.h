@@ -1,0 +1,200 @@
+// tg_device.h -- device-side building blocks for the gfx950 pseudo-spectral path.
+//
+// Complex arithmetic, radix-2/4/8 butterflies and a register-resident
+// Stockham FFT for lines of L = 8*TPL points handled by TPL threads.
+//
+// Register convention used by every kernel in this library: thread j of a
+// line holds the 8 points at positions j + TPL*s, s = 0..7.  The first
+// radix-8 Stockham pass reads exactly these positions and the last pass
+// produces exactly these positions, so a forward transform can feed an
+// inverse transform (and pointwise epilogues) without touching LDS.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace tg {
+
+template <typename T>
+struct cpx {
+    T x, y;
+};
+
+template <typename T>
+__device__ __forceinline__ cpx<T> mk(T a, T b) { cpx<T> r; r.x = a; r.y = b; return r; }
+template <typename T>
+__device__ __forceinline__ cpx<T> operator+(cpx<T> a, cpx<T> b) { return mk<T>(a.x + b.x, a.y + b.y); }
+template <typename T>
+__device__ __forceinline__ cpx<T> operator-(cpx<T> a, cpx<T> b) { return mk<T>(a.x - b.x, a.y - b.y); }
+template <typename T>
+__device__ __forceinline__ cpx<T> operator*(cpx<T> a, cpx<T> b)
+{
+    return mk<T>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+template <typename T>
+__device__ __forceinline__ cpx<T> operator*(T s, cpx<T> a) { return mk<T>(s * a.x, s * a.y); }
+template <typename T>
+__device__ __forceinline__ cpx<T> conj(cpx<T> a) { return mk<T>(a.x, -a.y); }
+// multiply by i*s (s real): i*s*(x+iy) = -s*y + i*s*x
+template <typename T>
+__device__ __forceinline__ cpx<T> mul_i(T s, cpx<T> a) { return mk<T>(-s * a.y, s * a.x); }
+// a * conj(w) when dir>0 (inverse), a*w when dir<0 (forward); w from the forward table
+template <int DIR, typename T>
+__device__ __forceinline__ cpx<T> twmul(cpx<T> a, cpx<T> w)
+{
+    if (DIR < 0) return a * w;
+    return mk<T>(a.x * w.x + a.y * w.y, a.y * w.x - a.x * w.y);
+}
+// multiply by -i (forward) or +i (inverse)
+template <int DIR, typename T>
+__device__ __forceinline__ cpx<T> rot90(cpx<T> a)
+{
+    if (DIR < 0) return mk<T>(a.y, -a.x);
+    return mk<T>(-a.y, a.x);
+}
+
+template <typename T> struct consts;
+template <> struct consts<double> { static constexpr double r2 = 0.70710678118654752440; };
+template <> struct consts<float> { static constexpr float r2 = 0.70710678118654752440f; };
+
+// In-place DFT of size R on v[0..R-1], natural order output.
+template <int DIR, typename T>
+__device__ __forceinline__ void dft2(cpx<T> &a0, cpx<T> &a1)
+{
+    cpx<T> t = a0 - a1;
+    a0 = a0 + a1;
+    a1 = t;
+}
+
+template <int DIR, typename T>
+__device__ __forceinline__ void dft4(cpx<T> &a0, cpx<T> &a1, cpx<T> &a2, cpx<T> &a3)
+{
+    cpx<T> b0 = a0 + a2, b2 = a0 - a2, b1 = a1 + a3, b3 = rot90<DIR>(a1 - a3);
+    a0 = b0 + b1;
+    a2 = b0 - b1;
+    a1 = b2 + b3;
+    a3 = b2 - b3;
+}
+
+template <int DIR, typename T>
+__device__ __forceinline__ void dft8(cpx<T> *a)
+{
+    const T r = consts<T>::r2;
+    cpx<T> b0 = a[0] + a[4], b4 = a[0] - a[4];
+    cpx<T> b1 = a[1] + a[5], b5 = a[1] - a[5];
+    cpx<T> b2 = a[2] + a[6], b6 = a[2] - a[6];
+    cpx<T> b3 = a[3] + a[7], b7 = a[3] - a[7];
+    // b5 *= W8, b6 *= W8^2, b7 *= W8^3 with W8 = exp(DIR*i*pi/4)
+    if (DIR < 0) {
+        b5 = mk<T>(r * (b5.x + b5.y), r * (b5.y - b5.x));
+        b7 = mk<T>(r * (b7.y - b7.x), -r * (b7.x + b7.y));
+    } else {
+        b5 = mk<T>(r * (b5.x - b5.y), r * (b5.x + b5.y));
+        b7 = mk<T>(-r * (b7.x + b7.y), r * (b7.x - b7.y));
+    }
+    b6 = rot90<DIR>(b6);
+    cpx<T> c0 = b0 + b2, c2 = b0 - b2, c1 = b1 + b3, c3 = rot90<DIR>(b1 - b3);
+    cpx<T> c4 = b4 + b6, c6 = b4 - b6, c5 = b5 + b7, c7 = rot90<DIR>(b5 - b7);
+    a[0] = c0 + c1;
+    a[4] = c0 - c1;
+    a[2] = c2 + c3;
+    a[6] = c2 - c3;
+    a[1] = c4 + c5;
+    a[5] = c4 - c5;
+    a[3] = c6 + c7;
+    a[7] = c6 - c7;
+}
+
+template <int R, int DIR, typename T>
+__device__ __forceinline__ void dftR(cpx<T> *a)
+{
+    if (R == 8) dft8<DIR>(a);
+    else if (R == 4) dft4<DIR>(a[0], a[1], a[2], a[3]);
+    else if (R == 2) dft2<DIR>(a[0], a[1]);
+}
+
+// Radix of pass p for a line of L points: 8 while it divides, then the rest.
+__host__ __device__ constexpr int pass_radix(int L, int p)
+{
+    int rem = L;
+    for (int i = 0; i < p; i++) rem /= 8;
+    return rem >= 8 ? 8 : rem;
+}
+__host__ __device__ constexpr int num_passes(int L)
+{
+    int n = 0;
+    int rem = L;
+    while (rem > 1) { rem = rem >= 8 ? rem / 8 : 1; n++; }
+    return n;
+}
+
+// LDS position with one pad entry every 16 points (breaks the stride-8
+// write pattern of the first radix-8 pass across banks).
+__device__ __forceinline__ int lpad(int i) { return i + (i >> 4); }
+__host__ __device__ constexpr int lpad_len(int L) { return L + L / 16 + 1; }
+
+// One Stockham pass, radix R, stride NS (product of the earlier radices),
+// over a line of L points; thread j of TPL = L/8.  v[] holds positions
+// j + TPL*s on entry.  If LAST, the results stay in v[] at the same
+// positions; otherwise they are written to the LDS line buffer (the caller
+// synchronises and reloads).
+template <int L, int R, int NS, int DIR, bool LAST, typename T>
+__device__ __forceinline__ void stockham_pass(cpx<T> *v, cpx<T> *lds, const cpx<T> *__restrict__ tw, int j)
+{
+    constexpr int TPL = L / 8;
+    constexpr int Q = 8 / R;
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        cpx<T> a[R];
+        const int b = j + TPL * q;
+        const int k = b % NS;
+#pragma unroll
+        for (int r = 0; r < R; r++) a[r] = v[q + r * Q];
+        if constexpr (NS > 1) {
+            const int step = (L / (NS * R)) * k;
+#pragma unroll
+            for (int r = 1; r < R; r++) a[r] = twmul<DIR>(a[r], tw[r * step]);
+        }
+        dftR<R, DIR>(a);
+        if constexpr (LAST) {
+#pragma unroll
+            for (int r = 0; r < R; r++) v[q + r * Q] = a[r];
+        } else {
+            const int base = (b / NS) * NS * R + k;
+#pragma unroll
+            for (int r = 0; r < R; r++) lds[lpad(base + r * NS)] = a[r];
+        }
+    }
+}
+
+template <int L, int DIR, int P, int NS, typename T>
+__device__ __forceinline__ void fft_passes(cpx<T> *v, cpx<T> *lds, const cpx<T> *__restrict__ tw, int j)
+{
+    constexpr int NP = num_passes(L);
+    if constexpr (P < NP) {
+        constexpr int R = pass_radix(L, P);
+        constexpr bool LAST = (P == NP - 1);
+        stockham_pass<L, R, NS, DIR, LAST>(v, lds, tw, j);
+        if constexpr (!LAST) {
+            constexpr int TPL = L / 8;
+            __syncthreads();
+#pragma unroll
+            for (int s = 0; s < 8; s++) v[s] = lds[lpad(j + TPL * s)];
+            __syncthreads();
+            fft_passes<L, DIR, P + 1, NS * R>(v, lds, tw, j);
+        }
+    }
+}
+
+// Full line FFT of L = 8*TPL points (L a power of two, 8..4096), values in
+// registers at positions j + TPL*s.  Exponent sign DIR (-1 forward, +1
+// inverse), unnormalised.  tw[k] = exp(-2 pi i k / L).  Uses the LDS line
+// buffer (lpad_len(L) entries) for the exchanges between passes.  Every
+// thread of the block must call it (block-wide barriers); the caller must
+// have synchronised if the LDS line was in use before.
+template <int L, int DIR, typename T>
+__device__ __forceinline__ void fft_line(cpx<T> *v, cpx<T> *lds, const cpx<T> *__restrict__ tw, int j)
+{
+    fft_passes<L, DIR, 0, 1>(v, lds, tw, j);
+}
+
+}  // namespace tg

@@ -1,0 +1,26 @@
+// tg_diag.h -- reduction and initial-field launchers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "tg_kernels.h"
+
+namespace tg {
+
+enum {
+    RED_ENERGY = 0,          // sum w|A|^2, w = 1/2 on kz = 0
+    RED_SN = 1,              // sum m K^n |A|^2 (K <= kcut; origin skipped for n = 0)
+    RED_SPECTRUM = 2,        // bins ceil(K) <= kcut: [m K^n |A|^2 K^2, 2m]
+    RED_SHELL_ALL = 3,       // bins first radii >= K, K <= kcut: 2m Re(A.B*)
+    RED_SHELL_ALL_NOCUT = 4, // same without the K cut
+    RED_SHELL_SINGLE = 5,    // sum over r_lo < K <= r_hi of 2m Re(A.B*)
+};
+
+hipError_t launch_reduce(const Grid &g, int mode, const double *const *A, const double *const *B, int ncomp, int n,
+                         const double *radii_dev, int nbins, int nq, int kcut, double r_lo, double r_hi,
+                         double *partial, int nblocks, double *out_dev, hipStream_t st);
+
+hipError_t launch_init_field(const Grid &g, double *const *V, int kind, double k0, double kmin, double kmax,
+                             uint64_t seed, hipStream_t st);
+hipError_t launch_scale_fields(const Grid &g, double *const *V, const double *target_and_e, hipStream_t st);
+
+}  // namespace tg

@@ -1,0 +1,80 @@
+// tg_kernels.h -- kernel argument blocks and host-side launch helpers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace tg {
+
+// Grid description shared by every kernel (reference FFF basis conventions:
+// kx = lx <= Nx/2 ? lx : lx - Nx, likewise ky; kz = lz; K_d = k_d*kfactor[d]).
+struct Grid {
+    int nx, ny, nz, nzc;  // nzc = nz/2 + 1
+    double kf[3];
+    int dealias;          // 0 none, 1 reference corner box, 2 two-thirds rule
+    int pad;
+};
+
+// Forcing band slots: modes with |kx|,|ky| <= kb and 0 <= kz <= kb.
+struct Band {
+    int kind;             // 0 none, 1 constant energy supply alpha-forcing
+    int kb;               // slot box half-width
+    double r_in, r_out, eps;
+};
+
+__host__ __device__ inline int band_nslots(int kb) { return (2 * kb + 1) * (2 * kb + 1) * (kb + 1); }
+
+// Launch helpers (defined in tg_kernels.hip).  All return hipError_t.
+// T is float or double (transform precision).
+template <typename T>
+hipError_t launch_colfft(const Grid &g, int axis, int dir, void *const *fields, int nfields,
+                         const void *tw, hipStream_t st);
+
+// x-inverse of an FP64 spectral field set into the T work layout.
+//   flags: 1 = dealias on read, 2 = write band slots, 4 = reality patch
+//   columns only (kz = 0 and kz = Nz/2), 8 = keep only r_lo < K <= r_hi.
+template <typename T>
+hipError_t launch_xinv_state(const Grid &g, const Band &b, double *const *V, int ncomp, void *const *U,
+                             double *slots, int flags, const void *twx, hipStream_t st, double r_lo = 0.0,
+                             double r_hi = 0.0);
+
+// x-forward of T work fields into FP64 spectral outputs, scaled.
+template <typename T>
+hipError_t launch_xfwd_spec(const Grid &g, void *const *W, int ncomp, double *const *out, double scale,
+                            const void *twx, hipStream_t st);
+
+// z-line c2r of T work fields into FP64 real arrays (Nx,Ny,Nz+2).
+template <typename T>
+hipError_t launch_z_c2r(const Grid &g, void *const *W, int ncomp, double *const *R, const void *twm,
+                        const void *twn, hipStream_t st);
+
+// z-line r2c of FP64 real arrays into T work fields.
+template <typename T>
+hipError_t launch_z_r2c(const Grid &g, const double *const *R, int ncomp, void *const *W, const void *twm,
+                        const void *twn, hipStream_t st);
+
+// z pass of the NS nonlinear term: 3 velocity lines in, 6 product lines out
+// (order xx, yy, zz, xy, yz, zx).  giver != 0: 6 lines in (u then w),
+// 9 products out (u_j w_i, order j-major: (x,x),(x,y),(x,z),(y,x)...).
+template <typename T>
+hipError_t launch_z_products(const Grid &g, void *const *U, void *const *P, int giver, const void *twm,
+                             const void *twn, hipStream_t st);
+
+struct RkParams {
+    int stage;            // 1..4; 0 = write the nonlinear term only; -1 = giver
+                          // form (9 products u_j w_i), nonlinear term only
+    int write_rhs;        // stage 0: apply force/projection/negation (Compute_rhs)
+    double dt, fh, f1;    // fh = -nu*0.5*dt, f1 = -nu*1.0*dt
+    double scale;         // 1/(Nx Ny Nz)
+};
+
+// Fused x pass: forward x-FFT of the 6 (giver: 9) products, nonlinear term, RHS and the
+// Lawson RK4 stage algebra, next stage input, dealias, inverse x-FFT.
+template <typename T>
+hipError_t launch_x_nlin(const Grid &g, const Band &b, const RkParams &rk, void *const *P, double *const *V,
+                         double *const *acc, void *const *U, const double *slots_in, double *slots_out,
+                         double *const *out, const void *twx, hipStream_t st);
+
+hipError_t launch_reality(const Grid &g, double *const *V, int ncomp, hipStream_t st);
+hipError_t launch_dealias(const Grid &g, double *const *V, int ncomp, hipStream_t st);
+
+}  // namespace tg

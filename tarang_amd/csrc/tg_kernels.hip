@@ -1,0 +1,976 @@
+// tg_kernels.hip -- gfx950 kernels of the pseudo-spectral FFF hot path.
+//
+// HBM layouts (all complex fields interleaved re/im):
+//   spectral state  V(lx,ly,lz)        (Nx, Ny, Nz/2+1) FP64, kz fastest
+//   work fields     W(x | ly | lz ...) (Nx, Ny, Nz/2+1) in the transform
+//                   precision T; the same shape serves every intermediate
+//                   of the 3D transform (x, y and z passes work in place on
+//                   it, each pass finishing one axis).
+//   real fields     R(x,y,z)           (Nx, Ny, Nz+2) FP64, z fastest.
+//
+// Passes:
+//   x pass  tiles of all Nx rows x CB contiguous (ly,lz) columns, staged
+//           through LDS (coalesced CB-wide row segments), one line per
+//           TPL = Nx/8 threads.
+//   y pass  tiles of all Ny rows x CB contiguous lz columns of one lx.
+//   z pass  whole z-lines (contiguous), TPL = Nz/16 threads per line; a
+//           real line of Nz points is packed as Nz/2 complex points.
+#include "tg_device.h"
+#include "tg_kernels.h"
+
+namespace tg {
+
+static constexpr int kLdsBudget = 73728;  // bytes; two blocks fit the 160 KB LDS of a CU
+
+__device__ __forceinline__ int kwave(int l, int n) { return l <= n / 2 ? l : l - n; }
+
+__device__ __forceinline__ bool dealias_zero(const Grid &g, int lx, int ly, int lz)
+{
+    if (g.dealias == 1) {
+        // FFF_PENCIL::Dealias (FFF_pencil_basic.cc:87-90) via Assign_sub_array
+        // (:228-259): intersection of lx in [Nx/3+1, 2Nx/3-1],
+        // ly in [Ny/3+1, 2Ny/3-1], lz in [Nz/3+1, Nz/2]; an empty y/z range is
+        // clamped to its first index.
+        int ylo = g.ny / 3 + 1, yhi = 2 * g.ny / 3 - 1;
+        if (yhi < ylo) yhi = ylo;
+        int zlo = g.nz / 3 + 1, zhi = g.nz / 2;
+        if (zhi < zlo) zhi = zlo;
+        return (lx >= g.nx / 3 + 1 && lx <= 2 * g.nx / 3 - 1) && (ly >= ylo && ly <= yhi) && (lz >= zlo && lz <= zhi);
+    }
+    if (g.dealias == 2) {
+        int kx = abs(kwave(lx, g.nx)), ky = abs(kwave(ly, g.ny));
+        return kx > g.nx / 3 || ky > g.ny / 3 || lz > g.nz / 3;
+    }
+    return false;
+}
+
+template <typename T>
+__host__ __device__ constexpr int col_block(int L, int elem_bytes)
+{
+    int tpl = L / 8;
+    int cb = 16;
+    while (cb > 1 && (cb * tpl > 1024 || cb * lpad_len(L) * elem_bytes > kLdsBudget)) cb /= 2;
+    return cb;
+}
+
+template <typename A, typename B>
+__device__ __forceinline__ cpx<A> cvt(cpx<B> v) { return mk<A>((A)v.x, (A)v.y); }
+
+// ---------------------------------------------------------------------------
+// Column FFT along a strided axis (the y pass, or a plain x pass).
+// ---------------------------------------------------------------------------
+struct ColArgs {
+    void *f[9];
+    const void *tw;
+    long long outer_stride;  // elements between tiles of different outer index
+    long long row_stride;    // elements between consecutive rows (transform axis)
+    int ncol;                // contiguous columns per row
+    int nouter;
+};
+
+template <typename T, int L, int DIR, int CB>
+__global__ void __launch_bounds__(CB *(L / 8)) k_colfft(ColArgs a)
+{
+    constexpr int TPL = L / 8;
+    constexpr int LP = lpad_len(L);
+    __shared__ cpx<T> lds[CB * LP];
+    const int nblk = (a.ncol + CB - 1) / CB;
+    const int outer = blockIdx.x / nblk;
+    const int c0 = (blockIdx.x % nblk) * CB;
+    cpx<T> *f = (cpx<T> *)a.f[blockIdx.y] + (size_t)outer * a.outer_stride;
+    const cpx<T> *tw = (const cpx<T> *)a.tw;
+    for (int e = threadIdx.x; e < L * CB; e += CB * TPL) {
+        const int r = e / CB, c = e % CB, col = c0 + c;
+        cpx<T> v = mk<T>(0, 0);
+        if (col < a.ncol) v = f[(size_t)r * a.row_stride + col];
+        lds[c * LP + lpad(r)] = v;
+    }
+    __syncthreads();
+    const int c = threadIdx.x / TPL, j = threadIdx.x % TPL;
+    cpx<T> v[8];
+#pragma unroll
+    for (int s = 0; s < 8; s++) v[s] = lds[c * LP + lpad(j + TPL * s)];
+    __syncthreads();
+    fft_line<L, DIR>(v, lds + c * LP, tw, j);
+#pragma unroll
+    for (int s = 0; s < 8; s++) lds[c * LP + lpad(j + TPL * s)] = v[s];
+    __syncthreads();
+    for (int e = threadIdx.x; e < L * CB; e += CB * TPL) {
+        const int r = e / CB, cc = e % CB, col = c0 + cc;
+        if (col < a.ncol) f[(size_t)r * a.row_stride + col] = lds[cc * LP + lpad(r)];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// x pass helpers: staging of an (Nx x CB) column tile through LDS.
+// ---------------------------------------------------------------------------
+template <int L, int CB, typename S, typename G>
+__device__ __forceinline__ void stage_in(cpx<S> *lds, const cpx<G> *__restrict__ f, const int *cols, int ncol_total)
+{
+    constexpr int TPL = L / 8;
+    constexpr int LP = lpad_len(L);
+    for (int e = threadIdx.x; e < L * CB; e += CB * TPL) {
+        const int r = e / CB, c = e % CB, col = cols[c];
+        cpx<S> v = mk<S>(0, 0);
+        if (col >= 0) v = cvt<S>(f[(size_t)r * ncol_total + col]);
+        lds[c * LP + lpad(r)] = v;
+    }
+}
+
+template <int L, int CB, typename S, typename G>
+__device__ __forceinline__ void stage_out(const cpx<S> *lds, cpx<G> *__restrict__ f, const int *cols, int ncol_total)
+{
+    constexpr int TPL = L / 8;
+    constexpr int LP = lpad_len(L);
+    for (int e = threadIdx.x; e < L * CB; e += CB * TPL) {
+        const int r = e / CB, c = e % CB, col = cols[c];
+        if (col >= 0) f[(size_t)r * ncol_total + col] = cvt<G>(lds[c * LP + lpad(r)]);
+    }
+}
+
+// Column of the tile handled by line c of this block: contiguous blocks of
+// CB columns, or (patch) the kz = 0 and kz = Nz/2 columns of every ky.
+template <int CB>
+__device__ __forceinline__ void tile_columns(int *cols, const Grid &g, bool patch)
+{
+    if (threadIdx.x < CB) {
+        const int c = threadIdx.x;
+        int col;
+        if (!patch) {
+            col = blockIdx.x * CB + c;
+            if (col >= g.ny * g.nzc) col = -1;
+        } else {
+            const int p = blockIdx.x * CB + c;
+            col = p < 2 * g.ny ? (p >> 1) * g.nzc + ((p & 1) ? g.nz / 2 : 0) : -1;
+        }
+        cols[c] = col;
+    }
+}
+
+__device__ __forceinline__ int band_slot(const Band &b, int kx, int ky, int kz)
+{
+    if (kx < -b.kb || kx > b.kb || ky < -b.kb || ky > b.kb || kz > b.kb) return -1;
+    return ((kx + b.kb) * (2 * b.kb + 1) + (ky + b.kb)) * (b.kb + 1) + kz;
+}
+
+__device__ __forceinline__ double kmag(const Grid &g, int kx, int ky, int kz)
+{
+    const double a = kx * g.kf[0], b = ky * g.kf[1], c = kz * g.kf[2];
+    return sqrt(a * a + b * b + c * c);
+}
+
+// ---------------------------------------------------------------------------
+// x-inverse of FP64 spectral fields into T work fields (first pass of the
+// 3D inverse transform; stage-1 input of an RK4 step).
+// ---------------------------------------------------------------------------
+struct XinvArgs {
+    double *V[3];
+    void *U[3];
+    double *slots;
+    const void *tw;
+    Grid g;
+    Band b;
+    int ncomp, flags;
+    double r_lo, r_hi;  // flags & 8: keep only r_lo < K <= r_hi (Fill_array_shell)
+};
+
+template <typename T, int L, int CB>
+__global__ void __launch_bounds__(CB *(L / 8)) k_xinv_state(XinvArgs a)
+{
+    constexpr int TPL = L / 8;
+    constexpr int LP = lpad_len(L);
+    __shared__ __attribute__((aligned(16))) char raw[CB * LP * sizeof(cpx<double>)];
+    __shared__ int cols[CB];
+    cpx<double> *ldd = (cpx<double> *)raw;
+    cpx<T> *ldt = (cpx<T> *)raw;
+    const Grid &g = a.g;
+    tile_columns<CB>(cols, g, (a.flags & 4) != 0);
+    __syncthreads();
+    const int c = threadIdx.x / TPL, j = threadIdx.x % TPL;
+    const int col = cols[c];
+    const int ly = col >= 0 ? col / g.nzc : 0, lz = col >= 0 ? col % g.nzc : 0;
+    const int ky = kwave(ly, g.ny);
+    const cpx<T> *tw = (const cpx<T> *)a.tw;
+    const int ncol = g.ny * g.nzc;
+    for (int comp = 0; comp < a.ncomp; comp++) {
+        stage_in<L, CB>(ldd, (const cpx<double> *)a.V[comp], cols, ncol);
+        __syncthreads();
+        cpx<T> v[8];
+#pragma unroll
+        for (int s = 0; s < 8; s++) {
+            const int lx = j + TPL * s;
+            cpx<double> d = ldd[c * LP + lpad(lx)];
+            if (col >= 0 && (a.flags & 2)) {
+                const int kx = kwave(lx, g.nx);
+                const int sl = band_slot(a.b, kx, ky, lz);
+                if (sl >= 0) {
+                    const double K = kmag(g, kx, ky, lz);
+                    if (K > a.b.r_in && K <= a.b.r_out) ((cpx<double> *)a.slots)[sl * 3 + comp] = d;
+                }
+            }
+            if ((a.flags & 1) && dealias_zero(g, lx, ly, lz)) d = mk<double>(0, 0);
+            if (a.flags & 8) {
+                const double K = kmag(g, kwave(lx, g.nx), ky, lz);
+                if (!(K > a.r_lo && K <= a.r_hi)) d = mk<double>(0, 0);
+            }
+            v[s] = cvt<T>(d);
+        }
+        __syncthreads();
+        fft_line<L, +1>(v, ldt + c * LP, tw, j);
+#pragma unroll
+        for (int s = 0; s < 8; s++) ldt[c * LP + lpad(j + TPL * s)] = v[s];
+        __syncthreads();
+        stage_out<L, CB>(ldt, (cpx<T> *)a.U[comp], cols, ncol);
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// x-forward of T work fields into FP64 spectral outputs (last pass of the 3D
+// forward transform), scaled.
+// ---------------------------------------------------------------------------
+struct XfwdArgs {
+    void *W[3];
+    double *out[3];
+    const void *tw;
+    Grid g;
+    double scale;
+    int ncomp;
+};
+
+template <typename T, int L, int CB>
+__global__ void __launch_bounds__(CB *(L / 8)) k_xfwd_spec(XfwdArgs a)
+{
+    constexpr int TPL = L / 8;
+    constexpr int LP = lpad_len(L);
+    __shared__ __attribute__((aligned(16))) char raw[CB * LP * sizeof(cpx<double>)];
+    __shared__ int cols[CB];
+    cpx<double> *ldd = (cpx<double> *)raw;
+    cpx<T> *ldt = (cpx<T> *)raw;
+    tile_columns<CB>(cols, a.g, false);
+    __syncthreads();
+    const int c = threadIdx.x / TPL, j = threadIdx.x % TPL;
+    const cpx<T> *tw = (const cpx<T> *)a.tw;
+    const int ncol = a.g.ny * a.g.nzc;
+    for (int comp = 0; comp < a.ncomp; comp++) {
+        stage_in<L, CB>(ldt, (const cpx<T> *)a.W[comp], cols, ncol);
+        __syncthreads();
+        cpx<T> v[8];
+#pragma unroll
+        for (int s = 0; s < 8; s++) v[s] = ldt[c * LP + lpad(j + TPL * s)];
+        __syncthreads();
+        fft_line<L, -1>(v, ldt + c * LP, tw, j);
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < 8; s++) ldd[c * LP + lpad(j + TPL * s)] = a.scale * cvt<double>(v[s]);
+        __syncthreads();
+        stage_out<L, CB>(ldd, (cpx<double> *)a.out[comp], cols, ncol);
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// z-line transforms.  A real line x[0..N) (N = 2M) is packed as
+// z[n] = x[2n] + i x[2n+1]; a complex line holds X[0..M].
+// c2r: Z'[k] = (X[k] + conj X[M-k]) + i W^-k (X[k] - conj X[M-k]), with the
+//      imaginary parts of X[0], X[M] dropped; z = IDFT_M(Z') (unscaled), so
+//      x = sum_k X_herm[k] e^{+2 pi i k n / N}.
+// r2c: Z = DFT_M(z); X[k] = ((Z[k] + conj Z[M-k]) - i W^k (Z[k] - conj Z[M-k]))/2.
+// W = exp(-2 pi i / N); twn holds W^k for k = 0..M.
+// ---------------------------------------------------------------------------
+template <typename T, int M>
+__device__ __forceinline__ void zline_c2r(cpx<T> *v, const cpx<T> *__restrict__ X, cpx<T> *lds,
+                                          const cpx<T> *__restrict__ twm, const cpx<T> *__restrict__ twn, int j)
+{
+    constexpr int TPL = M / 8;
+    // stage the line (M+1 points) in LDS
+#pragma unroll
+    for (int s = 0; s < 8; s++) lds[lpad(j + TPL * s)] = X[j + TPL * s];
+    if (j == 0) lds[lpad(M)] = X[M];
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < 8; s++) {
+        const int k = j + TPL * s;
+        cpx<T> xk = lds[lpad(k)];
+        cpx<T> xm = lds[lpad(M - k)];
+        if (k == 0) { xk.y = 0; xm.y = 0; }
+        const cpx<T> a = xk + conj(xm);
+        const cpx<T> d = xk - conj(xm);
+        const cpx<T> w = twn[k];  // W^k ; we need W^-k = conj(w)
+        const cpx<T> e = mk<T>(d.x * w.x + d.y * w.y, d.y * w.x - d.x * w.y);
+        v[s] = a + mk<T>(-e.y, e.x);
+    }
+    __syncthreads();
+    fft_line<M, +1>(v, lds, twm, j);
+}
+
+template <typename T, int M>
+__device__ __forceinline__ void zline_r2c_post(cpx<T> *v, cpx<T> *__restrict__ X, cpx<T> *lds,
+                                               const cpx<T> *__restrict__ twm, const cpx<T> *__restrict__ twn, int j,
+                                               bool store)
+{
+    constexpr int TPL = M / 8;
+    fft_line<M, -1>(v, lds, twm, j);
+#pragma unroll
+    for (int s = 0; s < 8; s++) lds[lpad(j + TPL * s)] = v[s];
+    __syncthreads();
+    const T h = (T)0.5;
+#pragma unroll
+    for (int s = 0; s < 8; s++) {
+        const int k = j + TPL * s;
+        const cpx<T> zk = v[s];
+        const cpx<T> zm = lds[lpad((M - k) & (M - 1))];
+        const cpx<T> a = zk + conj(zm);
+        const cpx<T> d = zk - conj(zm);
+        const cpx<T> w = twn[k];
+        const cpx<T> e = d * w;  // W^k d
+        if (store) X[k] = mk<T>(h * (a.x + e.y), h * (a.y - e.x));
+    }
+    if (store && j == 0) X[M] = mk<T>(v[0].x - v[0].y, (T)0);
+    __syncthreads();
+}
+
+struct ZArgs {
+    void *in[6];
+    void *out[9];
+    const void *twm, *twn;
+    Grid g;
+    int ncomp;
+};
+
+template <int M>
+__host__ __device__ constexpr int z_lines_per_block()
+{
+    return (M / 8) >= 256 ? 1 : 256 / (M / 8);
+}
+
+template <typename T, int M>
+__global__ void __launch_bounds__(256) k_z_c2r(ZArgs a)
+{
+    constexpr int TPL = M / 8;
+    constexpr int LPB = z_lines_per_block<M>();
+    constexpr int LP = lpad_len(M);
+    __shared__ cpx<T> lds[LPB * LP];
+    const int li = threadIdx.x / TPL, j = threadIdx.x % TPL;
+    const size_t line = (size_t)blockIdx.x * LPB + li;
+    const size_t nlines = (size_t)a.g.nx * a.g.ny;
+    const size_t lsafe = line < nlines ? line : nlines - 1;
+    for (int comp = 0; comp < a.ncomp; comp++) {
+        const cpx<T> *X = (const cpx<T> *)a.in[comp] + lsafe * a.g.nzc;
+        cpx<T> v[8];
+        zline_c2r<T, M>(v, X, lds + li * LP, (const cpx<T> *)a.twm, (const cpx<T> *)a.twn, j);
+        if (line < nlines) {
+            double *R = (double *)a.out[comp] + line * (size_t)(2 * M + 2);
+#pragma unroll
+            for (int s = 0; s < 8; s++) {
+                const int n = j + TPL * s;
+                ((cpx<double> *)R)[n] = cvt<double>(v[s]);
+            }
+            if (j == 0) ((cpx<double> *)R)[M] = mk<double>(0, 0);
+        }
+        __syncthreads();
+    }
+}
+
+template <typename T, int M>
+__global__ void __launch_bounds__(256) k_z_r2c(ZArgs a)
+{
+    constexpr int TPL = M / 8;
+    constexpr int LPB = z_lines_per_block<M>();
+    constexpr int LP = lpad_len(M);
+    __shared__ cpx<T> lds[LPB * LP];
+    const int li = threadIdx.x / TPL, j = threadIdx.x % TPL;
+    const size_t line = (size_t)blockIdx.x * LPB + li;
+    const size_t nlines = (size_t)a.g.nx * a.g.ny;
+    const size_t lsafe = line < nlines ? line : nlines - 1;
+    for (int comp = 0; comp < a.ncomp; comp++) {
+        const cpx<double> *R = (const cpx<double> *)((const double *)a.in[comp] + lsafe * (size_t)(2 * M + 2));
+        cpx<T> v[8];
+#pragma unroll
+        for (int s = 0; s < 8; s++) v[s] = cvt<T>(R[j + TPL * s]);
+        cpx<T> *X = (cpx<T> *)a.out[comp] + lsafe * a.g.nzc;
+        zline_r2c_post<T, M>(v, X, lds + li * LP, (const cpx<T> *)a.twm, (const cpx<T> *)a.twn, j, line < nlines);
+    }
+}
+
+// z pass of the nonlinear term.  NS: products u_a u_b for the pairs
+// (0,0),(1,1),(2,2),(0,1),(1,2),(2,0) -- the six real-space products of
+// Compute_nlin_diag / Compute_nlin_offdiag (compute_diag_offdiag.cc:47-112).
+// Giver: out[3*i + j] = u_j w_i (compute_diag_offdiag.cc:329-414).
+template <typename T, int M, bool GIVER>
+__global__ void __launch_bounds__(256) k_z_products(ZArgs a)
+{
+    constexpr int TPL = M / 8;
+    constexpr int LPB = z_lines_per_block<M>();
+    constexpr int LP = lpad_len(M);
+    __shared__ cpx<T> lds[LPB * LP];
+    const int li = threadIdx.x / TPL, j = threadIdx.x % TPL;
+    const size_t line = (size_t)blockIdx.x * LPB + li;
+    const size_t nlines = (size_t)a.g.nx * a.g.ny;
+    const bool valid = line < nlines;
+    const size_t lsafe = valid ? line : nlines - 1;
+    const cpx<T> *twm = (const cpx<T> *)a.twm, *twn = (const cpx<T> *)a.twn;
+    cpx<T> *L = lds + li * LP;
+    cpx<T> u[3][8];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        zline_c2r<T, M>(u[c], (const cpx<T> *)a.in[c] + lsafe * a.g.nzc, L, twm, twn, j);
+        __syncthreads();
+    }
+    auto emit = [&](cpx<T> *p, int o) {
+        cpx<T> *X = (cpx<T> *)a.out[o] + lsafe * a.g.nzc;
+        zline_r2c_post<T, M>(p, X, L, twm, twn, j, valid);
+    };
+    if (!GIVER) {
+        const int pa[6] = {0, 1, 2, 0, 1, 2};
+        const int pb[6] = {0, 1, 2, 1, 2, 0};
+#pragma unroll
+        for (int o = 0; o < 6; o++) {
+            cpx<T> p[8];
+#pragma unroll
+            for (int s = 0; s < 8; s++) p[s] = mk<T>(u[pa[o]][s].x * u[pb[o]][s].x, u[pa[o]][s].y * u[pb[o]][s].y);
+            emit(p, o);
+        }
+    } else {
+#pragma unroll 1
+        for (int i = 0; i < 3; i++) {
+            cpx<T> w[8];
+            zline_c2r<T, M>(w, (const cpx<T> *)a.in[3 + i] + lsafe * a.g.nzc, L, twm, twn, j);
+            __syncthreads();
+#pragma unroll
+            for (int jj = 0; jj < 3; jj++) {
+                cpx<T> p[8];
+#pragma unroll
+                for (int s = 0; s < 8; s++) p[s] = mk<T>(u[jj][s].x * w[s].x, u[jj][s].y * w[s].y);
+                emit(p, 3 * i + jj);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Fused x pass of the nonlinear term and the Lawson RK4 stage.
+// ---------------------------------------------------------------------------
+struct XnlArgs {
+    void *P[9];
+    double *V[3];
+    double *acc[3];
+    void *U[3];
+    double *out[3];
+    const double *slots_in;
+    double *slots_out;
+    const void *tw;
+    Grid g;
+    Band b;
+    RkParams rk;
+};
+
+__device__ double band_alpha(const Grid &g, const Band &b, const double *slots)
+{
+    // alpha = eps / (2 E_band) with E_band = sum w |V|^2 over the band modes,
+    // w = 1/2 on kz = 0 (Get_local_energy weights, FFF_pencil_energy.cc:59-67)
+    const int n = band_nslots(b.kb);
+    double e = 0.0;
+    for (int s = 0; s < n; s++) {
+        const int kz = s % (b.kb + 1);
+        const int ky = (s / (b.kb + 1)) % (2 * b.kb + 1) - b.kb;
+        const int kx = s / ((b.kb + 1) * (2 * b.kb + 1)) - b.kb;
+        const double K = kmag(g, kx, ky, kz);
+        if (K > b.r_in && K <= b.r_out) {
+            const double w = kz == 0 ? 0.5 : 1.0;
+            for (int c = 0; c < 3; c++) {
+                const double re = slots[2 * (s * 3 + c)], im = slots[2 * (s * 3 + c) + 1];
+                e += w * (re * re + im * im);
+            }
+        }
+    }
+    return e > 0.0 ? b.eps / (2.0 * e) : 0.0;
+}
+
+// Order in which the product spectra are folded into N, following the
+// reference's diag/offdiag accumulation (compute_diag_offdiag.cc:47-112 for
+// NS, :329-414 for the giver form with P[3i+j] = FT[u_j w_i]).
+__constant__ int kNsOrder[6] = {0, 1, 2, 3, 4, 5};
+__constant__ int kGiverOrder[9] = {0, 4, 8, 3, 1, 7, 5, 2, 6};
+
+template <typename T, int L, int CB, bool GIVER>
+__global__ void __launch_bounds__(CB *(L / 8)) k_x_nlin(XnlArgs a)
+{
+    constexpr int TPL = L / 8;
+    constexpr int LP = lpad_len(L);
+    __shared__ __attribute__((aligned(16))) char raw[CB * LP * sizeof(cpx<double>)];
+    __shared__ int cols[CB];
+    cpx<double> *ldd = (cpx<double> *)raw;
+    cpx<T> *ldt = (cpx<T> *)raw;
+    const Grid &g = a.g;
+    tile_columns<CB>(cols, g, false);
+    __syncthreads();
+    const int c = threadIdx.x / TPL, j = threadIdx.x % TPL;
+    const int col = cols[c];
+    const int ly = col >= 0 ? col / g.nzc : 0, lz = col >= 0 ? col % g.nzc : 0;
+    const int ky = kwave(ly, g.ny);
+    const double Ky = ky * g.kf[1], Kz = lz * g.kf[2];
+    const cpx<T> *tw = (const cpx<T> *)a.tw;
+    const int ncol = g.ny * g.nzc;
+    const T sc = (T)a.rk.scale;
+    const T Kyt = (T)Ky, Kzt = (T)Kz;
+
+    // --- forward x-FFT of the six products and the divergence-form sum ---
+    cpx<T> N[3][8];
+    constexpr int NPROD = GIVER ? 9 : 6;
+#pragma unroll 1
+    for (int pi = 0; pi < NPROD; pi++) {
+        const int p = GIVER ? kGiverOrder[pi] : kNsOrder[pi];
+        stage_in<L, CB>(ldt, (const cpx<T> *)a.P[p], cols, ncol);
+        __syncthreads();
+        cpx<T> v[8];
+#pragma unroll
+        for (int s = 0; s < 8; s++) v[s] = ldt[c * LP + lpad(j + TPL * s)];
+        __syncthreads();
+        fft_line<L, -1>(v, ldt + c * LP, tw, j);
+#pragma unroll
+        for (int s = 0; s < 8; s++) {
+            const T Kx = (T)(kwave(j + TPL * s, g.nx) * g.kf[0]);
+            const cpx<T> q = sc * v[s];
+            if (!GIVER) {
+                // Compute_nlin_diag then Compute_nlin_offdiag accumulation order
+                switch (p) {
+                    case 0: N[0][s] = mul_i(Kx, q); break;
+                    case 1: N[1][s] = mul_i(Kyt, q); break;
+                    case 2: N[2][s] = mul_i(Kzt, q); break;
+                    case 3: N[0][s] = N[0][s] + mul_i(Kyt, q); N[1][s] = N[1][s] + mul_i(Kx, q); break;
+                    case 4: N[1][s] = N[1][s] + mul_i(Kzt, q); N[2][s] = N[2][s] + mul_i(Kyt, q); break;
+                    default: N[0][s] = N[0][s] + mul_i(Kzt, q); N[2][s] = N[2][s] + mul_i(Kx, q); break;
+                }
+            } else {
+                // N_i += i K_j FT[u_j w_i] for p = 3i + j
+                const int ii = p / 3, jj = p % 3;
+                const T Kj = jj == 0 ? Kx : (jj == 1 ? Kyt : Kzt);
+                const cpx<T> t = mul_i(Kj, q);
+                if (pi < 3) {
+                    if (ii == 0) N[0][s] = t;
+                    else if (ii == 1) N[1][s] = t;
+                    else N[2][s] = t;
+                } else {
+                    if (ii == 0) N[0][s] = N[0][s] + t;
+                    else if (ii == 1) N[1][s] = N[1][s] + t;
+                    else N[2][s] = N[2][s] + t;
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+    const int stage = a.rk.stage;
+    const bool want_rhs = stage > 0 || a.rk.write_rhs;
+    cpx<double> R[3][8];
+    // --- force, pressure projection, negation (Compute_force_TO_rhs) ---
+    double alpha = 0.0;
+    bool alpha_done = false;
+#pragma unroll
+    for (int s = 0; s < 8; s++) {
+        const int lx = j + TPL * s;
+        const int kx = kwave(lx, g.nx);
+        const double Kx = kx * g.kf[0];
+        cpx<double> n0 = cvt<double>(N[0][s]), n1 = cvt<double>(N[1][s]), n2 = cvt<double>(N[2][s]);
+        if (want_rhs) {
+            if (a.b.kind == 1 && col >= 0) {
+                const int sl = band_slot(a.b, kx, ky, lz);
+                if (sl >= 0) {
+                    const double K = kmag(g, kx, ky, lz);
+                    if (K > a.b.r_in && K <= a.b.r_out) {
+                        if (!alpha_done) { alpha = band_alpha(g, a.b, a.slots_in); alpha_done = true; }
+                        const cpx<double> *sv = (const cpx<double> *)a.slots_in + sl * 3;
+                        n0 = n0 - alpha * sv[0];
+                        n1 = n1 - alpha * sv[1];
+                        n2 = n2 - alpha * sv[2];
+                    }
+                }
+            }
+            cpx<double> div = mul_i(Kx, n0);
+            div = div + mul_i(Ky, n1);
+            div = div + mul_i(Kz, n2);
+            const double kxs = Kx * Kx;
+            const double ks = (kxs + Ky * Ky) + Kz * Kz;
+            cpx<double> pr = mk<double>(0, 0);
+            if (!(lx == 0 && ly == 0 && lz == 0)) pr = mk<double>(div.x / ks, div.y / ks);
+            n0 = n0 + mul_i(Kx, pr);
+            n1 = n1 + mul_i(Ky, pr);
+            n2 = n2 + mul_i(Kz, pr);
+            n0 = mk<double>(-n0.x, -n0.y);
+            n1 = mk<double>(-n1.x, -n1.y);
+            n2 = mk<double>(-n2.x, -n2.y);
+        }
+        R[0][s] = n0;
+        R[1][s] = n1;
+        R[2][s] = n2;
+    }
+
+    if (stage == 0) {
+        // Compute_nlin (and optionally Compute_rhs) output, FP64
+#pragma unroll 1
+        for (int comp = 0; comp < 3; comp++) {
+#pragma unroll
+            for (int s = 0; s < 8; s++) ldd[c * LP + lpad(j + TPL * s)] = R[comp][s];
+            __syncthreads();
+            stage_out<L, CB>(ldd, (cpx<double> *)a.out[comp], cols, ncol);
+            __syncthreads();
+        }
+        return;
+    }
+
+    // --- Lawson RK4 stage algebra (RK.cc:91-170, Tadvance_functions.cc:298-324,566-570) ---
+    const double dt = a.rk.dt;
+    const double c_half = 0.5 * dt, c_one = 1.0 * dt;
+    const double c_sixth = (1.0 / 6) * dt, c_third = (1.0 / 3) * dt;
+#pragma unroll 1
+    for (int comp = 0; comp < 3; comp++) {
+        cpx<double> v0[8], ac[8], vn[8];
+        stage_in<L, CB>(ldd, (const cpx<double> *)a.V[comp], cols, ncol);
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < 8; s++) v0[s] = ldd[c * LP + lpad(j + TPL * s)];
+        __syncthreads();
+        if (stage >= 2) {
+            stage_in<L, CB>(ldd, (const cpx<double> *)a.acc[comp], cols, ncol);
+            __syncthreads();
+#pragma unroll
+            for (int s = 0; s < 8; s++) ac[s] = ldd[c * LP + lpad(j + TPL * s)];
+            __syncthreads();
+        }
+#pragma unroll
+        for (int s = 0; s < 8; s++) {
+            const int lx = j + TPL * s;
+            const double Kx = kwave(lx, g.nx) * g.kf[0];
+            const double kxs = Kx * Kx;
+            const double ks = (kxs + Ky * Ky) + Kz * Kz;
+            const double Eh = exp(a.rk.fh * ks);
+            const cpx<double> r = R[comp][s];
+            if (stage == 1) {
+                cpx<double> d0 = v0[s];
+                if (dealias_zero(g, lx, ly, lz)) d0 = mk<double>(0, 0);
+                vn[s] = Eh * (d0 + c_half * r);
+                const double E1 = exp(a.rk.f1 * ks);
+                ac[s] = c_sixth * (E1 * r);
+            } else if (stage == 2) {
+                vn[s] = Eh * v0[s] + c_half * r;
+                ac[s] = ac[s] + c_third * (Eh * r);
+            } else if (stage == 3) {
+                vn[s] = Eh * (Eh * v0[s] + c_one * r);
+                ac[s] = ac[s] + c_third * (Eh * r);
+            } else {
+                ac[s] = ac[s] + c_sixth * r;
+                const double E1 = exp(a.rk.f1 * ks);
+                vn[s] = E1 * v0[s] + ac[s];
+            }
+        }
+        // accumulator (stages 1-3) or the new state (stage 4)
+#pragma unroll
+        for (int s = 0; s < 8; s++) ldd[c * LP + lpad(j + TPL * s)] = stage == 4 ? vn[s] : ac[s];
+        __syncthreads();
+        stage_out<L, CB>(ldd, (cpx<double> *)(stage == 4 ? a.V[comp] : a.acc[comp]), cols, ncol);
+        __syncthreads();
+        // next stage input: band slots, dealias, inverse x-FFT
+        cpx<T> v[8];
+#pragma unroll
+        for (int s = 0; s < 8; s++) {
+            const int lx = j + TPL * s;
+            cpx<double> d = vn[s];
+            if (a.b.kind == 1 && col >= 0) {
+                const int kx = kwave(lx, g.nx);
+                const int sl = band_slot(a.b, kx, ky, lz);
+                if (sl >= 0) {
+                    const double K = kmag(g, kx, ky, lz);
+                    if (K > a.b.r_in && K <= a.b.r_out) ((cpx<double> *)a.slots_out)[sl * 3 + comp] = d;
+                }
+            }
+            if (dealias_zero(g, lx, ly, lz)) d = mk<double>(0, 0);
+            v[s] = cvt<T>(d);
+        }
+        fft_line<L, +1>(v, ldt + c * LP, tw, j);
+#pragma unroll
+        for (int s = 0; s < 8; s++) ldt[c * LP + lpad(j + TPL * s)] = v[s];
+        __syncthreads();
+        stage_out<L, CB>(ldt, (cpx<T> *)a.U[comp], cols, ncol);
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Strong reality condition (FFF_pencil_basic.cc:139-179) and dealias.
+// ---------------------------------------------------------------------------
+struct PlaneArgs {
+    double *V[3];
+    Grid g;
+    int ncomp;
+};
+
+__global__ void k_reality(PlaneArgs a)
+{
+    const Grid &g = a.g;
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= g.nx * g.ny) return;
+    const int lx = idx / g.ny, ly = idx % g.ny;
+    const int ky = kwave(ly, g.ny);
+    for (int comp = 0; comp < a.ncomp; comp++) {
+        cpx<double> *A = (cpx<double> *)a.V[comp];
+        const size_t t = ((size_t)lx * g.ny + ly) * g.nzc;
+        int src = -1;
+        if (lx > g.nx / 2 && ky != g.ny / 2) {
+            const int mkx = -kwave(lx, g.nx);
+            const int mky = -ky;
+            const int iy = mky >= 0 ? mky : mky + g.ny;
+            src = mkx * g.ny + iy;
+        } else if (lx == 0 && ky < 0) {
+            src = -ky;
+        }
+        if (src >= 0) A[t] = conj(A[(size_t)src * g.nzc]);
+        A[t + g.nzc - 1] = mk<double>(0, 0);
+    }
+}
+
+__global__ void k_dealias(PlaneArgs a)
+{
+    const Grid &g = a.g;
+    const size_t n = (size_t)g.nx * g.ny * g.nzc;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int lz = i % g.nzc;
+        const int ly = (i / g.nzc) % g.ny;
+        const int lx = i / ((size_t)g.nzc * g.ny);
+        if (dealias_zero(g, lx, ly, lz))
+            for (int comp = 0; comp < a.ncomp; comp++) ((cpx<double> *)a.V[comp])[i] = mk<double>(0, 0);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Launch helpers with size dispatch.
+// ---------------------------------------------------------------------------
+#define TG_DISPATCH_L(Lval, CALL) \
+    switch (Lval) {               \
+        case 8: { constexpr int LL = 8; CALL; } break;     \
+        case 16: { constexpr int LL = 16; CALL; } break;   \
+        case 32: { constexpr int LL = 32; CALL; } break;   \
+        case 64: { constexpr int LL = 64; CALL; } break;   \
+        case 128: { constexpr int LL = 128; CALL; } break; \
+        case 256: { constexpr int LL = 256; CALL; } break; \
+        case 512: { constexpr int LL = 512; CALL; } break; \
+        case 1024: { constexpr int LL = 1024; CALL; } break; \
+        default: return hipErrorInvalidValue;              \
+    }
+
+template <typename T>
+hipError_t launch_colfft(const Grid &g, int axis, int dir, void *const *fields, int nfields, const void *tw,
+                         hipStream_t st)
+{
+    ColArgs a{};
+    for (int i = 0; i < nfields; i++) a.f[i] = fields[i];
+    a.tw = tw;
+    int L;
+    if (axis == 1) {
+        L = g.ny;
+        a.nouter = g.nx;
+        a.outer_stride = (long long)g.ny * g.nzc;
+        a.row_stride = g.nzc;
+        a.ncol = g.nzc;
+    } else {
+        L = g.nx;
+        a.nouter = 1;
+        a.outer_stride = 0;
+        a.row_stride = (long long)g.ny * g.nzc;
+        a.ncol = g.ny * g.nzc;
+    }
+    TG_DISPATCH_L(L, {
+        constexpr int CB = col_block<T>(LL, sizeof(cpx<T>));
+        dim3 grid(a.nouter * ((a.ncol + CB - 1) / CB), nfields);
+        if (dir < 0) hipLaunchKernelGGL((k_colfft<T, LL, -1, CB>), grid, dim3(CB * (LL / 8)), 0, st, a);
+        else hipLaunchKernelGGL((k_colfft<T, LL, +1, CB>), grid, dim3(CB * (LL / 8)), 0, st, a);
+    });
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_xinv_state(const Grid &g, const Band &b, double *const *V, int ncomp, void *const *U,
+                             double *slots, int flags, const void *twx, hipStream_t st, double r_lo, double r_hi)
+{
+    XinvArgs a{};
+    a.r_lo = r_lo;
+    a.r_hi = r_hi;
+    for (int i = 0; i < ncomp; i++) { a.V[i] = V[i]; a.U[i] = U[i]; }
+    a.slots = slots;
+    a.tw = twx;
+    a.g = g;
+    a.b = b;
+    a.ncomp = ncomp;
+    a.flags = flags;
+    TG_DISPATCH_L(g.nx, {
+        constexpr int CB = col_block<T>(LL, sizeof(cpx<double>));
+        const int ncolumns = (flags & 4) ? 2 * g.ny : g.ny * g.nzc;
+        dim3 grid((ncolumns + CB - 1) / CB);
+        hipLaunchKernelGGL((k_xinv_state<T, LL, CB>), grid, dim3(CB * (LL / 8)), 0, st, a);
+    });
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_xfwd_spec(const Grid &g, void *const *W, int ncomp, double *const *out, double scale,
+                            const void *twx, hipStream_t st)
+{
+    XfwdArgs a{};
+    for (int i = 0; i < ncomp; i++) { a.W[i] = W[i]; a.out[i] = out[i]; }
+    a.tw = twx;
+    a.g = g;
+    a.scale = scale;
+    a.ncomp = ncomp;
+    TG_DISPATCH_L(g.nx, {
+        constexpr int CB = col_block<T>(LL, sizeof(cpx<double>));
+        dim3 grid((g.ny * g.nzc + CB - 1) / CB);
+        hipLaunchKernelGGL((k_xfwd_spec<T, LL, CB>), grid, dim3(CB * (LL / 8)), 0, st, a);
+    });
+    return hipGetLastError();
+}
+
+#define TG_DISPATCH_M(Mval, CALL) \
+    switch (Mval) {               \
+        case 8: { constexpr int MM = 8; CALL; } break;     \
+        case 16: { constexpr int MM = 16; CALL; } break;   \
+        case 32: { constexpr int MM = 32; CALL; } break;   \
+        case 64: { constexpr int MM = 64; CALL; } break;   \
+        case 128: { constexpr int MM = 128; CALL; } break; \
+        case 256: { constexpr int MM = 256; CALL; } break; \
+        case 512: { constexpr int MM = 512; CALL; } break; \
+        default: return hipErrorInvalidValue;              \
+    }
+
+template <typename T>
+hipError_t launch_z_c2r(const Grid &g, void *const *W, int ncomp, double *const *R, const void *twm, const void *twn,
+                        hipStream_t st)
+{
+    ZArgs a{};
+    for (int i = 0; i < ncomp; i++) { a.in[i] = W[i]; a.out[i] = R[i]; }
+    a.twm = twm;
+    a.twn = twn;
+    a.g = g;
+    a.ncomp = ncomp;
+    const size_t nlines = (size_t)g.nx * g.ny;
+    TG_DISPATCH_M(g.nz / 2, {
+        constexpr int LPB = z_lines_per_block<MM>();
+        dim3 grid((unsigned)((nlines + LPB - 1) / LPB));
+        hipLaunchKernelGGL((k_z_c2r<T, MM>), grid, dim3(LPB * (MM / 8)), 0, st, a);
+    });
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_z_r2c(const Grid &g, const double *const *R, int ncomp, void *const *W, const void *twm,
+                        const void *twn, hipStream_t st)
+{
+    ZArgs a{};
+    for (int i = 0; i < ncomp; i++) { a.in[i] = (void *)R[i]; a.out[i] = W[i]; }
+    a.twm = twm;
+    a.twn = twn;
+    a.g = g;
+    a.ncomp = ncomp;
+    const size_t nlines = (size_t)g.nx * g.ny;
+    TG_DISPATCH_M(g.nz / 2, {
+        constexpr int LPB = z_lines_per_block<MM>();
+        dim3 grid((unsigned)((nlines + LPB - 1) / LPB));
+        hipLaunchKernelGGL((k_z_r2c<T, MM>), grid, dim3(LPB * (MM / 8)), 0, st, a);
+    });
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_z_products(const Grid &g, void *const *U, void *const *P, int giver, const void *twm,
+                             const void *twn, hipStream_t st)
+{
+    ZArgs a{};
+    for (int i = 0; i < (giver ? 6 : 3); i++) a.in[i] = U[i];
+    for (int i = 0; i < (giver ? 9 : 6); i++) a.out[i] = P[i];
+    a.twm = twm;
+    a.twn = twn;
+    a.g = g;
+    const size_t nlines = (size_t)g.nx * g.ny;
+    TG_DISPATCH_M(g.nz / 2, {
+        constexpr int LPB = z_lines_per_block<MM>();
+        dim3 grid((unsigned)((nlines + LPB - 1) / LPB));
+        if (giver) hipLaunchKernelGGL((k_z_products<T, MM, true>), grid, dim3(LPB * (MM / 8)), 0, st, a);
+        else hipLaunchKernelGGL((k_z_products<T, MM, false>), grid, dim3(LPB * (MM / 8)), 0, st, a);
+    });
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_x_nlin(const Grid &g, const Band &b, const RkParams &rk, void *const *P, double *const *V,
+                         double *const *acc, void *const *U, const double *slots_in, double *slots_out,
+                         double *const *out, const void *twx, hipStream_t st)
+{
+    XnlArgs a{};
+    const bool giver = rk.stage < 0;
+    for (int i = 0; i < (giver ? 9 : 6); i++) a.P[i] = P[i];
+    for (int i = 0; i < 3; i++) {
+        a.V[i] = V ? V[i] : nullptr;
+        a.acc[i] = acc ? acc[i] : nullptr;
+        a.U[i] = U ? U[i] : nullptr;
+        a.out[i] = out ? out[i] : nullptr;
+    }
+    a.slots_in = slots_in;
+    a.slots_out = slots_out;
+    a.tw = twx;
+    a.g = g;
+    a.b = b;
+    a.rk = rk;
+    TG_DISPATCH_L(g.nx, {
+        constexpr int CB = col_block<T>(LL, sizeof(cpx<double>));
+        dim3 grid((g.ny * g.nzc + CB - 1) / CB);
+        if (giver) {
+            a.rk.stage = 0;
+            a.rk.write_rhs = 0;
+            hipLaunchKernelGGL((k_x_nlin<T, LL, CB, true>), grid, dim3(CB * (LL / 8)), 0, st, a);
+        } else {
+            hipLaunchKernelGGL((k_x_nlin<T, LL, CB, false>), grid, dim3(CB * (LL / 8)), 0, st, a);
+        }
+    });
+    return hipGetLastError();
+}
+
+hipError_t launch_reality(const Grid &g, double *const *V, int ncomp, hipStream_t st)
+{
+    PlaneArgs a{};
+    for (int i = 0; i < ncomp; i++) a.V[i] = V[i];
+    a.g = g;
+    a.ncomp = ncomp;
+    const int n = g.nx * g.ny;
+    hipLaunchKernelGGL(k_reality, dim3((n + 255) / 256), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_dealias(const Grid &g, double *const *V, int ncomp, hipStream_t st)
+{
+    PlaneArgs a{};
+    for (int i = 0; i < ncomp; i++) a.V[i] = V[i];
+    a.g = g;
+    a.ncomp = ncomp;
+    hipLaunchKernelGGL(k_dealias, dim3(2048), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+#define TG_INST(T)                                                                                              \
+    template hipError_t launch_colfft<T>(const Grid &, int, int, void *const *, int, const void *, hipStream_t); \
+    template hipError_t launch_xinv_state<T>(const Grid &, const Band &, double *const *, int, void *const *,   \
+                                             double *, int, const void *, hipStream_t, double, double);        \
+    template hipError_t launch_xfwd_spec<T>(const Grid &, void *const *, int, double *const *, double,          \
+                                            const void *, hipStream_t);                                        \
+    template hipError_t launch_z_c2r<T>(const Grid &, void *const *, int, double *const *, const void *,        \
+                                        const void *, hipStream_t);                                            \
+    template hipError_t launch_z_r2c<T>(const Grid &, const double *const *, int, void *const *, const void *,  \
+                                        const void *, hipStream_t);                                            \
+    template hipError_t launch_z_products<T>(const Grid &, void *const *, void *const *, int, const void *,     \
+                                             const void *, hipStream_t);                                       \
+    template hipError_t launch_x_nlin<T>(const Grid &, const Band &, const RkParams &, void *const *,           \
+                                         double *const *, double *const *, void *const *, const double *,      \
+                                         double *, double *const *, const void *, hipStream_t);
+
+TG_INST(double)
+TG_INST(float)
+
+}  // namespace tg
