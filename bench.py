@@ -1,0 +1,264 @@
+"""RK4 steps/s of the FFF Navier-Stokes hot path at 512^3 on one MI355X.
+
+Workload (one "step" = one Lawson RK4 step of the pseudo-spectral
+incompressible Navier-Stokes equations, all four nonlinear evaluations,
+pressure projection, forcing and the strong reality condition):
+  grid 512^3 (box 2 pi), FP64 state and transforms, reference corner
+  dealiasing, nu = 1e-3, dt = 1e-3; seeded solenoidal initial field with
+  E(k) ~ k^-5/3 on 2 <= k <= 170 (random phases, total energy 0.5);
+  constant-energy-supply forcing on 2 < k <= 3 injecting eps = 0.1.
+Inputs are generated on the device before the timed region (synthetic data).
+
+Besides the headline FP64 number the line reports:
+  fp32       the same step with FP32 transforms/products (FP64 state), its
+             steps/s and its relative energy error against FP64 after the
+             same steps;
+  diag       one evaluation of the energy flux over 20 log-spaced spheres
+             and the 20x20 shell-to-shell transfer (plus its sum check);
+  roofline   the dominant kernel's algorithmic HBM bytes per launch over its
+             HIP-event-measured average duration, against 8 TB/s;
+  step_roofline  the whole step's algorithmic bytes over the step time;
+  cpu_baseline   the FP64 CPU oracle (restatement of the reference
+             algorithm, own FFT, OpenMP) on one RK4 step of the same
+             512^3 workload, on the host's cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=512, help="grid size per axis")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-extra", action="store_true", help="headline FP64 number only")
+    ap.add_argument("--copy-calib", action="store_true", help="run a plain 1 GiB device copy (counter calibration)")
+    return ap.parse_args()
+
+
+def algorithmic_bytes(kernel, N, prec_bytes, nstate=3):
+    """Algorithmic HBM bytes of one launch class per RK4 step (all launches)."""
+    nc = N[0] * N[1] * (N[2] // 2 + 1)
+    Sp = nc * 2 * prec_bytes      # one complex work field in the transform precision
+    S64 = nc * 16                 # one complex FP64 state field
+    if kernel == "y_fft":         # 4 stages x (3 inverse + 6 forward fields), read + write
+        return 4 * 9 * 2 * Sp
+    if kernel == "z_products":    # 4 stages x (read 3, write 6)
+        return 4 * 9 * Sp
+    if kernel == "x_nlin_rk":     # 4 stages x (read 6 products, write 3) + 11 state fields per component
+        return 4 * 9 * Sp + 11 * nstate * S64
+    if kernel == "reality":       # two planes of V (read+write) and the patched columns
+        return 2 * 2 * nstate * N[0] * N[1] * 16 + nstate * N[0] * N[1] * 2 * (16 + 2 * prec_bytes)
+    if kernel == "x_inv":
+        return nstate * (S64 + Sp)
+    return 0
+
+
+def step_bytes_model(N, prec_bytes, nstate=3, n_tr=9):
+    """B_step = 8 (n_inv + n_fwd) S_p + 11 n_state S64 (two-sweep model)."""
+    nc = N[0] * N[1] * (N[2] // 2 + 1)
+    return 8 * n_tr * nc * 2 * prec_bytes + 11 * nstate * nc * 16
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU slab "
+              "decomposition is a design, not part of this build.", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    from tarang_amd import api
+    from tarang_amd._lib import KERNELS, check
+
+    torch.cuda.init()
+    dev = torch.device("cuda", 0)
+
+    if args.copy_calib:
+        a = torch.ones(1 << 27, dtype=torch.float64, device=dev)   # 1 GiB
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calib_bytes": a.numel() * 8, "launches": 3}))
+        return
+
+    N = (args.n, args.n, args.n)
+    nu, dt = 1e-3, 1e-3
+    force = dict(force_kind=1, force_eps=0.1, force_r_in=2.0, force_r_out=3.0)
+    kmax_ic = 170.0 if args.n >= 512 else args.n / 3.0
+
+    def make(prec):
+        b = api.FFF_Basis(N, precision=prec, dealias=api.DEALIAS_REF_CORNER, strong_reality=True)
+        U = api.FluidVF(b, nu)
+        return b, U
+
+    b64, U64 = make(api.PREC_FP64)
+    U64.Init_cond_spectrum(kind=1, kmin=2.0, kmax=kmax_ic, energy=0.5, seed=2)
+    V_init = [v.clone() for v in U64.cvf] if not args.no_extra else None
+    ta = api.Time_advance_incompress(U64, dt, **force)
+
+    for _ in range(args.warmup):
+        ta.Time_advance_step(1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ta.Time_advance_step(1)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_step = (t1 - t0) * 1e3 / max(args.steps, 1)
+    value = 1e3 / ms_step
+    print(f"[bench] fp64 {args.steps} steps: {ms_step:.3f} ms/step", file=sys.stderr, flush=True)
+
+    # per-kernel HIP-event timing on the plan's stream over 2 extra steps
+    L = b64.L
+    check("tg_profile_reset", L.tg_profile_reset(b64.plan))
+    check("tg_profile_enable", L.tg_profile_enable(b64.plan, 1))
+    prof_steps = 2
+    ta.Time_advance_step(prof_steps)
+    check("tg_profile_enable", L.tg_profile_enable(b64.plan, 0))
+    import ctypes
+    kern = {}
+    for k, name in enumerate(KERNELS):
+        ms = ctypes.c_double()
+        n = ctypes.c_int64()
+        check("tg_profile_read", L.tg_profile_read(b64.plan, k, ctypes.byref(ms), ctypes.byref(n)))
+        if n.value:
+            kern[name] = (ms.value / prof_steps, n.value // prof_steps)  # per step: ms, launches
+    dom = max(kern, key=lambda k: kern[k][0])
+    dom_ms_step, dom_launch = kern[dom]
+    dom_bytes_step = algorithmic_bytes(dom, N, 8)
+    dom_achieved = dom_bytes_step / (dom_ms_step * 1e-3) / 1e9
+    step_model = step_bytes_model(N, 8)
+    roofline = {
+        "kernel": dom, "bound": "hbm", "achieved": round(dom_achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": round(dom_achieved / HBM_PEAK_GBS, 4),
+        "bytes_per_launch": int(dom_bytes_step / dom_launch), "avg_launch_ms": round(dom_ms_step / dom_launch, 4),
+        "traffic": None,
+    }
+    per_kernel = {k: {"ms_per_step": round(v[0], 3), "launches_per_step": v[1],
+                      "GBps": round(algorithmic_bytes(k, N, 8) / (v[0] * 1e-3) / 1e9, 1)
+                      if algorithmic_bytes(k, N, 8) else None} for k, v in kern.items()}
+    step_roofline = {"bytes_model": step_model, "achieved": round(step_model / (ms_step * 1e-3) / 1e9, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(step_model / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+
+    extra = {}
+    if not args.no_extra:
+        # FP32 transforms, same initial field, same number of steps
+        E64 = U64.Compute_total_energy()
+        nsteps_total = args.warmup + args.steps + prof_steps
+        del ta
+        b32, U32 = make(api.PREC_FP32)
+        U32.set_field(V_init)
+        ta32 = api.Time_advance_incompress(U32, dt, **force)
+        for _ in range(args.warmup):
+            ta32.Time_advance_step(1)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            ta32.Time_advance_step(1)
+        torch.cuda.synchronize()
+        ms32 = (time.perf_counter() - t0) * 1e3 / max(args.steps, 1)
+        ta32.Time_advance_step(prof_steps)
+        E32 = U32.Compute_total_energy()
+        print(f"[bench] fp32 {args.steps} steps: {ms32:.3f} ms/step", file=sys.stderr, flush=True)
+        step_model32 = step_bytes_model(N, 4)
+        extra["fp32"] = {"steps_per_s": round(1e3 / ms32, 3), "ms_per_step": round(ms32, 3),
+                         "speedup_vs_fp64": round(ms_step / ms32, 3),
+                         "energy_rel_err_vs_fp64": abs(E32 - E64) / abs(E64), "steps_compared": nsteps_total,
+                         "step_roofline_frac": round(step_model32 / (ms32 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+        del ta32, U32, b32
+        torch.cuda.empty_cache()
+        # energy-transfer diagnostics on the FP64 state (20 spheres / 20 shells)
+        radii = b64.default_radii(20)
+        et = api.EnergyTr(b64, radii, radii)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        flux = et.Compute_flux(U64)
+        S = et.Compute_shell_tr(U64)
+        torch.cuda.synchronize()
+        tdiag = time.perf_counter() - t0
+        extra["diag"] = {"flux_and_shell_to_shell_ms": round(tdiag * 1e3, 1), "spheres": 20, "shells": 20,
+                         "shell_to_shell_sum": float(S.sum()), "shell_to_shell_abs_sum": float(np.abs(S).sum()),
+                         "flux_max": float(np.max(flux))}
+        print(f"[bench] diagnostics {tdiag * 1e3:.1f} ms", file=sys.stderr, flush=True)
+        extra["energy"] = E64
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(N, nu, dt, force, V_init if V_init is not None else U64.cvf)
+
+    out = {
+        "metric": "RK4 steps/s at 512^3 (FP64 vs FP32, % HBM roofline); adjoint/forward cost",
+        "value": round(value, 4),
+        "unit": "RK4 steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded k^-5/3 solenoidal field generated on device)",
+        "config": {"workload": f"FFF incompressible Navier-Stokes RK4 step, {N[0]}x{N[1]}x{N[2]}, FP64, "
+                               "corner dealias, strong reality, alpha-forcing eps=0.1 on 2<k<=3",
+                   "grid": list(N), "nu": nu, "dt": dt, "parallelism": "single GPU"},
+        "roofline": roofline,
+        "step_roofline": step_roofline,
+        "kernels": per_kernel,
+        "cpu_baseline": cpu,
+    }
+    out.update(extra)
+    print(json.dumps(out))
+
+
+def cpu_baseline(N, nu, dt, force, V_dev):
+    """One RK4 stage (Compute_force_TO_rhs: force, nonlinear term, pressure,
+    RHS) of the same workload on the FP64 CPU oracle (own FFT, OpenMP); a
+    step is four such stages plus O(N^3) pointwise updates, so steps/s is
+    reported as 1 / (4 x stage time)."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(HERE, "oracle"))
+    import tg_oracle as to
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    o = to.Oracle(N, dealias=to.DEALIAS_REF_CORNER, strong_reality=True)
+    V = [np.ascontiguousarray(v.cpu().numpy()) for v in V_dev]
+    fl = to.Fluid(nu, dt, force["force_kind"], force["force_eps"], force["force_r_in"], force["force_r_out"])
+    print(f"[bench] cpu baseline: one RK4 stage of {N} on {threads} threads", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    F = o.compute_force(fl, V)
+    Nl = o.compute_nlin(V)
+    o.compute_rhs(Nl, F)
+    t = time.perf_counter() - t0
+    print(f"[bench] cpu baseline: {t:.2f} s/stage", file=sys.stderr, flush=True)
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"value": round(1.0 / (4 * t), 5), "unit": "RK4 steps/s", "cores": threads, "kind": "port",
+            "sample": f"one RK4 stage (force + nonlinear term + pressure/RHS, 9 FP64 3D transforms) of the "
+                      f"same {N[0]}^3 workload on the FP64 oracle (own FFT, OpenMP): {t:.1f} s; "
+                      f"steps/s = 1/(4 x stage)", "cpu_model": model}
+
+
+if __name__ == "__main__":
+    main()

@@ -128,6 +128,28 @@ int tg_flux(tg_plan *plan, double *const V_dev[3], const double *radii, int nsph
  * S[(nshells+1)*(nshells+1)], row = giver shell.  Dealiases V in place. */
 int tg_shell_to_shell(tg_plan *plan, double *const V_dev[3], const double *radii, int nshells, double *S);
 
+/* Per-kernel timing with HIP events on the plan's stream (no counterpart
+ * in the reference, whose only timing is the total wall time printed by
+ * main.cc:146-157).  Kernel classes: */
+enum {
+    TG_KERNEL_X_INV = 0,        /* x-inverse of the state (step start) */
+    TG_KERNEL_Y_FFT = 1,        /* y passes (inverse and forward) */
+    TG_KERNEL_Z_PRODUCTS = 2,   /* z c2r + real-space products + z r2c */
+    TG_KERNEL_X_NLIN_RK = 3,    /* x forward + nonlinear term + RK4 stage + x inverse */
+    TG_KERNEL_X_NLIN_GIVER = 4, /* x forward of the giver products */
+    TG_KERNEL_REALITY = 5,      /* strong reality condition + patch */
+    TG_KERNEL_Z_R2C = 6,
+    TG_KERNEL_Z_C2R = 7,
+    TG_KERNEL_X_FWD = 8,
+    TG_KERNEL_REDUCE = 9,
+    TG_KERNEL_COUNT = 10
+};
+int tg_profile_enable(tg_plan *plan, int on);
+int tg_profile_reset(tg_plan *plan);
+/* Synchronises the stream; total milliseconds and launch count of a class. */
+int tg_profile_read(tg_plan *plan, int kernel, double *total_ms, int64_t *launches);
+const char *tg_kernel_name(int kernel);
+
 /* Seeded solenoidal initial field (this library's own generator; the
  * reference's Init_cond_energy_helicity_spectrum, init_cond_energy.cc:67-176,
  * is seeded by time(0)). kind 0: E(K) ~ K^4 exp(-2 (K/k0)^2) with Gaussian

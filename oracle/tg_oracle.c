@@ -85,13 +85,37 @@ static void fft_rec(const fftplan *p, const cplx *in, int is, cplx *out, int n, 
     int m = n / r;
     for (int j = 0; j < r; j++)
         fft_rec(p, in + (size_t)j * is, is * r, out + (size_t)j * m, m, fi + 1, wstride * r, sign);
+    const int N = p->n;
+    /* twiddle index j*k*wstride < r*m*wstride = N: no reduction needed */
+    if (r == 2) {
+        for (int k = 0; k < m; k++) {
+            cplx w = p->w[(size_t)k * wstride];
+            if (sign > 0) w = conj(w);
+            cplx a = out[k], b = out[k + m] * w;
+            out[k] = a + b;
+            out[k + m] = a - b;
+        }
+        return;
+    }
+    if (r == 4) {
+        const cplx rot = sign > 0 ? I : -I; /* w_4^1 */
+        for (int k = 0; k < m; k++) {
+            cplx w1 = p->w[(size_t)k * wstride], w2 = p->w[(size_t)2 * k * wstride], w3 = p->w[(size_t)3 * k * wstride];
+            if (sign > 0) { w1 = conj(w1); w2 = conj(w2); w3 = conj(w3); }
+            cplx a0 = out[k], a1 = out[k + m] * w1, a2 = out[k + 2 * m] * w2, a3 = out[k + 3 * m] * w3;
+            cplx b0 = a0 + a2, b1 = a0 - a2, b2 = a1 + a3, b3 = (a1 - a3) * rot;
+            out[k] = b0 + b2;
+            out[k + m] = b1 + b3;
+            out[k + 2 * m] = b0 - b2;
+            out[k + 3 * m] = b1 - b3;
+        }
+        return;
+    }
     cplx t[64];
     cplx *tt = r <= 64 ? t : (cplx *)malloc(sizeof(cplx) * (size_t)r);
-    const int N = p->n;
     for (int k = 0; k < m; k++) {
         for (int j = 0; j < r; j++) {
-            long e = ((long)j * k * wstride) % N;
-            cplx w = p->w[e];
+            cplx w = p->w[(size_t)j * k * wstride];
             if (sign > 0) w = conj(w);
             tt[j] = out[(size_t)j * m + k] * w;
         }

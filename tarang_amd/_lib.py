@@ -66,7 +66,14 @@ SIGNATURES = {
     "tg_flux": (_I, [_P, _PP, _DP, _I, _DP]),
     "tg_shell_to_shell": (_I, [_P, _PP, _DP, _I, _DP]),
     "tg_init_field": (_I, [_P, _I, _D, _D, _D, _D, ctypes.c_uint64, _PP]),
+    "tg_profile_enable": (_I, [_P, _I]),
+    "tg_profile_reset": (_I, [_P]),
+    "tg_profile_read": (_I, [_P, _I, _DP, ctypes.POINTER(ctypes.c_int64)]),
+    "tg_kernel_name": (ctypes.c_char_p, [_I]),
 }
+
+KERNELS = ["x_inv", "y_fft", "z_products", "x_nlin_rk", "x_nlin_giver", "reality", "z_r2c", "z_c2r", "x_fwd",
+           "reduce"]
 
 
 def header_symbols():

@@ -66,9 +66,71 @@ struct tg_plan {
     double *unext_V[3] = {};
     // host-side band config for the current step
     Band band{};
+    // per-kernel HIP event timing (tg_profile_*)
+    bool prof_on = false;
+    struct ProfRec { int kid; hipEvent_t a, b; };
+    std::vector<ProfRec> prof_recs;
+    std::vector<hipEvent_t> prof_pool;
+    double prof_ms[TG_KERNEL_COUNT] = {};
+    int64_t prof_n[TG_KERNEL_COUNT] = {};
 };
 
 namespace {
+
+hipEvent_t prof_event(tg_plan *p)
+{
+    if (!p->prof_pool.empty()) {
+        hipEvent_t e = p->prof_pool.back();
+        p->prof_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+// Brackets one launch with events when profiling is on.
+struct Prof {
+    tg_plan *p;
+    int kid;
+    hipEvent_t a = nullptr;
+    Prof(tg_plan *p_, int kid_) : p(p_), kid(kid_)
+    {
+        if (p->prof_on) {
+            a = prof_event(p);
+            if (a) (void)hipEventRecord(a, p->stream);
+        }
+    }
+    ~Prof()
+    {
+        if (a) {
+            hipEvent_t b = prof_event(p);
+            if (b) {
+                (void)hipEventRecord(b, p->stream);
+                p->prof_recs.push_back({kid, a, b});
+            } else {
+                p->prof_pool.push_back(a);
+            }
+        }
+    }
+};
+
+int prof_collect(tg_plan *p)
+{
+    if (p->prof_recs.empty()) return TG_OK;
+    HIPCHK(hipStreamSynchronize(p->stream));
+    for (auto &r : p->prof_recs) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+            p->prof_ms[r.kid] += ms;
+            p->prof_n[r.kid] += 1;
+        }
+        p->prof_pool.push_back(r.a);
+        p->prof_pool.push_back(r.b);
+    }
+    p->prof_recs.clear();
+    return TG_OK;
+}
 
 template <typename T>
 int upload_twiddles(tg_plan *p, void **dst, int L, int count)
@@ -107,6 +169,8 @@ void free_plan(tg_plan *p)
         if (q) (void)hipFree(q);
     for (void *q : p->P)
         if (q) (void)hipFree(q);
+    for (auto &r : p->prof_recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
+    for (auto e : p->prof_pool) (void)hipEventDestroy(e);
     delete p;
 }
 
@@ -159,11 +223,11 @@ int run_forward(tg_plan *p, const double *real_dev, double *spec_dev)
 {
     const double *R[1] = {real_dev};
     void *W[1] = {p->U[0]};
-    HIPCHK(launch_z_r2c<T>(p->g, R, 1, W, p->twm, p->twn, p->stream));
-    HIPCHK(launch_colfft<T>(p->g, 1, -1, W, 1, p->twy, p->stream));
+    { Prof pr(p, TG_KERNEL_Z_R2C); HIPCHK(launch_z_r2c<T>(p->g, R, 1, W, p->twm, p->twn, p->stream)); }
+    { Prof pr(p, TG_KERNEL_Y_FFT); HIPCHK(launch_colfft<T>(p->g, 1, -1, W, 1, p->twy, p->stream)); }
     double *O[1] = {spec_dev};
     const double scale = 1.0 / ((double)p->g.nx * p->g.ny * p->g.nz);
-    HIPCHK(launch_xfwd_spec<T>(p->g, W, 1, O, scale, p->twx, p->stream));
+    { Prof pr(p, TG_KERNEL_X_FWD); HIPCHK(launch_xfwd_spec<T>(p->g, W, 1, O, scale, p->twx, p->stream)); }
     return TG_OK;
 }
 
@@ -173,10 +237,10 @@ int run_inverse(tg_plan *p, const double *spec_dev, double *real_dev)
     double *V[1] = {const_cast<double *>(spec_dev)};
     void *W[1] = {p->U[0]};
     Band nb{};
-    HIPCHK(launch_xinv_state<T>(p->g, nb, V, 1, W, nullptr, 0, p->twx, p->stream));
-    HIPCHK(launch_colfft<T>(p->g, 1, +1, W, 1, p->twy, p->stream));
+    { Prof pr(p, TG_KERNEL_X_INV); HIPCHK(launch_xinv_state<T>(p->g, nb, V, 1, W, nullptr, 0, p->twx, p->stream)); }
+    { Prof pr(p, TG_KERNEL_Y_FFT); HIPCHK(launch_colfft<T>(p->g, 1, +1, W, 1, p->twy, p->stream)); }
     double *R[1] = {real_dev};
-    HIPCHK(launch_z_c2r<T>(p->g, W, 1, R, p->twm, p->twn, p->stream));
+    { Prof pr(p, TG_KERNEL_Z_C2R); HIPCHK(launch_z_c2r<T>(p->g, W, 1, R, p->twm, p->twn, p->stream)); }
     return TG_OK;
 }
 
@@ -184,9 +248,9 @@ int run_inverse(tg_plan *p, const double *spec_dev, double *real_dev)
 template <typename T>
 int ns_middle(tg_plan *p)
 {
-    HIPCHK(launch_colfft<T>(p->g, 1, +1, p->U, 3, p->twy, p->stream));
-    HIPCHK(launch_z_products<T>(p->g, p->U, p->P, 0, p->twm, p->twn, p->stream));
-    HIPCHK(launch_colfft<T>(p->g, 1, -1, p->P, 6, p->twy, p->stream));
+    { Prof pr(p, TG_KERNEL_Y_FFT); HIPCHK(launch_colfft<T>(p->g, 1, +1, p->U, 3, p->twy, p->stream)); }
+    { Prof pr(p, TG_KERNEL_Z_PRODUCTS); HIPCHK(launch_z_products<T>(p->g, p->U, p->P, 0, p->twm, p->twn, p->stream)); }
+    { Prof pr(p, TG_KERNEL_Y_FFT); HIPCHK(launch_colfft<T>(p->g, 1, -1, p->P, 6, p->twy, p->stream)); }
     return TG_OK;
 }
 
@@ -240,9 +304,10 @@ int run_giver_nlin(tg_plan *p, double *const V[3], double *const out[3], double 
     else
         HIPCHK(launch_xinv_state<T>(p->g, nb, W, 3, p->U + 3, nullptr, 0, p->twx, p->stream));
     HIPCHK(launch_colfft<T>(p->g, 1, +1, p->U + 3, 3, p->twy, p->stream));
-    HIPCHK(launch_z_products<T>(p->g, p->U, p->P, 1, p->twm, p->twn, p->stream));
-    HIPCHK(launch_colfft<T>(p->g, 1, -1, p->P, 9, p->twy, p->stream));
+    { Prof pr(p, TG_KERNEL_Z_PRODUCTS); HIPCHK(launch_z_products<T>(p->g, p->U, p->P, 1, p->twm, p->twn, p->stream)); }
+    { Prof pr(p, TG_KERNEL_Y_FFT); HIPCHK(launch_colfft<T>(p->g, 1, -1, p->P, 9, p->twy, p->stream)); }
     RkParams rk = rk_params(p, nullptr, -1);
+    Prof pr(p, TG_KERNEL_X_NLIN_GIVER);
     HIPCHK(launch_x_nlin<T>(p->g, nb, rk, p->P, nullptr, nullptr, nullptr, nullptr, nullptr, out, p->twx,
                             p->stream));
     p->unext_valid = false;
@@ -257,6 +322,7 @@ int run_rk4(tg_plan *p, double *const V[3], const tg_fluid *f, int nsteps, int f
     const bool cont = (flags & TG_STEP_CONTINUE) && p->unext_valid && p->unext_V[0] == V[0] &&
                       p->unext_V[1] == V[1] && p->unext_V[2] == V[2];
     if (!cont) {
+        Prof pr(p, TG_KERNEL_X_INV);
         HIPCHK(launch_xinv_state<T>(p->g, p->band, V, 3, p->U, p->slots[0], 1 | (p->band.kind ? 2 : 0), p->twx,
                                     p->stream));
     }
@@ -265,10 +331,12 @@ int run_rk4(tg_plan *p, double *const V[3], const tg_fluid *f, int nsteps, int f
             rc = ns_middle<T>(p);
             if (rc) return rc;
             RkParams rk = rk_params(p, f, s);
+            Prof pr(p, TG_KERNEL_X_NLIN_RK);
             HIPCHK(launch_x_nlin<T>(p->g, p->band, rk, p->P, V, p->acc, p->U, p->slots[(s - 1) & 1],
                                     p->slots[s & 1], nullptr, p->twx, p->stream));
         }
         if (p->cfg.strong_reality) {
+            Prof pr(p, TG_KERNEL_REALITY);
             HIPCHK(launch_reality(p->g, V, 3, p->stream));
             HIPCHK(launch_xinv_state<T>(p->g, p->band, V, 3, p->U, p->slots[0], 1 | 4 | (p->band.kind ? 2 : 0),
                                         p->twx, p->stream));
@@ -581,6 +649,41 @@ int tg_shell_to_shell(tg_plan *plan, double *const V_dev[3], const double *radii
         for (int n = 0; n < nr; n++) S[(size_t)m * nr + n] = -row[n];
     }
     return TG_OK;
+}
+
+int tg_profile_enable(tg_plan *plan, int on)
+{
+    if (!plan) return fail(TG_ERR_ARG, "null plan");
+    int rc = prof_collect(plan);
+    if (rc) return rc;
+    plan->prof_on = on != 0;
+    return TG_OK;
+}
+
+int tg_profile_reset(tg_plan *plan)
+{
+    if (!plan) return fail(TG_ERR_ARG, "null plan");
+    int rc = prof_collect(plan);
+    if (rc) return rc;
+    for (int k = 0; k < TG_KERNEL_COUNT; k++) { plan->prof_ms[k] = 0; plan->prof_n[k] = 0; }
+    return TG_OK;
+}
+
+int tg_profile_read(tg_plan *plan, int kernel, double *total_ms, int64_t *launches)
+{
+    if (!plan || kernel < 0 || kernel >= TG_KERNEL_COUNT) return fail(TG_ERR_ARG, "bad argument");
+    int rc = prof_collect(plan);
+    if (rc) return rc;
+    if (total_ms) *total_ms = plan->prof_ms[kernel];
+    if (launches) *launches = plan->prof_n[kernel];
+    return TG_OK;
+}
+
+const char *tg_kernel_name(int kernel)
+{
+    static const char *names[TG_KERNEL_COUNT] = {"x_inv", "y_fft", "z_products", "x_nlin_rk", "x_nlin_giver",
+                                                 "reality", "z_r2c", "z_c2r", "x_fwd", "reduce"};
+    return (kernel >= 0 && kernel < TG_KERNEL_COUNT) ? names[kernel] : "unknown";
 }
 
 int tg_init_field(tg_plan *plan, int kind, double k0, double kmin, double kmax, double energy, uint64_t seed,
