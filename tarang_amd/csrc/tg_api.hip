@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -48,6 +49,7 @@ struct tg_plan {
     int prec = TG_PREC_FP64;
     size_t elem = 16;     // bytes of one complex work element
     size_t nc = 0;        // complex modes per field
+    size_t ncw = 0;       // complex entries of one padded work field
     int64_t dev_bytes = 0;
     // twiddle tables in the transform precision: x (Nx), y (Ny), m (Nz/2), n (Nz, k = 0..Nz/2)
     void *twx = nullptr, *twy = nullptr, *twm = nullptr, *twn = nullptr;
@@ -66,6 +68,7 @@ struct tg_plan {
     double *unext_V[3] = {};
     // host-side band config for the current step
     Band band{};
+    int x_tile_columns = 0;  // fused x-pass tile width (0 = LDS-budget width)
     // per-kernel HIP event timing (tg_profile_*)
     bool prof_on = false;
     struct ProfRec { int kid; hipEvent_t a, b; };
@@ -333,7 +336,7 @@ int run_rk4(tg_plan *p, double *const V[3], const tg_fluid *f, int nsteps, int f
             RkParams rk = rk_params(p, f, s);
             Prof pr(p, TG_KERNEL_X_NLIN_RK);
             HIPCHK(launch_x_nlin<T>(p->g, p->band, rk, p->P, V, p->acc, p->U, p->slots[(s - 1) & 1],
-                                    p->slots[s & 1], nullptr, p->twx, p->stream));
+                                    p->slots[s & 1], nullptr, p->twx, p->stream, p->x_tile_columns));
         }
         if (p->cfg.strong_reality) {
             Prof pr(p, TG_KERNEL_REALITY);
@@ -418,7 +421,13 @@ int tg_plan_create(const tg_config *cfg, tg_plan **plan)
     p->g.dealias = cfg->dealias;
     p->prec = cfg->precision;
     p->elem = p->prec == TG_PREC_FP32 ? 8 : 16;
+    {
+        const int per_line = 128 / (int)p->elem;  // elements per 128-byte line
+        p->g.nzp = (p->g.nzc + per_line - 1) / per_line * per_line;
+    }
+    if (const char *e = getenv("TG_X_TILE_COLUMNS")) p->x_tile_columns = atoi(e);
     p->nc = (size_t)cfg->nx * cfg->ny * (cfg->nz / 2 + 1);
+    p->ncw = (size_t)cfg->nx * cfg->ny * p->g.nzp;
     int rc = set_device(p);
     if (rc) { free_plan(p); return rc; }
     if (p->prec == TG_PREC_FP32) {
@@ -432,9 +441,10 @@ int tg_plan_create(const tg_config *cfg, tg_plan **plan)
         if (!rc) rc = upload_twiddles<double>(p, &p->twm, cfg->nz / 2, cfg->nz / 2);
         if (!rc) rc = upload_twiddles<double>(p, &p->twn, cfg->nz, cfg->nz / 2 + 1);
     }
-    for (int i = 0; i < 6 && !rc; i++) rc = dmalloc(p, &p->U[i], p->nc * p->elem);
-    for (int i = 0; i < 9 && !rc; i++) rc = dmalloc(p, &p->P[i], p->nc * p->elem);
-    for (int i = 0; i < 3 && !rc; i++) rc = dmalloc(p, (void **)&p->acc[i], p->nc * 16);
+    for (int i = 0; i < 6 && !rc; i++) rc = dmalloc(p, &p->U[i], p->ncw * p->elem);
+    for (int i = 0; i < 9 && !rc; i++) rc = dmalloc(p, &p->P[i], p->ncw * p->elem);
+    // accumulator: FP64, padded work layout (also holds giver outputs in the boundary layout)
+    for (int i = 0; i < 3 && !rc; i++) rc = dmalloc(p, (void **)&p->acc[i], p->ncw * 16);
     p->red_blocks = (int)std::min<size_t>(2048, (p->nc + 1023) / 1024);
     if (p->red_blocks < 1) p->red_blocks = 1;
     p->partial_bytes = (size_t)p->red_blocks * 2 * 1100 * sizeof(double);

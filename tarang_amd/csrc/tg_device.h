@@ -127,6 +127,24 @@ __host__ __device__ constexpr int num_passes(int L)
     return n;
 }
 
+// Synchronisation of the threads that share one line.  A line of L points
+// is handled by TPL = L/8 consecutive threads; when TPL <= 64 the line lives
+// inside one wavefront, whose LDS operations execute in order, so only the
+// compiler has to be kept from moving LDS accesses across the exchange
+// (wavefront-scope fence).  Longer lines span waves and need the block
+// barrier.
+template <int TPL>
+__device__ __forceinline__ void line_sync()
+{
+    if constexpr (TPL <= 64) {
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    } else {
+        __syncthreads();
+    }
+}
+
 // LDS position with one pad entry every 16 points (breaks the stride-8
 // write pattern of the first radix-8 pass across banks).
 __device__ __forceinline__ int lpad(int i) { return i + (i >> 4); }
@@ -176,10 +194,10 @@ __device__ __forceinline__ void fft_passes(cpx<T> *v, cpx<T> *lds, const cpx<T> 
         stockham_pass<L, R, NS, DIR, LAST>(v, lds, tw, j);
         if constexpr (!LAST) {
             constexpr int TPL = L / 8;
-            __syncthreads();
+            line_sync<TPL>();
 #pragma unroll
             for (int s = 0; s < 8; s++) v[s] = lds[lpad(j + TPL * s)];
-            __syncthreads();
+            line_sync<TPL>();
             fft_passes<L, DIR, P + 1, NS * R>(v, lds, tw, j);
         }
     }
@@ -188,9 +206,9 @@ __device__ __forceinline__ void fft_passes(cpx<T> *v, cpx<T> *lds, const cpx<T> 
 // Full line FFT of L = 8*TPL points (L a power of two, 8..4096), values in
 // registers at positions j + TPL*s.  Exponent sign DIR (-1 forward, +1
 // inverse), unnormalised.  tw[k] = exp(-2 pi i k / L).  Uses the LDS line
-// buffer (lpad_len(L) entries) for the exchanges between passes.  Every
-// thread of the block must call it (block-wide barriers); the caller must
-// have synchronised if the LDS line was in use before.
+// buffer (lpad_len(L) entries) for the exchanges between passes; every
+// thread of the line must call it (line_sync); the caller must have
+// synchronised if the LDS line was in use before.
 template <int L, int DIR, typename T>
 __device__ __forceinline__ void fft_line(cpx<T> *v, cpx<T> *lds, const cpx<T> *__restrict__ tw, int j)
 {

@@ -8,10 +8,11 @@ namespace tg {
 // Grid description shared by every kernel (reference FFF basis conventions:
 // kx = lx <= Nx/2 ? lx : lx - Nx, likewise ky; kz = lz; K_d = k_d*kfactor[d]).
 struct Grid {
-    int nx, ny, nz, nzc;  // nzc = nz/2 + 1
+    int nx, ny, nz, nzc;  // nzc = nz/2 + 1 (kz stride of FP64 state fields)
     double kf[3];
     int dealias;          // 0 none, 1 reference corner box, 2 two-thirds rule
-    int pad;
+    int nzp;              // kz stride of work fields and the RK accumulator:
+                          // nzc rounded up to a multiple of 128 bytes
 };
 
 // Forcing band slots: modes with |kx|,|ky| <= kb and 0 <= kz <= kb.
@@ -69,10 +70,12 @@ struct RkParams {
 
 // Fused x pass: forward x-FFT of the 6 (giver: 9) products, nonlinear term, RHS and the
 // Lawson RK4 stage algebra, next stage input, dealias, inverse x-FFT.
+// x_tile_columns selects the tile width: the LDS-budget width, or half of it
+// (smaller blocks, more of them resident per CU).
 template <typename T>
 hipError_t launch_x_nlin(const Grid &g, const Band &b, const RkParams &rk, void *const *P, double *const *V,
                          double *const *acc, void *const *U, const double *slots_in, double *slots_out,
-                         double *const *out, const void *twx, hipStream_t st);
+                         double *const *out, const void *twx, hipStream_t st, int x_tile_columns = 0);
 
 hipError_t launch_reality(const Grid &g, double *const *V, int ncomp, hipStream_t st);
 hipError_t launch_dealias(const Grid &g, double *const *V, int ncomp, hipStream_t st);

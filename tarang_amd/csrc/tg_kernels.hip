@@ -128,22 +128,49 @@ __device__ __forceinline__ void stage_out(const cpx<S> *lds, cpx<G> *__restrict_
     }
 }
 
-// Column of the tile handled by line c of this block: contiguous blocks of
-// CB columns, or (patch) the kz = 0 and kz = Nz/2 columns of every ky.
+// Tiles of the x pass.  Work fields store each ky row of kz with the padded
+// stride nzp (a multiple of 128 bytes); the FP64 state in the boundary layout
+// uses nzc = Nz/2+1.  A tile is CB consecutive columns of the padded (ly,lz)
+// space (pad columns lz >= nzc are skipped), or -- patch tiles -- the
+// kz = 0 and kz = Nz/2 columns of every ky.  colw = ly*nzp + lz indexes the
+// work fields, cols = ly*nzc + lz the state.
+//
+// Blocks are dispatched to the 8 XCDs round-robin; xcd_tile() hands each XCD
+// a contiguous range of tiles so that neighbouring tiles (which share the
+// 128-byte lines of the unpadded state rows) run on the same L2.
+__device__ __forceinline__ int xcd_tile(int b, int nb)
+{
+    const int per = nb / 8;
+    return (b % 8) * per + b / 8;
+}
+
+__host__ __device__ inline int x_tiles(const Grid &g, int CB, bool patch)
+{
+    return patch ? (2 * g.ny + CB - 1) / CB : (g.ny * g.nzp + CB - 1) / CB;
+}
+
+__host__ inline int x_grid(int ntiles) { return 8 * ((ntiles + 7) / 8); }
+
 template <int CB>
-__device__ __forceinline__ void tile_columns(int *cols, const Grid &g, bool patch)
+__device__ __forceinline__ void tile_columns(int *colw, int *cols, const Grid &g, bool patch, int tile)
 {
     if (threadIdx.x < CB) {
         const int c = threadIdx.x;
-        int col;
+        int w = -1, st = -1;
         if (!patch) {
-            col = blockIdx.x * CB + c;
-            if (col >= g.ny * g.nzc) col = -1;
+            const int cw = tile * CB + c;
+            const int ly = cw / g.nzp, lz = cw % g.nzp;
+            if (cw < g.ny * g.nzp && lz < g.nzc) { w = cw; st = ly * g.nzc + lz; }
         } else {
-            const int p = blockIdx.x * CB + c;
-            col = p < 2 * g.ny ? (p >> 1) * g.nzc + ((p & 1) ? g.nz / 2 : 0) : -1;
+            const int p = tile * CB + c;
+            if (p < 2 * g.ny) {
+                const int ly = p >> 1, lz = (p & 1) ? g.nz / 2 : 0;
+                w = ly * g.nzp + lz;
+                st = ly * g.nzc + lz;
+            }
         }
-        cols[c] = col;
+        colw[c] = w;
+        cols[c] = st;
     }
 }
 
@@ -180,27 +207,30 @@ __global__ void __launch_bounds__(CB *(L / 8)) k_xinv_state(XinvArgs a)
     constexpr int TPL = L / 8;
     constexpr int LP = lpad_len(L);
     __shared__ __attribute__((aligned(16))) char raw[CB * LP * sizeof(cpx<double>)];
-    __shared__ int cols[CB];
+    __shared__ int colw[CB], cols[CB];
     cpx<double> *ldd = (cpx<double> *)raw;
     cpx<T> *ldt = (cpx<T> *)raw;
     const Grid &g = a.g;
-    tile_columns<CB>(cols, g, (a.flags & 4) != 0);
+    const bool patch = (a.flags & 4) != 0;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    if (tile >= x_tiles(g, CB, patch)) return;
+    tile_columns<CB>(colw, cols, g, patch, tile);
     __syncthreads();
     const int c = threadIdx.x / TPL, j = threadIdx.x % TPL;
-    const int col = cols[c];
-    const int ly = col >= 0 ? col / g.nzc : 0, lz = col >= 0 ? col % g.nzc : 0;
+    const int cw = colw[c];
+    const int ly = cw >= 0 ? cw / g.nzp : 0, lz = cw >= 0 ? cw % g.nzp : 0;
     const int ky = kwave(ly, g.ny);
     const cpx<T> *tw = (const cpx<T> *)a.tw;
-    const int ncol = g.ny * g.nzc;
+    const int ncw = g.ny * g.nzp, ncs = g.ny * g.nzc;
     for (int comp = 0; comp < a.ncomp; comp++) {
-        stage_in<L, CB>(ldd, (const cpx<double> *)a.V[comp], cols, ncol);
+        stage_in<L, CB>(ldd, (const cpx<double> *)a.V[comp], cols, ncs);
         __syncthreads();
         cpx<T> v[8];
 #pragma unroll
         for (int s = 0; s < 8; s++) {
             const int lx = j + TPL * s;
             cpx<double> d = ldd[c * LP + lpad(lx)];
-            if (col >= 0 && (a.flags & 2)) {
+            if (cw >= 0 && (a.flags & 2)) {
                 const int kx = kwave(lx, g.nx);
                 const int sl = band_slot(a.b, kx, ky, lz);
                 if (sl >= 0) {
@@ -220,7 +250,7 @@ __global__ void __launch_bounds__(CB *(L / 8)) k_xinv_state(XinvArgs a)
 #pragma unroll
         for (int s = 0; s < 8; s++) ldt[c * LP + lpad(j + TPL * s)] = v[s];
         __syncthreads();
-        stage_out<L, CB>(ldt, (cpx<T> *)a.U[comp], cols, ncol);
+        stage_out<L, CB>(ldt, (cpx<T> *)a.U[comp], colw, ncw);
         __syncthreads();
     }
 }
@@ -244,16 +274,18 @@ __global__ void __launch_bounds__(CB *(L / 8)) k_xfwd_spec(XfwdArgs a)
     constexpr int TPL = L / 8;
     constexpr int LP = lpad_len(L);
     __shared__ __attribute__((aligned(16))) char raw[CB * LP * sizeof(cpx<double>)];
-    __shared__ int cols[CB];
+    __shared__ int colw[CB], cols[CB];
     cpx<double> *ldd = (cpx<double> *)raw;
     cpx<T> *ldt = (cpx<T> *)raw;
-    tile_columns<CB>(cols, a.g, false);
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    if (tile >= x_tiles(a.g, CB, false)) return;
+    tile_columns<CB>(colw, cols, a.g, false, tile);
     __syncthreads();
     const int c = threadIdx.x / TPL, j = threadIdx.x % TPL;
     const cpx<T> *tw = (const cpx<T> *)a.tw;
-    const int ncol = a.g.ny * a.g.nzc;
+    const int ncw = a.g.ny * a.g.nzp, ncs = a.g.ny * a.g.nzc;
     for (int comp = 0; comp < a.ncomp; comp++) {
-        stage_in<L, CB>(ldt, (const cpx<T> *)a.W[comp], cols, ncol);
+        stage_in<L, CB>(ldt, (const cpx<T> *)a.W[comp], colw, ncw);
         __syncthreads();
         cpx<T> v[8];
 #pragma unroll
@@ -264,7 +296,7 @@ __global__ void __launch_bounds__(CB *(L / 8)) k_xfwd_spec(XfwdArgs a)
 #pragma unroll
         for (int s = 0; s < 8; s++) ldd[c * LP + lpad(j + TPL * s)] = a.scale * cvt<double>(v[s]);
         __syncthreads();
-        stage_out<L, CB>(ldd, (cpx<double> *)a.out[comp], cols, ncol);
+        stage_out<L, CB>(ldd, (cpx<double> *)a.out[comp], cols, ncs);
         __syncthreads();
     }
 }
@@ -287,7 +319,7 @@ __device__ __forceinline__ void zline_c2r(cpx<T> *v, const cpx<T> *__restrict__ 
 #pragma unroll
     for (int s = 0; s < 8; s++) lds[lpad(j + TPL * s)] = X[j + TPL * s];
     if (j == 0) lds[lpad(M)] = X[M];
-    __syncthreads();
+    line_sync<M / 8>();
 #pragma unroll
     for (int s = 0; s < 8; s++) {
         const int k = j + TPL * s;
@@ -300,7 +332,7 @@ __device__ __forceinline__ void zline_c2r(cpx<T> *v, const cpx<T> *__restrict__ 
         const cpx<T> e = mk<T>(d.x * w.x + d.y * w.y, d.y * w.x - d.x * w.y);
         v[s] = a + mk<T>(-e.y, e.x);
     }
-    __syncthreads();
+    line_sync<M / 8>();
     fft_line<M, +1>(v, lds, twm, j);
 }
 
@@ -313,7 +345,7 @@ __device__ __forceinline__ void zline_r2c_post(cpx<T> *v, cpx<T> *__restrict__ X
     fft_line<M, -1>(v, lds, twm, j);
 #pragma unroll
     for (int s = 0; s < 8; s++) lds[lpad(j + TPL * s)] = v[s];
-    __syncthreads();
+    line_sync<M / 8>();
     const T h = (T)0.5;
 #pragma unroll
     for (int s = 0; s < 8; s++) {
@@ -327,7 +359,7 @@ __device__ __forceinline__ void zline_r2c_post(cpx<T> *v, cpx<T> *__restrict__ X
         if (store) X[k] = mk<T>(h * (a.x + e.y), h * (a.y - e.x));
     }
     if (store && j == 0) X[M] = mk<T>(v[0].x - v[0].y, (T)0);
-    __syncthreads();
+    line_sync<M / 8>();
 }
 
 struct ZArgs {
@@ -356,7 +388,7 @@ __global__ void __launch_bounds__(256) k_z_c2r(ZArgs a)
     const size_t nlines = (size_t)a.g.nx * a.g.ny;
     const size_t lsafe = line < nlines ? line : nlines - 1;
     for (int comp = 0; comp < a.ncomp; comp++) {
-        const cpx<T> *X = (const cpx<T> *)a.in[comp] + lsafe * a.g.nzc;
+        const cpx<T> *X = (const cpx<T> *)a.in[comp] + lsafe * a.g.nzp;
         cpx<T> v[8];
         zline_c2r<T, M>(v, X, lds + li * LP, (const cpx<T> *)a.twm, (const cpx<T> *)a.twn, j);
         if (line < nlines) {
@@ -368,7 +400,7 @@ __global__ void __launch_bounds__(256) k_z_c2r(ZArgs a)
             }
             if (j == 0) ((cpx<double> *)R)[M] = mk<double>(0, 0);
         }
-        __syncthreads();
+        line_sync<M / 8>();
     }
 }
 
@@ -388,7 +420,7 @@ __global__ void __launch_bounds__(256) k_z_r2c(ZArgs a)
         cpx<T> v[8];
 #pragma unroll
         for (int s = 0; s < 8; s++) v[s] = cvt<T>(R[j + TPL * s]);
-        cpx<T> *X = (cpx<T> *)a.out[comp] + lsafe * a.g.nzc;
+        cpx<T> *X = (cpx<T> *)a.out[comp] + lsafe * a.g.nzp;
         zline_r2c_post<T, M>(v, X, lds + li * LP, (const cpx<T> *)a.twm, (const cpx<T> *)a.twn, j, line < nlines);
     }
 }
@@ -414,11 +446,11 @@ __global__ void __launch_bounds__(256) k_z_products(ZArgs a)
     cpx<T> u[3][8];
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-        zline_c2r<T, M>(u[c], (const cpx<T> *)a.in[c] + lsafe * a.g.nzc, L, twm, twn, j);
-        __syncthreads();
+        zline_c2r<T, M>(u[c], (const cpx<T> *)a.in[c] + lsafe * a.g.nzp, L, twm, twn, j);
+        line_sync<M / 8>();
     }
     auto emit = [&](cpx<T> *p, int o) {
-        cpx<T> *X = (cpx<T> *)a.out[o] + lsafe * a.g.nzc;
+        cpx<T> *X = (cpx<T> *)a.out[o] + lsafe * a.g.nzp;
         zline_r2c_post<T, M>(p, X, L, twm, twn, j, valid);
     };
     if (!GIVER) {
@@ -435,8 +467,8 @@ __global__ void __launch_bounds__(256) k_z_products(ZArgs a)
 #pragma unroll 1
         for (int i = 0; i < 3; i++) {
             cpx<T> w[8];
-            zline_c2r<T, M>(w, (const cpx<T> *)a.in[3 + i] + lsafe * a.g.nzc, L, twm, twn, j);
-            __syncthreads();
+            zline_c2r<T, M>(w, (const cpx<T> *)a.in[3 + i] + lsafe * a.g.nzp, L, twm, twn, j);
+            line_sync<M / 8>();
 #pragma unroll
             for (int jj = 0; jj < 3; jj++) {
                 cpx<T> p[8];
@@ -491,37 +523,64 @@ __device__ double band_alpha(const Grid &g, const Band &b, const double *slots)
 // reference's diag/offdiag accumulation (compute_diag_offdiag.cc:47-112 for
 // NS, :329-414 for the giver form with P[3i+j] = FT[u_j w_i]).
 __constant__ int kNsOrder[6] = {0, 1, 2, 3, 4, 5};
+// NS products xx, yy, zz, xy, yz, zx: derivative applied for N_0, N_1, N_2
+// (Compute_nlin_diag, then Compute_nlin_offdiag's Add_*deriv calls).
+__constant__ int kNsSel[6][3] = {{1, 0, 0}, {0, 2, 0}, {0, 0, 3}, {2, 1, 0}, {0, 3, 2}, {3, 0, 1}};
 __constant__ int kGiverOrder[9] = {0, 4, 8, 3, 1, 7, 5, 2, 6};
 
 template <typename T, int L, int CB, bool GIVER>
-__global__ void __launch_bounds__(CB *(L / 8)) k_x_nlin(XnlArgs a)
+__global__ void __launch_bounds__(CB *(L / 8), (CB * (L / 8) >= 512 ? 2 : 2)) k_x_nlin(XnlArgs a)
 {
     constexpr int TPL = L / 8;
     constexpr int LP = lpad_len(L);
-    __shared__ __attribute__((aligned(16))) char raw[CB * LP * sizeof(cpx<double>)];
-    __shared__ int cols[CB];
+    // two FP64 tile buffers: A stages inputs and hosts the FFT exchanges, B
+    // holds the state/accumulator tile of the RK update
+    __shared__ __attribute__((aligned(16))) char raw[2 * CB * LP * sizeof(cpx<double>)];
+    __shared__ int colw[CB], cols[CB];
+    // twiddles are read from LDS: kept out of registers across the product loop
+    __shared__ cpx<T> twl[L];
     cpx<double> *ldd = (cpx<double> *)raw;
     cpx<T> *ldt = (cpx<T> *)raw;
+    cpx<double> *ldb = (cpx<double> *)raw + CB * LP;
     const Grid &g = a.g;
-    tile_columns<CB>(cols, g, false);
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    if (tile >= x_tiles(g, CB, false)) return;
+    tile_columns<CB>(colw, cols, g, false, tile);
     __syncthreads();
     const int c = threadIdx.x / TPL, j = threadIdx.x % TPL;
-    const int col = cols[c];
-    const int ly = col >= 0 ? col / g.nzc : 0, lz = col >= 0 ? col % g.nzc : 0;
+    const int cw = colw[c];
+    const int ly = cw >= 0 ? cw / g.nzp : 0, lz = cw >= 0 ? cw % g.nzp : 0;
     const int ky = kwave(ly, g.ny);
     const double Ky = ky * g.kf[1], Kz = lz * g.kf[2];
-    const cpx<T> *tw = (const cpx<T> *)a.tw;
-    const int ncol = g.ny * g.nzc;
+    for (int i = threadIdx.x; i < L; i += CB * TPL) twl[i] = ((const cpx<T> *)a.tw)[i];
+    const cpx<T> *tw = twl;
+    const int ncw = g.ny * g.nzp, ncs = g.ny * g.nzc;
     const T sc = (T)a.rk.scale;
     const T Kyt = (T)Ky, Kzt = (T)Kz;
 
-    // --- forward x-FFT of the six products and the divergence-form sum ---
+    // --- forward x-FFT of the products and the divergence-form sum ---
+    // Product p adds i K_d FT[p] to N_i for the (i, d) pairs of the
+    // divergence form; sel_i = 0 (none), 1 (Kx), 2 (Ky), 3 (Kz).  Summing
+    // from zero in this order reproduces the reference's accumulation order.
     cpx<T> N[3][8];
+#pragma unroll
+    for (int s = 0; s < 8; s++) N[0][s] = N[1][s] = N[2][s] = mk<T>(0, 0);
     constexpr int NPROD = GIVER ? 9 : 6;
 #pragma unroll 1
     for (int pi = 0; pi < NPROD; pi++) {
         const int p = GIVER ? kGiverOrder[pi] : kNsOrder[pi];
-        stage_in<L, CB>(ldt, (const cpx<T> *)a.P[p], cols, ncol);
+        int sel0, sel1, sel2;
+        if (GIVER) {
+            // P[3i+j] = FT[u_j w_i] contributes i K_j to N_i
+            sel0 = p / 3 == 0 ? p % 3 + 1 : 0;
+            sel1 = p / 3 == 1 ? p % 3 + 1 : 0;
+            sel2 = p / 3 == 2 ? p % 3 + 1 : 0;
+        } else {
+            sel0 = kNsSel[p][0];
+            sel1 = kNsSel[p][1];
+            sel2 = kNsSel[p][2];
+        }
+        stage_in<L, CB>(ldt, (const cpx<T> *)a.P[p], colw, ncw);
         __syncthreads();
         cpx<T> v[8];
 #pragma unroll
@@ -532,49 +591,31 @@ __global__ void __launch_bounds__(CB *(L / 8)) k_x_nlin(XnlArgs a)
         for (int s = 0; s < 8; s++) {
             const T Kx = (T)(kwave(j + TPL * s, g.nx) * g.kf[0]);
             const cpx<T> q = sc * v[s];
-            if (!GIVER) {
-                // Compute_nlin_diag then Compute_nlin_offdiag accumulation order
-                switch (p) {
-                    case 0: N[0][s] = mul_i(Kx, q); break;
-                    case 1: N[1][s] = mul_i(Kyt, q); break;
-                    case 2: N[2][s] = mul_i(Kzt, q); break;
-                    case 3: N[0][s] = N[0][s] + mul_i(Kyt, q); N[1][s] = N[1][s] + mul_i(Kx, q); break;
-                    case 4: N[1][s] = N[1][s] + mul_i(Kzt, q); N[2][s] = N[2][s] + mul_i(Kyt, q); break;
-                    default: N[0][s] = N[0][s] + mul_i(Kzt, q); N[2][s] = N[2][s] + mul_i(Kx, q); break;
-                }
-            } else {
-                // N_i += i K_j FT[u_j w_i] for p = 3i + j
-                const int ii = p / 3, jj = p % 3;
-                const T Kj = jj == 0 ? Kx : (jj == 1 ? Kyt : Kzt);
-                const cpx<T> t = mul_i(Kj, q);
-                if (pi < 3) {
-                    if (ii == 0) N[0][s] = t;
-                    else if (ii == 1) N[1][s] = t;
-                    else N[2][s] = t;
-                } else {
-                    if (ii == 0) N[0][s] = N[0][s] + t;
-                    else if (ii == 1) N[1][s] = N[1][s] + t;
-                    else N[2][s] = N[2][s] + t;
-                }
-            }
+            const T K0 = sel0 == 1 ? Kx : (sel0 == 2 ? Kyt : (sel0 == 3 ? Kzt : (T)0));
+            const T K1 = sel1 == 1 ? Kx : (sel1 == 2 ? Kyt : (sel1 == 3 ? Kzt : (T)0));
+            const T K2 = sel2 == 1 ? Kx : (sel2 == 2 ? Kyt : (sel2 == 3 ? Kzt : (T)0));
+            N[0][s] = N[0][s] + mul_i(K0, q);
+            N[1][s] = N[1][s] + mul_i(K1, q);
+            N[2][s] = N[2][s] + mul_i(K2, q);
         }
         __syncthreads();
     }
 
     const int stage = a.rk.stage;
     const bool want_rhs = stage > 0 || a.rk.write_rhs;
-    cpx<double> R[3][8];
-    // --- force, pressure projection, negation (Compute_force_TO_rhs) ---
+    // --- force, pressure projection, negation (Compute_force_TO_rhs), in
+    // place on N: N <- -(N - F + i K p), p = i K.(N - F) / K^2 (computed in
+    // FP64, stored in the transform precision) ---
     double alpha = 0.0;
     bool alpha_done = false;
+    if (want_rhs) {
 #pragma unroll
-    for (int s = 0; s < 8; s++) {
-        const int lx = j + TPL * s;
-        const int kx = kwave(lx, g.nx);
-        const double Kx = kx * g.kf[0];
-        cpx<double> n0 = cvt<double>(N[0][s]), n1 = cvt<double>(N[1][s]), n2 = cvt<double>(N[2][s]);
-        if (want_rhs) {
-            if (a.b.kind == 1 && col >= 0) {
+        for (int s = 0; s < 8; s++) {
+            const int lx = j + TPL * s;
+            const int kx = kwave(lx, g.nx);
+            const double Kx = kx * g.kf[0];
+            cpx<double> n0 = cvt<double>(N[0][s]), n1 = cvt<double>(N[1][s]), n2 = cvt<double>(N[2][s]);
+            if (a.b.kind == 1 && cw >= 0) {
                 const int sl = band_slot(a.b, kx, ky, lz);
                 if (sl >= 0) {
                     const double K = kmag(g, kx, ky, lz);
@@ -597,104 +638,105 @@ __global__ void __launch_bounds__(CB *(L / 8)) k_x_nlin(XnlArgs a)
             n0 = n0 + mul_i(Kx, pr);
             n1 = n1 + mul_i(Ky, pr);
             n2 = n2 + mul_i(Kz, pr);
-            n0 = mk<double>(-n0.x, -n0.y);
-            n1 = mk<double>(-n1.x, -n1.y);
-            n2 = mk<double>(-n2.x, -n2.y);
+            N[0][s] = cvt<T>(mk<double>(-n0.x, -n0.y));
+            N[1][s] = cvt<T>(mk<double>(-n1.x, -n1.y));
+            N[2][s] = cvt<T>(mk<double>(-n2.x, -n2.y));
         }
-        R[0][s] = n0;
-        R[1][s] = n1;
-        R[2][s] = n2;
     }
+#define TG_TAKE(comp, nc)                                                           \
+    do {                                                                            \
+        if ((comp) == 0) {                                                          \
+            _Pragma("unroll") for (int s_ = 0; s_ < 8; s_++) nc[s_] = N[0][s_];    \
+        } else if ((comp) == 1) {                                                   \
+            _Pragma("unroll") for (int s_ = 0; s_ < 8; s_++) nc[s_] = N[1][s_];    \
+        } else {                                                                    \
+            _Pragma("unroll") for (int s_ = 0; s_ < 8; s_++) nc[s_] = N[2][s_];    \
+        }                                                                           \
+    } while (0)
 
     if (stage == 0) {
-        // Compute_nlin (and optionally Compute_rhs) output, FP64
+        // Compute_nlin (and optionally Compute_rhs) output, FP64 boundary layout
 #pragma unroll 1
         for (int comp = 0; comp < 3; comp++) {
+            cpx<T> nc[8];
+            TG_TAKE(comp, nc);
 #pragma unroll
-            for (int s = 0; s < 8; s++) ldd[c * LP + lpad(j + TPL * s)] = R[comp][s];
+            for (int s = 0; s < 8; s++) ldd[c * LP + lpad(j + TPL * s)] = cvt<double>(nc[s]);
             __syncthreads();
-            stage_out<L, CB>(ldd, (cpx<double> *)a.out[comp], cols, ncol);
+            stage_out<L, CB>(ldd, (cpx<double> *)a.out[comp], cols, ncs);
             __syncthreads();
         }
         return;
     }
 
     // --- Lawson RK4 stage algebra (RK.cc:91-170, Tadvance_functions.cc:298-324,566-570) ---
+    // V: FP64 state, boundary layout; acc: FP64, padded work layout.
     const double dt = a.rk.dt;
     const double c_half = 0.5 * dt, c_one = 1.0 * dt;
     const double c_sixth = (1.0 / 6) * dt, c_third = (1.0 / 3) * dt;
 #pragma unroll 1
     for (int comp = 0; comp < 3; comp++) {
-        cpx<double> v0[8], ac[8], vn[8];
-        stage_in<L, CB>(ldd, (const cpx<double> *)a.V[comp], cols, ncol);
+        cpx<T> nc[8];
+        TG_TAKE(comp, nc);
+        stage_in<L, CB>(ldd, (const cpx<double> *)a.V[comp], cols, ncs);
+        if (stage >= 2) stage_in<L, CB>(ldb, (const cpx<double> *)a.acc[comp], colw, ncw);
         __syncthreads();
-#pragma unroll
-        for (int s = 0; s < 8; s++) v0[s] = ldd[c * LP + lpad(j + TPL * s)];
-        __syncthreads();
-        if (stage >= 2) {
-            stage_in<L, CB>(ldd, (const cpx<double> *)a.acc[comp], cols, ncol);
-            __syncthreads();
-#pragma unroll
-            for (int s = 0; s < 8; s++) ac[s] = ldd[c * LP + lpad(j + TPL * s)];
-            __syncthreads();
-        }
-#pragma unroll
-        for (int s = 0; s < 8; s++) {
-            const int lx = j + TPL * s;
-            const double Kx = kwave(lx, g.nx) * g.kf[0];
-            const double kxs = Kx * Kx;
-            const double ks = (kxs + Ky * Ky) + Kz * Kz;
-            const double Eh = exp(a.rk.fh * ks);
-            const cpx<double> r = R[comp][s];
-            if (stage == 1) {
-                cpx<double> d0 = v0[s];
-                if (dealias_zero(g, lx, ly, lz)) d0 = mk<double>(0, 0);
-                vn[s] = Eh * (d0 + c_half * r);
-                const double E1 = exp(a.rk.f1 * ks);
-                ac[s] = c_sixth * (E1 * r);
-            } else if (stage == 2) {
-                vn[s] = Eh * v0[s] + c_half * r;
-                ac[s] = ac[s] + c_third * (Eh * r);
-            } else if (stage == 3) {
-                vn[s] = Eh * (Eh * v0[s] + c_one * r);
-                ac[s] = ac[s] + c_third * (Eh * r);
-            } else {
-                ac[s] = ac[s] + c_sixth * r;
-                const double E1 = exp(a.rk.f1 * ks);
-                vn[s] = E1 * v0[s] + ac[s];
-            }
-        }
-        // accumulator (stages 1-3) or the new state (stage 4)
-#pragma unroll
-        for (int s = 0; s < 8; s++) ldd[c * LP + lpad(j + TPL * s)] = stage == 4 ? vn[s] : ac[s];
-        __syncthreads();
-        stage_out<L, CB>(ldd, (cpx<double> *)(stage == 4 ? a.V[comp] : a.acc[comp]), cols, ncol);
-        __syncthreads();
-        // next stage input: band slots, dealias, inverse x-FFT
         cpx<T> v[8];
 #pragma unroll
         for (int s = 0; s < 8; s++) {
             const int lx = j + TPL * s;
-            cpx<double> d = vn[s];
-            if (a.b.kind == 1 && col >= 0) {
+            const int li = c * LP + lpad(lx);
+            const double Kx = kwave(lx, g.nx) * g.kf[0];
+            const double kxs = Kx * Kx;
+            const double ks = (kxs + Ky * Ky) + Kz * Kz;
+            const double Eh = exp(a.rk.fh * ks);
+            const cpx<double> r = cvt<double>(nc[s]);
+            const cpx<double> v0 = ldd[li];
+            cpx<double> vn, ac;
+            if (stage == 1) {
+                cpx<double> d0 = v0;
+                if (dealias_zero(g, lx, ly, lz)) d0 = mk<double>(0, 0);
+                const double E1 = exp(a.rk.f1 * ks);
+                ac = c_sixth * (E1 * r);
+                vn = Eh * (d0 + c_half * r);
+            } else if (stage == 2) {
+                ac = ldb[li] + c_third * (Eh * r);
+                vn = Eh * v0 + c_half * r;
+            } else if (stage == 3) {
+                ac = ldb[li] + c_third * (Eh * r);
+                vn = Eh * (Eh * v0 + c_one * r);
+            } else {
+                ac = ldb[li] + c_sixth * r;
+                const double E1 = exp(a.rk.f1 * ks);
+                vn = E1 * v0 + ac;
+            }
+            // accumulator (stages 1-3) or the new state (stage 4) back to B
+            ldb[li] = stage == 4 ? vn : ac;
+            // next stage input: band slots, dealias
+            if (a.b.kind == 1 && cw >= 0) {
                 const int kx = kwave(lx, g.nx);
                 const int sl = band_slot(a.b, kx, ky, lz);
                 if (sl >= 0) {
                     const double K = kmag(g, kx, ky, lz);
-                    if (K > a.b.r_in && K <= a.b.r_out) ((cpx<double> *)a.slots_out)[sl * 3 + comp] = d;
+                    if (K > a.b.r_in && K <= a.b.r_out) ((cpx<double> *)a.slots_out)[sl * 3 + comp] = vn;
                 }
             }
-            if (dealias_zero(g, lx, ly, lz)) d = mk<double>(0, 0);
-            v[s] = cvt<T>(d);
+            if (dealias_zero(g, lx, ly, lz)) vn = mk<double>(0, 0);
+            v[s] = cvt<T>(vn);
         }
+        __syncthreads();
+        if (stage == 4) stage_out<L, CB>(ldb, (cpx<double> *)a.V[comp], cols, ncs);
+        else stage_out<L, CB>(ldb, (cpx<double> *)a.acc[comp], colw, ncw);
+        // inverse x-FFT of the next stage input (exchanges in buffer A)
         fft_line<L, +1>(v, ldt + c * LP, tw, j);
 #pragma unroll
         for (int s = 0; s < 8; s++) ldt[c * LP + lpad(j + TPL * s)] = v[s];
         __syncthreads();
-        stage_out<L, CB>(ldt, (cpx<T> *)a.U[comp], cols, ncol);
+        stage_out<L, CB>(ldt, (cpx<T> *)a.U[comp], colw, ncw);
         __syncthreads();
     }
 }
+#undef TG_TAKE
 
 // ---------------------------------------------------------------------------
 // Strong reality condition (FFF_pencil_basic.cc:139-179) and dealias.
@@ -769,15 +811,15 @@ hipError_t launch_colfft(const Grid &g, int axis, int dir, void *const *fields, 
     if (axis == 1) {
         L = g.ny;
         a.nouter = g.nx;
-        a.outer_stride = (long long)g.ny * g.nzc;
-        a.row_stride = g.nzc;
+        a.outer_stride = (long long)g.ny * g.nzp;
+        a.row_stride = g.nzp;
         a.ncol = g.nzc;
     } else {
         L = g.nx;
         a.nouter = 1;
         a.outer_stride = 0;
-        a.row_stride = (long long)g.ny * g.nzc;
-        a.ncol = g.ny * g.nzc;
+        a.row_stride = (long long)g.ny * g.nzp;
+        a.ncol = g.ny * g.nzp;
     }
     TG_DISPATCH_L(L, {
         constexpr int CB = col_block<T>(LL, sizeof(cpx<T>));
@@ -804,8 +846,7 @@ hipError_t launch_xinv_state(const Grid &g, const Band &b, double *const *V, int
     a.flags = flags;
     TG_DISPATCH_L(g.nx, {
         constexpr int CB = col_block<T>(LL, sizeof(cpx<double>));
-        const int ncolumns = (flags & 4) ? 2 * g.ny : g.ny * g.nzc;
-        dim3 grid((ncolumns + CB - 1) / CB);
+        dim3 grid(x_grid(x_tiles(g, CB, (flags & 4) != 0)));
         hipLaunchKernelGGL((k_xinv_state<T, LL, CB>), grid, dim3(CB * (LL / 8)), 0, st, a);
     });
     return hipGetLastError();
@@ -823,7 +864,7 @@ hipError_t launch_xfwd_spec(const Grid &g, void *const *W, int ncomp, double *co
     a.ncomp = ncomp;
     TG_DISPATCH_L(g.nx, {
         constexpr int CB = col_block<T>(LL, sizeof(cpx<double>));
-        dim3 grid((g.ny * g.nzc + CB - 1) / CB);
+        dim3 grid(x_grid(x_tiles(g, CB, false)));
         hipLaunchKernelGGL((k_xfwd_spec<T, LL, CB>), grid, dim3(CB * (LL / 8)), 0, st, a);
     });
     return hipGetLastError();
@@ -902,7 +943,7 @@ hipError_t launch_z_products(const Grid &g, void *const *U, void *const *P, int 
 template <typename T>
 hipError_t launch_x_nlin(const Grid &g, const Band &b, const RkParams &rk, void *const *P, double *const *V,
                          double *const *acc, void *const *U, const double *slots_in, double *slots_out,
-                         double *const *out, const void *twx, hipStream_t st)
+                         double *const *out, const void *twx, hipStream_t st, int x_tile_columns)
 {
     XnlArgs a{};
     const bool giver = rk.stage < 0;
@@ -920,14 +961,19 @@ hipError_t launch_x_nlin(const Grid &g, const Band &b, const RkParams &rk, void 
     a.b = b;
     a.rk = rk;
     TG_DISPATCH_L(g.nx, {
-        constexpr int CB = col_block<T>(LL, sizeof(cpx<double>));
-        dim3 grid((g.ny * g.nzc + CB - 1) / CB);
+        // two FP64 tile buffers per block: the LDS-budget width (two blocks
+        // per CU), or twice that width (one block per CU)
+        constexpr int CB = col_block<T>(LL, 2 * sizeof(cpx<double>));
+        constexpr int CBW = 2 * CB * (LL / 8) <= 1024 ? 2 * CB : CB;
+        const bool wide = x_tile_columns == CBW && CBW != CB;
         if (giver) {
             a.rk.stage = 0;
             a.rk.write_rhs = 0;
-            hipLaunchKernelGGL((k_x_nlin<T, LL, CB, true>), grid, dim3(CB * (LL / 8)), 0, st, a);
+            if (wide) hipLaunchKernelGGL((k_x_nlin<T, LL, CBW, true>), dim3(x_grid(x_tiles(g, CBW, false))), dim3(CBW * (LL / 8)), 0, st, a);
+            else hipLaunchKernelGGL((k_x_nlin<T, LL, CB, true>), dim3(x_grid(x_tiles(g, CB, false))), dim3(CB * (LL / 8)), 0, st, a);
         } else {
-            hipLaunchKernelGGL((k_x_nlin<T, LL, CB, false>), grid, dim3(CB * (LL / 8)), 0, st, a);
+            if (wide) hipLaunchKernelGGL((k_x_nlin<T, LL, CBW, false>), dim3(x_grid(x_tiles(g, CBW, false))), dim3(CBW * (LL / 8)), 0, st, a);
+            else hipLaunchKernelGGL((k_x_nlin<T, LL, CB, false>), dim3(x_grid(x_tiles(g, CB, false))), dim3(CB * (LL / 8)), 0, st, a);
         }
     });
     return hipGetLastError();
@@ -968,7 +1014,7 @@ hipError_t launch_dealias(const Grid &g, double *const *V, int ncomp, hipStream_
                                              const void *, hipStream_t);                                       \
     template hipError_t launch_x_nlin<T>(const Grid &, const Band &, const RkParams &, void *const *,           \
                                          double *const *, double *const *, void *const *, const double *,      \
-                                         double *, double *const *, const void *, hipStream_t);
+                                         double *, double *const *, const void *, hipStream_t, int);
 
 TG_INST(double)
 TG_INST(float)
