@@ -113,6 +113,23 @@ int tg_compute_nlin_giver(tg_plan *plan, double *const V_dev[3], double *const W
  * this library since). */
 int tg_rk4_step(tg_plan *plan, double *const V_dev[3], const tg_fluid *fluid, int nsteps, int flags);
 
+/* MHD, Elsasser form.  Nlin_incompress::Compute_nlin(FluidVF& U, FluidVF& W)
+ * (compute_nlin_VF_MHD.cc:71-123): dealiases V (u) and W (b) in place;
+ * nlinV = (N+ + N-)/2 = FT[(u.grad)u - (b.grad)b], nlinW = (N+ - N-)/2
+ * = FT[(u.grad)b - (b.grad)u] in divergence form. */
+int tg_compute_nlin_mhd(tg_plan *plan, double *const V_dev[3], double *const W_dev[3], double *const nlinV_dev[3],
+                        double *const nlinW_dev[3]);
+/* Time_advance_incompress::RK4(U, W, P, F) (RK_VF_MHD.cc:126-190), nu =
+ * fluid->nu, eta for the magnetic field; no forcing (fluid->force_kind 0). */
+int tg_rk4_step_mhd(tg_plan *plan, double *const V_dev[3], double *const W_dev[3], const tg_fluid *fluid, double eta,
+                    int nsteps);
+/* Correlation::Get_cross_helicity (Correlation.cc:225-230). */
+int tg_cross_helicity(tg_plan *plan, double *const V_dev[3], double *const W_dev[3], double *hc);
+/* EnergyTr::Compute_flux(FluidVF& U, FluidVF& W) (compute_flux.cc:158-211):
+ * u2b = flux_VF_Uin_Wout, b2b = flux_VF_Win_Wout.  Dealiases V and W. */
+int tg_flux_mhd(tg_plan *plan, double *const V_dev[3], double *const W_dev[3], const double *radii, int nspheres,
+                double *u2b, double *b2b);
+
 /* Universal::Get_total_energy (universal_energy.cc:63-71) of one component:
  * sum |A|^2 - sum_{kz=0} |A|^2 / 2. */
 int tg_energy(tg_plan *plan, const double *A_dev, double *energy);

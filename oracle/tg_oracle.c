@@ -682,6 +682,178 @@ void or_euler_step(const or_grid *g, const or_fluid *f, double *V[3])
 }
 
 /* ------------------------------------------------------------------------ */
+/* MHD (Elsasser form)                                                       */
+/* ------------------------------------------------------------------------ */
+
+/* Nlin_incompress::Compute_nlin(FluidVF& U, FluidVF& W) (compute_nlin_VF_MHD.cc:71-123):
+ * dealias U, W in place; inverse; Z+ = u + b, Z- = u - b in real space
+ * (MHD::UB_to_Elsasser_real_field, MHD.cc:75-88); N+_i = sum_j d_j(Z-_j Z+_i)
+ * into NU, N-_i = sum_j d_j(Z+_j Z-_i) into NW, with the diag/offdiag order
+ * of compute_diag_offdiag.cc:139-257; then NU = (N+ + N-)/2, NW = (N+ - N-)/2
+ * (MHD::Elsasser_to_UB_nlin, MHD.cc:150-163). */
+void or_compute_nlin_mhd(const or_grid *g, double *V[3], double *W[3], double *NU[3], double *NW[3])
+{
+    const size_t nr = nreal(g), nc = ncplx(g);
+    for (int c = 0; c < 3; c++) { or_dealias(g, V[c]); or_dealias(g, W[c]); }
+    double *zp[3], *zm[3], *p = (double *)malloc(sizeof(double) * nr);
+    for (int c = 0; c < 3; c++) {
+        zp[c] = (double *)malloc(sizeof(double) * nr);
+        zm[c] = (double *)malloc(sizeof(double) * nr);
+        or_inverse(g, V[c], zp[c]);
+        or_inverse(g, W[c], zm[c]);
+        for (size_t i = 0; i < nr; i++) {
+            double u = zp[c][i], b = zm[c][i];
+            zp[c][i] = u + b;
+            zm[c][i] = u - b;
+        }
+    }
+    cplx *X = (cplx *)malloc(sizeof(cplx) * nc);
+    cplx *u1 = (cplx *)NU[0], *u2 = (cplx *)NU[1], *u3 = (cplx *)NU[2];
+    cplx *w1 = (cplx *)NW[0], *w2 = (cplx *)NW[1], *w3 = (cplx *)NW[2];
+    /* diag */
+    real_mult(g, zp[0], zm[0], p); or_forward(g, p, NU[0]); deriv(g, 0, u1, u1, 0);
+    real_mult(g, zp[1], zm[1], p); or_forward(g, p, NU[1]); deriv(g, 1, u2, u2, 0);
+    real_mult(g, zp[2], zm[2], p); or_forward(g, p, NU[2]); deriv(g, 2, u3, u3, 0);
+    memcpy(w1, u1, sizeof(cplx) * nc); memcpy(w2, u2, sizeof(cplx) * nc); memcpy(w3, u3, sizeof(cplx) * nc);
+    /* offdiag */
+    real_mult(g, zp[0], zm[1], p); or_forward(g, p, (double *)X); deriv(g, 1, X, u1, 1); deriv(g, 0, X, w2, 1);
+    real_mult(g, zm[0], zp[1], p); or_forward(g, p, (double *)X); deriv(g, 1, X, w1, 1); deriv(g, 0, X, u2, 1);
+    real_mult(g, zp[1], zm[2], p); or_forward(g, p, (double *)X); deriv(g, 2, X, u2, 1); deriv(g, 1, X, w3, 1);
+    real_mult(g, zm[1], zp[2], p); or_forward(g, p, (double *)X); deriv(g, 2, X, w2, 1); deriv(g, 1, X, u3, 1);
+    real_mult(g, zp[2], zm[0], p); or_forward(g, p, (double *)X); deriv(g, 2, X, w1, 1); deriv(g, 0, X, u3, 1);
+    real_mult(g, zm[2], zp[0], p); or_forward(g, p, (double *)X); deriv(g, 2, X, u1, 1); deriv(g, 0, X, w3, 1);
+    for (int c = 0; c < 3; c++) {
+        cplx *a = (cplx *)NU[c], *b = (cplx *)NW[c];
+        for (size_t i = 0; i < nc; i++) {
+            cplx t = a[i];
+            a[i] = 0.5 * (a[i] + b[i]);
+            b[i] = 0.5 * (t - b[i]);
+        }
+    }
+    free(X); free(p);
+    for (int c = 0; c < 3; c++) { free(zp[c]); free(zm[c]); }
+}
+
+/* Compute_rhs(U, W, P) (Tadvance_functions.cc:262-270): U as the fluid, W
+ * negated (no pressure, no forcing on W). */
+static void mhd_force_to_rhs(const or_grid *g, const or_fluid *fu, double *V[3], double *W[3], double *NU[3],
+                             double *NW[3], double *F[3])
+{
+    or_compute_force(g, fu, V, F);
+    or_compute_nlin_mhd(g, V, W, NU, NW);
+    or_compute_rhs(g, NU, F);
+    const size_t nc = ncplx(g);
+    for (int c = 0; c < 3; c++) {
+        cplx *b = (cplx *)NW[c];
+        for (size_t i = 0; i < nc; i++) b[i] = -b[i];
+    }
+}
+
+/* Time_advance_incompress::RK4(U, W, P, F) (RK_VF_MHD.cc:126-190): the same
+ * Lawson stages for both fields, U with nu = fu->nu, W with eta = eta. */
+void or_rk4_step_mhd(const or_grid *g, const or_fluid *fu, double eta, double *V[3], double *W[3])
+{
+    const size_t nc = ncplx(g);
+    or_fluid fb = *fu;
+    fb.nu = eta;
+    fb.force_kind = 0;
+    double *V0[3], *W0[3], *accV[3], *accW[3], *NU[3], *NW[3], *F[3];
+    for (int c = 0; c < 3; c++) {
+        V0[c] = (double *)malloc(sizeof(cplx) * nc);
+        W0[c] = (double *)malloc(sizeof(cplx) * nc);
+        accV[c] = (double *)calloc(nc, sizeof(cplx));
+        accW[c] = (double *)calloc(nc, sizeof(cplx));
+        NU[c] = (double *)malloc(sizeof(cplx) * nc);
+        NW[c] = (double *)malloc(sizeof(cplx) * nc);
+        F[c] = (double *)malloc(sizeof(cplx) * nc);
+        memcpy(V0[c], V[c], sizeof(cplx) * nc);
+        memcpy(W0[c], W[c], sizeof(cplx) * nc);
+    }
+    mhd_force_to_rhs(g, fu, V, W, NU, NW, F);
+    single_time_step(g, fu, V, NU, 0.5, 0.5, 0.5);
+    single_time_step(g, &fb, W, NW, 0.5, 0.5, 0.5);
+    rk4_parts(g, fu, NU, accV, 1.0, 1.0 / 6);
+    rk4_parts(g, &fb, NW, accW, 1.0, 1.0 / 6);
+
+    mhd_force_to_rhs(g, fu, V, W, NU, NW, F);
+    for (int c = 0; c < 3; c++) { memcpy(V[c], V0[c], sizeof(cplx) * nc); memcpy(W[c], W0[c], sizeof(cplx) * nc); }
+    single_time_step(g, fu, V, NU, 0.5, 0, 0.5);
+    single_time_step(g, &fb, W, NW, 0.5, 0, 0.5);
+    rk4_parts(g, fu, NU, accV, 0.5, 1.0 / 3);
+    rk4_parts(g, &fb, NW, accW, 0.5, 1.0 / 3);
+
+    mhd_force_to_rhs(g, fu, V, W, NU, NW, F);
+    for (int c = 0; c < 3; c++) { memcpy(V[c], V0[c], sizeof(cplx) * nc); memcpy(W[c], W0[c], sizeof(cplx) * nc); }
+    single_time_step(g, fu, V, NU, 1, 0.5, 1);
+    single_time_step(g, &fb, W, NW, 1, 0.5, 1);
+    rk4_parts(g, fu, NU, accV, 0.5, 1.0 / 3);
+    rk4_parts(g, &fb, NW, accW, 0.5, 1.0 / 3);
+
+    mhd_force_to_rhs(g, fu, V, W, NU, NW, F);
+    rk4_parts(g, fu, NU, accV, 0, 1.0 / 6);
+    rk4_parts(g, &fb, NW, accW, 0, 1.0 / 6);
+
+    for (int c = 0; c < 3; c++) { memcpy(V[c], V0[c], sizeof(cplx) * nc); memcpy(W[c], W0[c], sizeof(cplx) * nc); }
+    mult_exp(g, fu, V, 1.0);
+    mult_exp(g, &fb, W, 1.0);
+    for (int c = 0; c < 3; c++) {
+        cplx *v = (cplx *)V[c], *w = (cplx *)W[c];
+        const cplx *a = (const cplx *)accV[c], *b = (const cplx *)accW[c];
+        for (size_t i = 0; i < nc; i++) { v[i] = v[i] + a[i]; w[i] = w[i] + b[i]; }
+    }
+    if (g->strong_reality)
+        for (int c = 0; c < 3; c++) { or_strong_reality(g, V[c]); or_strong_reality(g, W[c]); }
+    for (int c = 0; c < 3; c++) {
+        free(V0[c]); free(W0[c]); free(accV[c]); free(accW[c]); free(NU[c]); free(NW[c]); free(F[c]);
+    }
+}
+
+/* Get_local_energy(A, B) (FFF_pencil_energy.cc:75-84):
+ * sum Re(A B*) - sum_{kz=0} Re(A B*)/2; cross helicity = sum over components
+ * (Correlation::Get_cross_helicity, Correlation.cc:225-230). */
+double or_cross_energy(const or_grid *g, const double *a, const double *b)
+{
+    const cplx *A = (const cplx *)a, *B = (const cplx *)b;
+    const size_t nc = ncplx(g);
+    const int nzc = g->nz / 2 + 1;
+    double s = 0.0, s0 = 0.0;
+    for (size_t i = 0; i < nc; i++) s += creal(A[i]) * creal(B[i]) + cimag(A[i]) * cimag(B[i]);
+    for (size_t i = 0; i < nc; i += nzc) s0 += creal(A[i]) * creal(B[i]) + cimag(A[i]) * cimag(B[i]);
+    return s - s0 / 2;
+}
+
+/* EnergyTr::Compute_flux(FluidVF& U, FluidVF& W) (compute_flux.cc:158-211),
+ * the u< -> b> and b< -> b> parts:
+ *   u2b[s] = sum_{K > r_s} 2m Re(N.b*), N = FT[d_j(b_j u<_i)]
+ *   b2b[s] = -sum_{K > r_s} 2m Re(N.b*), N = FT[d_j(u_j b<_i)]. */
+void or_flux_mhd(const or_grid *g, double *V[3], double *W[3], const double *radii, int nspheres, double *u2b,
+                 double *b2b)
+{
+    const size_t nc = ncplx(g);
+    double *G[3], *N[3];
+    for (int c = 0; c < 3; c++) {
+        G[c] = (double *)malloc(sizeof(cplx) * nc);
+        N[c] = (double *)malloc(sizeof(cplx) * nc);
+    }
+    u2b[0] = b2b[0] = 0.0;
+    for (int s = 1; s <= nspheres; s++) {
+        for (int c = 0; c < 3; c++) or_fill_array_shell(g, V[c], G[c], 0.0, radii[s]);
+        or_compute_nlin_giver(g, W, G, N);
+        double p = 0.0;
+        for (int c = 0; c < 3; c++) p += or_shell_mult_single(g, N[c], W[c], radii[s], INFINITY);
+        u2b[s] = p;
+    }
+    for (int s = 1; s <= nspheres; s++) {
+        for (int c = 0; c < 3; c++) or_fill_array_shell(g, W[c], G[c], 0.0, radii[s]);
+        or_compute_nlin_giver(g, V, G, N);
+        double p = 0.0;
+        for (int c = 0; c < 3; c++) p += or_shell_mult_single(g, N[c], W[c], radii[s], INFINITY);
+        b2b[s] = -p;
+    }
+    for (int c = 0; c < 3; c++) { free(G[c]); free(N[c]); }
+}
+
+/* ------------------------------------------------------------------------ */
 /* Diagnostics                                                              */
 /* ------------------------------------------------------------------------ */
 

@@ -21,8 +21,7 @@
  * parts of the kz=0 and kz=Nz/2 coefficients of each z-line (it returns the
  * real part of the inverse DFT of the Hermitian extension).
  *
- * Parity status per function is stated in oracle/README.md-free form in the
- * header of tg_oracle.c.
+ * How each function is pinned is stated in the header comment of tg_oracle.c.
  */
 #ifndef TG_ORACLE_H
 #define TG_ORACLE_H
@@ -74,6 +73,13 @@ void or_compute_force(const or_grid *g, const or_fluid *f, double *const V[3], d
 void or_compute_rhs(const or_grid *g, double *N[3], double *const F[3]);
 void or_rk4_step(const or_grid *g, const or_fluid *f, double *V[3]);
 void or_euler_step(const or_grid *g, const or_fluid *f, double *V[3]);
+
+/* ---- MHD ---- */
+void or_compute_nlin_mhd(const or_grid *g, double *V[3], double *W[3], double *NU[3], double *NW[3]);
+void or_rk4_step_mhd(const or_grid *g, const or_fluid *fu, double eta, double *V[3], double *W[3]);
+double or_cross_energy(const or_grid *g, const double *A, const double *B);
+void or_flux_mhd(const or_grid *g, double *V[3], double *W[3], const double *radii, int nspheres, double *u2b,
+                 double *b2b);
 
 /* ---- diagnostics ---- */
 double or_energy(const or_grid *g, const double *A);

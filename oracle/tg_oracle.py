@@ -78,6 +78,10 @@ def lib():
             "or_shell_to_shell": (None, [gp, pp, p, i, p]),
             "or_default_radii": (None, [gp, i, p]),
             "or_init_field": (None, [gp, i, d, d, d, d, ctypes.c_uint64, pp]),
+            "or_compute_nlin_mhd": (None, [gp, pp, pp, pp, pp]),
+            "or_rk4_step_mhd": (None, [gp, fp, d, pp, pp]),
+            "or_cross_energy": (d, [gp, p, p]),
+            "or_flux_mhd": (None, [gp, pp, pp, p, i, p, p]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(_lib, name)
@@ -91,7 +95,15 @@ def _ptr(a):
     return a.ctypes.data_as(ctypes.c_void_p)
 
 
+def _cptr(a):
+    """Pointer to a complex128 spectral array (rejects any other dtype)."""
+    assert a.dtype == np.complex128 and a.flags["C_CONTIGUOUS"], (a.dtype, a.flags["C_CONTIGUOUS"])
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
 def _pp(arrs):
+    for a in arrs:
+        assert a.dtype == np.complex128 and a.flags["C_CONTIGUOUS"]
     return (ctypes.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
 
 
@@ -130,10 +142,10 @@ class Oracle:
         return out
 
     def dealias(self, spec):
-        self.L.or_dealias(ctypes.byref(self.g), _ptr(spec))
+        self.L.or_dealias(ctypes.byref(self.g), _cptr(spec))
 
     def strong_reality(self, spec):
-        self.L.or_strong_reality(ctypes.byref(self.g), _ptr(spec))
+        self.L.or_strong_reality(ctypes.byref(self.g), _cptr(spec))
 
     def compute_nlin(self, V):
         N = [self.czeros() for _ in range(3)]
@@ -160,10 +172,10 @@ class Oracle:
         self.L.or_euler_step(ctypes.byref(self.g), ctypes.byref(fluid), _pp(V))
 
     def energy(self, A):
-        return self.L.or_energy(ctypes.byref(self.g), _ptr(A))
+        return self.L.or_energy(ctypes.byref(self.g), _cptr(A))
 
     def Sn(self, A, n):
-        return self.L.or_Sn(ctypes.byref(self.g), _ptr(A), n)
+        return self.L.or_Sn(ctypes.byref(self.g), _cptr(A), n)
 
     def multiplicity(self, lx, ly, lz):
         return self.L.or_multiplicity(ctypes.byref(self.g), lx, ly, lz)
@@ -180,16 +192,16 @@ class Oracle:
     def shell_spectrum(self, A, n=0):
         nb = self.min_radius_outside() + 1
         Sk = np.zeros(nb)
-        self.L.or_shell_spectrum(ctypes.byref(self.g), _ptr(A), n, _ptr(Sk), nb)
+        self.L.or_shell_spectrum(ctypes.byref(self.g), _cptr(A), n, _ptr(Sk), nb)
         return Sk
 
     def fill_array_shell(self, A, r_in, r_out):
         B = self.czeros()
-        self.L.or_fill_array_shell(ctypes.byref(self.g), _ptr(A), _ptr(B), r_in, r_out)
+        self.L.or_fill_array_shell(ctypes.byref(self.g), _cptr(A), _cptr(B), r_in, r_out)
         return B
 
     def shell_mult_single(self, A, B, r_in, r_out):
-        return self.L.or_shell_mult_single(ctypes.byref(self.g), _ptr(A), _ptr(B), r_in, r_out)
+        return self.L.or_shell_mult_single(ctypes.byref(self.g), _cptr(A), _cptr(B), r_in, r_out)
 
     def shell_mult_all(self, A, B, radii):
         radii = np.ascontiguousarray(radii, np.float64)
@@ -214,6 +226,25 @@ class Oracle:
         out = np.zeros((n + 1, n + 1))
         self.L.or_shell_to_shell(ctypes.byref(self.g), _pp(V), _ptr(radii), n, _ptr(out))
         return out
+
+    def compute_nlin_mhd(self, V, W):
+        NU = [self.czeros() for _ in range(3)]
+        NW = [self.czeros() for _ in range(3)]
+        self.L.or_compute_nlin_mhd(ctypes.byref(self.g), _pp(V), _pp(W), _pp(NU), _pp(NW))
+        return NU, NW
+
+    def rk4_step_mhd(self, fluid, eta, V, W):
+        self.L.or_rk4_step_mhd(ctypes.byref(self.g), ctypes.byref(fluid), eta, _pp(V), _pp(W))
+
+    def cross_energy(self, A, B):
+        return self.L.or_cross_energy(ctypes.byref(self.g), _cptr(A), _cptr(B))
+
+    def flux_mhd(self, V, W, radii):
+        radii = np.ascontiguousarray(radii, np.float64)
+        u2b = np.zeros(len(radii))
+        b2b = np.zeros(len(radii))
+        self.L.or_flux_mhd(ctypes.byref(self.g), _pp(V), _pp(W), _ptr(radii), len(radii) - 1, _ptr(u2b), _ptr(b2b))
+        return u2b, b2b
 
     def init_field(self, kind=0, k0=4.0, kmin=2.0, kmax=170.0, energy=0.5, seed=1):
         V = [self.czeros() for _ in range(3)]

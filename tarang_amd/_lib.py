@@ -66,6 +66,10 @@ SIGNATURES = {
     "tg_flux": (_I, [_P, _PP, _DP, _I, _DP]),
     "tg_shell_to_shell": (_I, [_P, _PP, _DP, _I, _DP]),
     "tg_init_field": (_I, [_P, _I, _D, _D, _D, _D, ctypes.c_uint64, _PP]),
+    "tg_compute_nlin_mhd": (_I, [_P, _PP, _PP, _PP, _PP]),
+    "tg_rk4_step_mhd": (_I, [_P, _PP, _PP, ctypes.POINTER(TgFluid), _D, _I]),
+    "tg_cross_helicity": (_I, [_P, _PP, _PP, _DP]),
+    "tg_flux_mhd": (_I, [_P, _PP, _PP, _DP, _I, _DP, _DP]),
     "tg_profile_enable": (_I, [_P, _I]),
     "tg_profile_reset": (_I, [_P]),
     "tg_profile_read": (_I, [_P, _I, _DP, ctypes.POINTER(ctypes.c_int64)]),

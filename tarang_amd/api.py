@@ -26,7 +26,8 @@ from . import _lib
 from ._lib import (DEALIAS_NONE, DEALIAS_REF_CORNER, DEALIAS_TWOTHIRDS, PREC_FP32, PREC_FP64, STEP_CONTINUE,
                    TgConfig, TgFluid, check, ptr_array)
 
-__all__ = ["FFF_Basis", "FluidVF", "Nlin_incompress", "Time_advance_incompress", "EnergyTr",
+__all__ = ["FFF_Basis", "FluidVF", "Nlin_incompress", "Time_advance_incompress", "Time_advance_incompress_MHD",
+           "Correlation", "EnergyTr",
            "DEALIAS_NONE", "DEALIAS_REF_CORNER", "DEALIAS_TWOTHIRDS", "PREC_FP64", "PREC_FP32"]
 
 
@@ -197,12 +198,26 @@ class FluidVF:
 
 class Nlin_incompress:
     @staticmethod
-    def Compute_nlin(U, W=None):
+    def Compute_nlin(U, W=None, mhd=False):
+        """Compute_nlin(U); Compute_nlin(U, Giver) (giver form); or, with
+        mhd=True, Compute_nlin(U, W) of MHD filling U.nlin and W.nlin."""
         b = U.basis
         if W is None:
             check("tg_compute_nlin", b.L.tg_compute_nlin(b.plan, U.ptrs(), U.ptrs("nlin")))
+        elif mhd:
+            check("tg_compute_nlin_mhd", b.L.tg_compute_nlin_mhd(b.plan, U.ptrs(), W.ptrs(), U.ptrs("nlin"),
+                                                                  W.ptrs("nlin")))
         else:
             check("tg_compute_nlin_giver", b.L.tg_compute_nlin_giver(b.plan, U.ptrs(), W.ptrs(), U.ptrs("nlin")))
+
+
+class Correlation:
+    @staticmethod
+    def Get_cross_helicity(U, W):
+        out = ctypes.c_double()
+        b = U.basis
+        check("tg_cross_helicity", b.L.tg_cross_helicity(b.plan, U.ptrs(), W.ptrs(), ctypes.byref(out)))
+        return out.value
 
 
 class Time_advance_incompress:
@@ -230,6 +245,19 @@ class Time_advance_incompress:
         self._continue = False
 
 
+class Time_advance_incompress_MHD:
+    """RK4 of velocity U (nu) and magnetic field W (eta), RK_VF_MHD.cc:126-190."""
+
+    def __init__(self, U, W, dt):
+        self.U, self.W = U, W
+        self.fluid = TgFluid(U.dissipation_coefficient, float(dt), 0, 0.0, 0.0, 0.0)
+
+    def Time_advance_step(self, nsteps=1):
+        b = self.U.basis
+        check("tg_rk4_step_mhd", b.L.tg_rk4_step_mhd(b.plan, self.U.ptrs(), self.W.ptrs(), ctypes.byref(self.fluid),
+                                                     self.W.dissipation_coefficient, int(nsteps)))
+
+
 class EnergyTr:
     def __init__(self, basis, flux_radii=None, shell_radii=None):
         self.basis = basis
@@ -245,6 +273,17 @@ class EnergyTr:
                                      out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
         self.flux_self = out
         return out
+
+    def Compute_flux_MHD(self, U, W):
+        """u< -> b> and b< -> b> fluxes (flux_VF_Uin_Wout, flux_VF_Win_Wout)."""
+        r = np.ascontiguousarray(self.flux_radii, dtype=np.float64)
+        n = len(r) - 1
+        u2b, b2b = np.zeros(n + 1), np.zeros(n + 1)
+        b = self.basis
+        dp = ctypes.POINTER(ctypes.c_double)
+        check("tg_flux_mhd", b.L.tg_flux_mhd(b.plan, U.ptrs(), W.ptrs(), r.ctypes.data_as(dp), n,
+                                             u2b.ctypes.data_as(dp), b2b.ctypes.data_as(dp)))
+        return u2b, b2b
 
     def Compute_shell_tr(self, U):
         r = np.ascontiguousarray(self.shell_radii, dtype=np.float64)

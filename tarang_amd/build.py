@@ -14,7 +14,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libtarang_amd.so")
-SOURCES = ["tg_kernels.hip", "tg_diag.hip", "tg_api.hip"]
+DRIVER = os.path.join(HERE, "tarang_ifluid")
+SOURCES = ["tg_kernels.hip", "tg_diag.hip", "tg_mhd.hip", "tg_api.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("TG_OFFLOAD_ARCH", "gfx950")
 FLAGS = [
@@ -44,8 +45,16 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    # C++ host driver over the C ABI (tarang_amd.hpp)
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"),
+           os.path.join(HERE, "host", "ifluid_main.cpp"), "-o", DRIVER + ".tmp", "-L", HERE, "-ltarang_amd",
+           "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"driver build failed:\n{r.stdout}\n{r.stderr}")
+    os.replace(DRIVER + ".tmp", DRIVER)
     if verbose:
-        print(f"built {LIB}")
+        print(f"built {LIB} and {DRIVER}")
     return LIB
 
 

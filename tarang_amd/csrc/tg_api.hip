@@ -13,6 +13,7 @@
 #include "../../include/tarang_gpu.h"
 #include "tg_diag.h"
 #include "tg_kernels.h"
+#include "tg_mhd.h"
 
 using namespace tg;
 
@@ -69,6 +70,9 @@ struct tg_plan {
     // host-side band config for the current step
     Band band{};
     int x_tile_columns = 0;  // fused x-pass tile width (0 = LDS-budget width)
+    // MHD buffers (FP64, boundary layout), allocated on first use:
+    // [0..2] accW, [3..5] N+, [6..8] N-, [9..11] Z+, [12..14] Z-
+    double *mhd[15] = {};
     // per-kernel HIP event timing (tg_profile_*)
     bool prof_on = false;
     struct ProfRec { int kid; hipEvent_t a, b; };
@@ -171,6 +175,8 @@ void free_plan(tg_plan *p)
     for (void *q : p->U)
         if (q) (void)hipFree(q);
     for (void *q : p->P)
+        if (q) (void)hipFree(q);
+    for (void *q : p->mhd)
         if (q) (void)hipFree(q);
     for (auto &r : p->prof_recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
     for (auto e : p->prof_pool) (void)hipEventDestroy(e);
@@ -293,20 +299,20 @@ int run_nlin(tg_plan *p, double *const V[3], double *const out[3], const tg_flui
     return TG_OK;
 }
 
+// N_i = FT[d_j(x_j w_i)] with x the advecting field X and w = Y (filter:
+// Y restricted to r_lo < K <= r_hi, Fill_array_shell, universal_ET.cc:880-892).
 template <typename T>
-int run_giver_nlin(tg_plan *p, double *const V[3], double *const out[3], double r_lo, double r_hi, bool filter,
-                   double *const W[3])
+int run_giver_nlin(tg_plan *p, double *const X[3], double *const Y[3], bool filter, double r_lo, double r_hi,
+                   double *const out[3])
 {
     Band nb{};
-    // u = inverse of the (dealiased) state, x and y passes
-    HIPCHK(launch_xinv_state<T>(p->g, nb, V, 3, p->U, nullptr, 0, p->twx, p->stream));
-    HIPCHK(launch_colfft<T>(p->g, 1, +1, p->U, 3, p->twy, p->stream));
-    // w = giver field
-    if (filter)
-        HIPCHK(launch_xinv_state<T>(p->g, nb, V, 3, p->U + 3, nullptr, 8, p->twx, p->stream, r_lo, r_hi));
-    else
-        HIPCHK(launch_xinv_state<T>(p->g, nb, W, 3, p->U + 3, nullptr, 0, p->twx, p->stream));
-    HIPCHK(launch_colfft<T>(p->g, 1, +1, p->U + 3, 3, p->twy, p->stream));
+    {
+        Prof pr(p, TG_KERNEL_X_INV);
+        HIPCHK(launch_xinv_state<T>(p->g, nb, X, 3, p->U, nullptr, 0, p->twx, p->stream));
+        HIPCHK(launch_xinv_state<T>(p->g, nb, Y, 3, p->U + 3, nullptr, filter ? 8 : 0, p->twx, p->stream, r_lo,
+                                    r_hi));
+    }
+    { Prof pr(p, TG_KERNEL_Y_FFT); HIPCHK(launch_colfft<T>(p->g, 1, +1, p->U, 6, p->twy, p->stream)); }
     { Prof pr(p, TG_KERNEL_Z_PRODUCTS); HIPCHK(launch_z_products<T>(p->g, p->U, p->P, 1, p->twm, p->twn, p->stream)); }
     { Prof pr(p, TG_KERNEL_Y_FFT); HIPCHK(launch_colfft<T>(p->g, 1, -1, p->P, 9, p->twy, p->stream)); }
     RkParams rk = rk_params(p, nullptr, -1);
@@ -347,6 +353,99 @@ int run_rk4(tg_plan *p, double *const V[3], const tg_fluid *f, int nsteps, int f
     }
     p->unext_valid = true;
     for (int c = 0; c < 3; c++) p->unext_V[c] = V[c];
+    return TG_OK;
+}
+
+int ensure_mhd(tg_plan *p)
+{
+    for (int i = 0; i < 15; i++)
+        if (!p->mhd[i]) {
+            int rc = dmalloc(p, (void **)&p->mhd[i], p->nc * 16);
+            if (rc) return rc;
+        }
+    return TG_OK;
+}
+
+// N+ (giver form) and N- (transposed) of the Elsasser fields whose x-inverse
+// sits in U[0..2] (Z-) and U[3..5] (Z+).
+template <typename T>
+int mhd_nonlinear(tg_plan *p, double *const *Np, double *const *Nm)
+{
+    Band nb{};
+    { Prof pr(p, TG_KERNEL_Y_FFT); HIPCHK(launch_colfft<T>(p->g, 1, +1, p->U, 6, p->twy, p->stream)); }
+    { Prof pr(p, TG_KERNEL_Z_PRODUCTS); HIPCHK(launch_z_products<T>(p->g, p->U, p->P, 1, p->twm, p->twn, p->stream)); }
+    { Prof pr(p, TG_KERNEL_Y_FFT); HIPCHK(launch_colfft<T>(p->g, 1, -1, p->P, 9, p->twy, p->stream)); }
+    Prof pr(p, TG_KERNEL_X_NLIN_GIVER);
+    RkParams rk = rk_params(p, nullptr, -1);
+    HIPCHK(launch_x_nlin<T>(p->g, nb, rk, p->P, nullptr, nullptr, nullptr, nullptr, nullptr, Np, p->twx, p->stream));
+    rk.stage = -2;
+    HIPCHK(launch_x_nlin<T>(p->g, nb, rk, p->P, nullptr, nullptr, nullptr, nullptr, nullptr, Nm, p->twx, p->stream));
+    return TG_OK;
+}
+
+template <typename T>
+int mhd_xinv(tg_plan *p, double *const *Zp, double *const *Zm)
+{
+    Band nb{};
+    Prof pr(p, TG_KERNEL_X_INV);
+    HIPCHK(launch_xinv_state<T>(p->g, nb, Zm, 3, p->U, nullptr, 0, p->twx, p->stream));
+    HIPCHK(launch_xinv_state<T>(p->g, nb, Zp, 3, p->U + 3, nullptr, 0, p->twx, p->stream));
+    return TG_OK;
+}
+
+template <typename T>
+int run_rk4_mhd(tg_plan *p, double *const V[3], double *const W[3], const tg_fluid *f, double eta, int nsteps)
+{
+    int rc = ensure_mhd(p);
+    if (rc) return rc;
+    double *accW[3] = {p->mhd[0], p->mhd[1], p->mhd[2]};
+    double *Np[3] = {p->mhd[3], p->mhd[4], p->mhd[5]};
+    double *Nm[3] = {p->mhd[6], p->mhd[7], p->mhd[8]};
+    double *Zp[3] = {p->mhd[9], p->mhd[10], p->mhd[11]};
+    double *Zm[3] = {p->mhd[12], p->mhd[13], p->mhd[14]};
+    p->unext_valid = false;
+    for (int step = 0; step < nsteps; step++) {
+        HIPCHK(launch_mhd_elsasser(p->g, V, W, Zp, Zm, p->stream));
+        rc = mhd_xinv<T>(p, Zp, Zm);
+        if (rc) return rc;
+        for (int s = 1; s <= 4; s++) {
+            rc = mhd_nonlinear<T>(p, Np, Nm);
+            if (rc) return rc;
+            {
+                Prof pr(p, TG_KERNEL_X_NLIN_RK);
+                HIPCHK(launch_mhd_rk(p->g, s, f->dt, f->nu, eta, V, W, p->acc, accW, Np, Nm, Zp, Zm, p->stream));
+            }
+            if (s < 4) {
+                rc = mhd_xinv<T>(p, Zp, Zm);
+                if (rc) return rc;
+            }
+        }
+        if (p->cfg.strong_reality) {
+            Prof pr(p, TG_KERNEL_REALITY);
+            HIPCHK(launch_reality(p->g, V, 3, p->stream));
+            HIPCHK(launch_reality(p->g, W, 3, p->stream));
+        }
+    }
+    return TG_OK;
+}
+
+template <typename T>
+int run_nlin_mhd(tg_plan *p, double *const V[3], double *const W[3], double *const NU[3], double *const NW[3])
+{
+    int rc = ensure_mhd(p);
+    if (rc) return rc;
+    double *Zp[3] = {p->mhd[9], p->mhd[10], p->mhd[11]};
+    double *Zm[3] = {p->mhd[12], p->mhd[13], p->mhd[14]};
+    p->unext_valid = false;
+    if (p->g.dealias) {
+        HIPCHK(launch_dealias(p->g, V, 3, p->stream));
+        HIPCHK(launch_dealias(p->g, W, 3, p->stream));
+    }
+    HIPCHK(launch_mhd_elsasser(p->g, V, W, Zp, Zm, p->stream));
+    rc = mhd_xinv<T>(p, Zp, Zm);
+    if (!rc) rc = mhd_nonlinear<T>(p, NU, NW);
+    if (rc) return rc;
+    HIPCHK(launch_mhd_combine(p->g, NU, NW, p->stream));
     return TG_OK;
 }
 
@@ -563,7 +662,7 @@ int tg_compute_nlin_giver(tg_plan *plan, double *const V_dev[3], double *const W
         HIPCHK(launch_dealias(plan->g, V_dev, 3, plan->stream));
         HIPCHK(launch_dealias(plan->g, W_dev, 3, plan->stream));
     }
-    return DISPATCH_PREC(plan, run_giver_nlin)(plan, V_dev, nlin_dev, 0.0, 0.0, false, W_dev);
+    return DISPATCH_PREC(plan, run_giver_nlin)(plan, V_dev, W_dev, false, 0.0, 0.0, nlin_dev);
 }
 
 int tg_rk4_step(tg_plan *plan, double *const V_dev[3], const tg_fluid *fluid, int nsteps, int flags)
@@ -625,7 +724,7 @@ int tg_flux(tg_plan *plan, double *const V_dev[3], const double *radii, int nsph
     if (plan->g.dealias) HIPCHK(launch_dealias(plan->g, V_dev, 3, plan->stream));
     flux[0] = 0.0;
     for (int s = 1; s <= nspheres; s++) {
-        rc = DISPATCH_PREC(plan, run_giver_nlin)(plan, V_dev, plan->acc, 0.0, radii[s], true, nullptr);
+        rc = DISPATCH_PREC(plan, run_giver_nlin)(plan, V_dev, V_dev, true, 0.0, radii[s], plan->acc);
         if (rc) return rc;
         double v = 0.0;
         const double *A[3] = {plan->acc[0], plan->acc[1], plan->acc[2]};
@@ -650,13 +749,94 @@ int tg_shell_to_shell(tg_plan *plan, double *const V_dev[3], const double *radii
     std::vector<double> row(nr);
     const int kin = max_radius_inside(plan->g);
     for (int m = 1; m < nshells; m++) {
-        rc = DISPATCH_PREC(plan, run_giver_nlin)(plan, V_dev, plan->acc, radii[m - 1], radii[m], true, nullptr);
+        rc = DISPATCH_PREC(plan, run_giver_nlin)(plan, V_dev, V_dev, true, radii[m - 1], radii[m], plan->acc);
         if (rc) return rc;
         const double *A[3] = {plan->acc[0], plan->acc[1], plan->acc[2]};
         const double *B[3] = {V_dev[0], V_dev[1], V_dev[2]};
         rc = reduce_to_host(plan, RED_SHELL_ALL, A, B, 3, 0, radii, nr, 1, kin, 0, 0, row.data());
         if (rc) return rc;
         for (int n = 0; n < nr; n++) S[(size_t)m * nr + n] = -row[n];
+    }
+    return TG_OK;
+}
+
+int tg_compute_nlin_mhd(tg_plan *plan, double *const V_dev[3], double *const W_dev[3], double *const nlinV_dev[3],
+                        double *const nlinW_dev[3])
+{
+    if (!plan) return fail(TG_ERR_ARG, "null plan");
+    int rc = check_ptrs((const void *const *)V_dev, 3, "V");
+    if (!rc) rc = check_ptrs((const void *const *)W_dev, 3, "W");
+    if (!rc) rc = check_ptrs((const void *const *)nlinV_dev, 3, "nlinV");
+    if (!rc) rc = check_ptrs((const void *const *)nlinW_dev, 3, "nlinW");
+    if (!rc) rc = set_device(plan);
+    if (rc) return rc;
+    return DISPATCH_PREC(plan, run_nlin_mhd)(plan, V_dev, W_dev, nlinV_dev, nlinW_dev);
+}
+
+int tg_rk4_step_mhd(tg_plan *plan, double *const V_dev[3], double *const W_dev[3], const tg_fluid *fluid, double eta,
+                    int nsteps)
+{
+    if (!plan || !fluid) return fail(TG_ERR_ARG, "null argument");
+    if (nsteps < 0) return fail(TG_ERR_ARG, "nsteps must be >= 0");
+    if (!(fluid->dt > 0) || !(fluid->nu >= 0) || !(eta >= 0)) return fail(TG_ERR_ARG, "need dt > 0, nu, eta >= 0");
+    if (fluid->force_kind != 0) return fail(TG_ERR_ARG, "forcing is not supported for MHD");
+    int rc = check_ptrs((const void *const *)V_dev, 3, "V");
+    if (!rc) rc = check_ptrs((const void *const *)W_dev, 3, "W");
+    if (!rc) rc = set_device(plan);
+    if (rc) return rc;
+    return DISPATCH_PREC(plan, run_rk4_mhd)(plan, V_dev, W_dev, fluid, eta, nsteps);
+}
+
+int tg_cross_helicity(tg_plan *plan, double *const V_dev[3], double *const W_dev[3], double *hc)
+{
+    if (!plan || !hc) return fail(TG_ERR_ARG, "null argument");
+    int rc = check_ptrs((const void *const *)V_dev, 3, "V");
+    if (!rc) rc = check_ptrs((const void *const *)W_dev, 3, "W");
+    if (!rc) rc = set_device(plan);
+    if (rc) return rc;
+    double t = 0.0;
+    for (int c = 0; c < 3; c++) {
+        double v = 0.0;
+        const double *A[1] = {V_dev[c]};
+        const double *B[1] = {W_dev[c]};
+        rc = reduce_to_host(plan, RED_CROSS, A, B, 1, 0, nullptr, 1, 1, 0, 0, 0, &v);
+        if (rc) return rc;
+        t += v;
+    }
+    *hc = t;
+    return TG_OK;
+}
+
+int tg_flux_mhd(tg_plan *plan, double *const V_dev[3], double *const W_dev[3], const double *radii, int nspheres,
+                double *u2b, double *b2b)
+{
+    if (!plan || !radii || !u2b || !b2b || nspheres < 1) return fail(TG_ERR_ARG, "bad argument");
+    int rc = check_ptrs((const void *const *)V_dev, 3, "V");
+    if (!rc) rc = check_ptrs((const void *const *)W_dev, 3, "W");
+    if (!rc) rc = set_device(plan);
+    if (rc) return rc;
+    if (plan->g.dealias) {
+        HIPCHK(launch_dealias(plan->g, V_dev, 3, plan->stream));
+        HIPCHK(launch_dealias(plan->g, W_dev, 3, plan->stream));
+    }
+    const double *A[3] = {plan->acc[0], plan->acc[1], plan->acc[2]};
+    const double *B[3] = {W_dev[0], W_dev[1], W_dev[2]};
+    u2b[0] = b2b[0] = 0.0;
+    for (int s = 1; s <= nspheres; s++) {
+        // W.nlin = FT[d_j(b_j u<_i)]; u< -> b> = sum_{K > r} 2m Re(N.b*)
+        rc = DISPATCH_PREC(plan, run_giver_nlin)(plan, W_dev, V_dev, true, 0.0, radii[s], plan->acc);
+        if (rc) return rc;
+        rc = reduce_to_host(plan, RED_SHELL_SINGLE, A, B, 3, 0, nullptr, 1, 1, 0, radii[s], INFINITY, &u2b[s]);
+        if (rc) return rc;
+    }
+    for (int s = 1; s <= nspheres; s++) {
+        // U.nlin = FT[d_j(u_j b<_i)]; b< -> b> = -sum_{K > r} 2m Re(N.b*)
+        rc = DISPATCH_PREC(plan, run_giver_nlin)(plan, V_dev, W_dev, true, 0.0, radii[s], plan->acc);
+        if (rc) return rc;
+        double v = 0.0;
+        rc = reduce_to_host(plan, RED_SHELL_SINGLE, A, B, 3, 0, nullptr, 1, 1, 0, radii[s], INFINITY, &v);
+        if (rc) return rc;
+        b2b[s] = -v;
     }
     return TG_OK;
 }

@@ -13,6 +13,7 @@ enum {
     RED_SHELL_ALL = 3,       // bins first radii >= K, K <= kcut: 2m Re(A.B*)
     RED_SHELL_ALL_NOCUT = 4, // same without the K cut
     RED_SHELL_SINGLE = 5,    // sum over r_lo < K <= r_hi of 2m Re(A.B*)
+    RED_CROSS = 6,           // sum w Re(A B*), w = 1/2 on kz = 0
 };
 
 hipError_t launch_reduce(const Grid &g, int mode, const double *const *A, const double *const *B, int ncomp, int n,

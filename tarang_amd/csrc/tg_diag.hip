@@ -88,6 +88,10 @@ __global__ void __launch_bounds__(kRedThreads) k_reduce(RedArgs a)
             const double re = A[2 * i], im = A[2 * i + 1];
             const double w = lz == 0 ? 0.5 : 1.0;
             neumaier(s0, c0, w * (re * re + im * im));
+        } else if (MODE == RED_CROSS) {
+            const double *A = a.A[0], *B = a.B[0];
+            const double w = lz == 0 ? 0.5 : 1.0;
+            neumaier(s0, c0, w * (A[2 * i] * B[2 * i] + A[2 * i + 1] * B[2 * i + 1]));
         } else if (MODE == RED_SN) {
             if (i == 0 && a.n == 0) continue;
             const double K = kmag_l(g, lx, ly, lz);
@@ -133,7 +137,7 @@ __global__ void __launch_bounds__(kRedThreads) k_reduce(RedArgs a)
             }
         }
     }
-    if (MODE == RED_ENERGY || MODE == RED_SN || MODE == RED_SHELL_SINGLE) {
+    if (MODE == RED_ENERGY || MODE == RED_SN || MODE == RED_SHELL_SINGLE || MODE == RED_CROSS) {
         // block tree reduction of (s0 + c0)
         double v = s0 + c0;
         for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
@@ -184,7 +188,7 @@ hipError_t launch_reduce(const Grid &g, int mode, const double *const *A, const 
     a.kcut = kcut;
     a.r_lo = r_lo;
     a.r_hi = r_hi;
-    const bool single = mode == RED_ENERGY || mode == RED_SN || mode == RED_SHELL_SINGLE;
+    const bool single = mode == RED_ENERGY || mode == RED_SN || mode == RED_SHELL_SINGLE || mode == RED_CROSS;
     const int width = single ? 1 : nq * nbins;
     if (!single && nbins > kRedMaxBins) return hipErrorInvalidValue;
     const size_t shm = single ? 64 : (size_t)(kRedThreads / 64) * nq * nbins * sizeof(double);
@@ -199,6 +203,7 @@ hipError_t launch_reduce(const Grid &g, int mode, const double *const *A, const 
         case RED_SHELL_SINGLE:
             hipLaunchKernelGGL(k_reduce<RED_SHELL_SINGLE>, dim3(nblocks), dim3(kRedThreads), shm, st, a);
             break;
+        case RED_CROSS: hipLaunchKernelGGL(k_reduce<RED_CROSS>, dim3(nblocks), dim3(kRedThreads), shm, st, a); break;
         default: return hipErrorInvalidValue;
     }
     hipError_t e = hipGetLastError();

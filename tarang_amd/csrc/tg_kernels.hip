@@ -528,8 +528,8 @@ __constant__ int kNsOrder[6] = {0, 1, 2, 3, 4, 5};
 __constant__ int kNsSel[6][3] = {{1, 0, 0}, {0, 2, 0}, {0, 0, 3}, {2, 1, 0}, {0, 3, 2}, {3, 0, 1}};
 __constant__ int kGiverOrder[9] = {0, 4, 8, 3, 1, 7, 5, 2, 6};
 
-template <typename T, int L, int CB, bool GIVER>
-__global__ void __launch_bounds__(CB *(L / 8), (CB * (L / 8) >= 512 ? 2 : 2)) k_x_nlin(XnlArgs a)
+template <typename T, int L, int CB, int MODE>
+__global__ void __launch_bounds__(CB *(L / 8), 2) k_x_nlin(XnlArgs a)
 {
     constexpr int TPL = L / 8;
     constexpr int LP = lpad_len(L);
@@ -562,6 +562,7 @@ __global__ void __launch_bounds__(CB *(L / 8), (CB * (L / 8) >= 512 ? 2 : 2)) k_
     // Product p adds i K_d FT[p] to N_i for the (i, d) pairs of the
     // divergence form; sel_i = 0 (none), 1 (Kx), 2 (Ky), 3 (Kz).  Summing
     // from zero in this order reproduces the reference's accumulation order.
+    constexpr bool GIVER = MODE != 0;
     cpx<T> N[3][8];
 #pragma unroll
     for (int s = 0; s < 8; s++) N[0][s] = N[1][s] = N[2][s] = mk<T>(0, 0);
@@ -570,11 +571,16 @@ __global__ void __launch_bounds__(CB *(L / 8), (CB * (L / 8) >= 512 ? 2 : 2)) k_
     for (int pi = 0; pi < NPROD; pi++) {
         const int p = GIVER ? kGiverOrder[pi] : kNsOrder[pi];
         int sel0, sel1, sel2;
-        if (GIVER) {
+        if (MODE == 1) {
             // P[3i+j] = FT[u_j w_i] contributes i K_j to N_i
             sel0 = p / 3 == 0 ? p % 3 + 1 : 0;
             sel1 = p / 3 == 1 ? p % 3 + 1 : 0;
             sel2 = p / 3 == 2 ? p % 3 + 1 : 0;
+        } else if (MODE == 2) {
+            // transposed: P[3i+j] contributes i K_i to N_j (MHD: N-_j = sum_i d_i(Z+_i Z-_j))
+            sel0 = p % 3 == 0 ? p / 3 + 1 : 0;
+            sel1 = p % 3 == 1 ? p / 3 + 1 : 0;
+            sel2 = p % 3 == 2 ? p / 3 + 1 : 0;
         } else {
             sel0 = kNsSel[p][0];
             sel1 = kNsSel[p][1];
@@ -940,13 +946,17 @@ hipError_t launch_z_products(const Grid &g, void *const *U, void *const *P, int 
     return hipGetLastError();
 }
 
+#define TG_XNL(CBV, M) \
+    hipLaunchKernelGGL((k_x_nlin<T, LL, CBV, M>), dim3(x_grid(x_tiles(g, CBV, false))), dim3(CBV * (LL / 8)), 0, st, a)
+
 template <typename T>
 hipError_t launch_x_nlin(const Grid &g, const Band &b, const RkParams &rk, void *const *P, double *const *V,
                          double *const *acc, void *const *U, const double *slots_in, double *slots_out,
                          double *const *out, const void *twx, hipStream_t st, int x_tile_columns)
 {
     XnlArgs a{};
-    const bool giver = rk.stage < 0;
+    // rk.stage -1: giver form, -2: transposed giver form (nonlinear term only)
+    const int giver = rk.stage == -1 ? 1 : (rk.stage == -2 ? 2 : 0);
     for (int i = 0; i < (giver ? 9 : 6); i++) a.P[i] = P[i];
     for (int i = 0; i < 3; i++) {
         a.V[i] = V ? V[i] : nullptr;
@@ -963,17 +973,24 @@ hipError_t launch_x_nlin(const Grid &g, const Band &b, const RkParams &rk, void 
     TG_DISPATCH_L(g.nx, {
         // two FP64 tile buffers per block: the LDS-budget width (two blocks
         // per CU), or twice that width (one block per CU)
+        // two FP64 tile buffers per block: the LDS-budget width (two blocks
+        // per CU), or twice that width (one block per CU)
         constexpr int CB = col_block<T>(LL, 2 * sizeof(cpx<double>));
         constexpr int CBW = 2 * CB * (LL / 8) <= 1024 ? 2 * CB : CB;
         const bool wide = x_tile_columns == CBW && CBW != CB;
-        if (giver) {
+        const int mode = giver;
+        if (mode) {
             a.rk.stage = 0;
             a.rk.write_rhs = 0;
-            if (wide) hipLaunchKernelGGL((k_x_nlin<T, LL, CBW, true>), dim3(x_grid(x_tiles(g, CBW, false))), dim3(CBW * (LL / 8)), 0, st, a);
-            else hipLaunchKernelGGL((k_x_nlin<T, LL, CB, true>), dim3(x_grid(x_tiles(g, CB, false))), dim3(CB * (LL / 8)), 0, st, a);
+        }
+        if (wide) {
+            if (mode == 1) TG_XNL(CBW, 1);
+            else if (mode == 2) TG_XNL(CBW, 2);
+            else TG_XNL(CBW, 0);
         } else {
-            if (wide) hipLaunchKernelGGL((k_x_nlin<T, LL, CBW, false>), dim3(x_grid(x_tiles(g, CBW, false))), dim3(CBW * (LL / 8)), 0, st, a);
-            else hipLaunchKernelGGL((k_x_nlin<T, LL, CB, false>), dim3(x_grid(x_tiles(g, CB, false))), dim3(CB * (LL / 8)), 0, st, a);
+            if (mode == 1) TG_XNL(CB, 1);
+            else if (mode == 2) TG_XNL(CB, 2);
+            else TG_XNL(CB, 0);
         }
     });
     return hipGetLastError();

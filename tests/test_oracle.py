@@ -234,7 +234,7 @@ def test_rhs_is_solenoidal_and_energy_conserving():
     assert np.abs(div).max() < 1e-14 * max(np.abs(n).max() for n in N)
     # sum_k 2m Re(RHS . V*) = 0 (Galerkin truncation with the 2/3 rule)
     tr = sum(o.shell_mult_single(N[c], V[c], -1.0, np.inf) for c in range(3))
-    scale = sum(o.shell_mult_single(np.abs(N[c]), np.abs(V[c]), -1.0, np.inf) for c in range(3))
+    scale = sum(o.shell_mult_single(np.abs(N[c]).astype(np.complex128), np.abs(V[c]).astype(np.complex128), -1.0, np.inf) for c in range(3))
     assert abs(tr) < 1e-14 * scale
 
 
@@ -377,3 +377,39 @@ def test_init_field_properties():
             w = v.copy()
             o.strong_reality(w)
             assert np.abs(w - v).max() < 1e-17
+
+
+# ---------------------------------------------------------------- MHD
+
+def test_mhd_elsasser_matches_direct_form():
+    """N_u = (u.grad)u - (b.grad)b and N_b = (u.grad)b - (b.grad)u, built from
+    the giver form, agree with the Elsasser evaluation."""
+    o = to.Oracle(16, dealias=to.DEALIAS_REF_CORNER)
+    V = o.init_field(kind=1, kmin=1.0, kmax=6.0, energy=0.5, seed=11)
+    W = o.init_field(kind=1, kmin=1.0, kmax=6.0, energy=0.25, seed=12)
+    NU, NW = o.compute_nlin_mhd([v.copy() for v in V], [w.copy() for w in W])
+    uu = o.compute_nlin_giver([v.copy() for v in V], [v.copy() for v in V])
+    bb = o.compute_nlin_giver([w.copy() for w in W], [w.copy() for w in W])
+    ub = o.compute_nlin_giver([v.copy() for v in V], [w.copy() for w in W])
+    bu = o.compute_nlin_giver([w.copy() for w in W], [v.copy() for v in V])
+    for c in range(3):
+        scale = np.abs(uu[c]).max()
+        assert np.abs(NU[c] - (uu[c] - bb[c])).max() < 1e-14 * scale * 10
+        assert np.abs(NW[c] - (ub[c] - bu[c])).max() < 1e-14 * scale * 10
+
+
+def test_mhd_invariants_conserved_by_nonlinear_term():
+    o = to.Oracle(32, dealias=to.DEALIAS_TWOTHIRDS)
+    V = o.init_field(kind=1, kmin=1.0, kmax=10.0, energy=0.5, seed=13)
+    W = o.init_field(kind=1, kmin=1.0, kmax=10.0, energy=0.25, seed=14)
+    for c in range(3):
+        o.dealias(V[c]); o.dealias(W[c])
+    NU, NW = o.compute_nlin_mhd([v.copy() for v in V], [w.copy() for w in W])
+    o.compute_rhs(NU, None)
+    NW = [-n for n in NW]
+    dE = sum(o.shell_mult_single(NU[c], V[c], -1.0, np.inf) + o.shell_mult_single(NW[c], W[c], -1.0, np.inf)
+             for c in range(3))
+    dHc = sum(o.shell_mult_single(NU[c], W[c], -1.0, np.inf) + o.shell_mult_single(NW[c], V[c], -1.0, np.inf)
+              for c in range(3))
+    scale = sum(o.shell_mult_single(np.abs(NU[c]).astype(np.complex128), np.abs(V[c]).astype(np.complex128), -1.0, np.inf) for c in range(3))
+    assert abs(dE) < 1e-13 * scale and abs(dHc) < 1e-13 * scale
