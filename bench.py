@@ -20,7 +20,12 @@ Besides the headline FP64 number the line reports:
   step_roofline  the whole step's algorithmic bytes over the step time;
   cpu_baseline   the FP64 CPU oracle (restatement of the reference
              algorithm, own FFT, OpenMP) on one RK4 step of the same
-             512^3 workload, on the host's cores.
+             512^3 workload, on the host's cores;
+  adjoint    the adjoint/forward cost ratio on its own workload: 256^3 FP64,
+             200 unforced RK4 steps (no reality condition), checkpoints
+             every 20 steps, loss = energy in the shell 8 <= |k| < 16 at the
+             last step; ratio = (forward sweep with tape + backward sweep
+             with recomputation) / (200 plain forward steps), measured time.
 """
 import argparse
 import json
@@ -42,6 +47,7 @@ def parse():
     ap.add_argument("--n", type=int, default=512, help="grid size per axis")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-extra", action="store_true", help="headline FP64 number only")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint cost measurement")
     ap.add_argument("--copy-calib", action="store_true", help="run a plain 1 GiB device copy (counter calibration)")
     return ap.parse_args()
 
@@ -195,6 +201,10 @@ def main():
                          "flux_max": float(np.max(flux))}
         print(f"[bench] diagnostics {tdiag * 1e3:.1f} ms", file=sys.stderr, flush=True)
         extra["energy"] = E64
+        if not args.no_adjoint:
+            del et
+            torch.cuda.empty_cache()
+            extra["adjoint"] = adjoint_cost(api, torch, np)
 
     cpu = None
     if not args.no_cpu:
@@ -223,6 +233,51 @@ def main():
     }
     out.update(extra)
     print(json.dumps(out))
+
+
+def adjoint_cost(api, torch, np, n=256, nsteps=200, every=20, nu=1e-3, dt=1e-3, r_lo=8.0, r_hi=16.0):
+    N = (n, n, n)
+    b = api.FFF_Basis(N, precision=api.PREC_FP64, dealias=api.DEALIAS_REF_CORNER, strong_reality=False)
+    U = api.FluidVF(b, nu)
+    U.Init_cond_spectrum(kind=1, kmin=2.0, kmax=n / 3.0, energy=0.5, seed=7)
+    V0 = [v.clone() for v in U.cvf]
+    ta = api.Time_advance_incompress(U, dt)
+    ta.Time_advance_step(2, chain=False)
+    U.set_field(V0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ta.Time_advance_step(nsteps, chain=False)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    kx = torch.fft.fftfreq(n, 1.0 / n, dtype=torch.float64, device="cuda")
+    kz = torch.arange(n // 2 + 1, dtype=torch.float64, device="cuda")
+    K = torch.sqrt(kx[:, None, None] ** 2 + kx[None, :, None] ** 2 + kz[None, None, :] ** 2)
+    m = ((K >= r_lo) & (K < r_hi)).to(torch.float64)
+    m[:, :, 0] *= 0.5
+    adj = api.Adjoint_incompress(U, dt, checkpoint_every=every)
+    # allocation of checkpoints and tape outside the timed region
+    U.set_field(V0)
+    adj.Forward(every)
+    U.set_field(V0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    adj.Forward(nsteps)
+    lam = [2.0 * m * v for v in U.cvf]
+    adj.Backward(lam)
+    torch.cuda.synchronize()
+    t_adj = time.perf_counter() - t0
+    loss = float(sum((m * (v.real ** 2 + v.imag ** 2)).sum().item() for v in U.cvf))
+    gnorm = float(sum((x.real ** 2 + x.imag ** 2).sum().item() for x in lam)) ** 0.5
+    nc = n * n * (n // 2 + 1)
+    tape_bytes = (nsteps + every - 1) // every * 3 * nc * 16 + every * 12 * n * n * (n // 2) * 16
+    adj.Free()
+    print(f"[bench] adjoint {n}^3 x {nsteps}: forward {t_fwd:.3f} s, forward+backward {t_adj:.3f} s",
+          file=sys.stderr, flush=True)
+    return {"workload": f"{n}^3 FP64, {nsteps} unforced RK4 steps, checkpoints every {every}, "
+                        f"loss = E in {r_lo:g} <= |k| < {r_hi:g}",
+            "forward_s": round(t_fwd, 4), "forward_and_backward_s": round(t_adj, 4),
+            "cost_ratio": round(t_adj / t_fwd, 3), "target_ratio": 3.0, "loss": loss, "grad_norm": gnorm,
+            "checkpoint_and_tape_GB": round(tape_bytes / 1e9, 2)}
 
 
 def cpu_baseline(N, nu, dt, force, V_dev):

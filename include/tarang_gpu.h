@@ -159,13 +159,39 @@ enum {
     TG_KERNEL_Z_C2R = 7,
     TG_KERNEL_X_FWD = 8,
     TG_KERNEL_REDUCE = 9,
-    TG_KERNEL_COUNT = 10
+    TG_KERNEL_X_INV_SYM = 10,   /* adjoint: x-inverse of the symmetrised cotangent */
+    TG_KERNEL_Z_ADJ = 11,       /* adjoint: z c2r + products with the velocity tape + r2c */
+    TG_KERNEL_X_FWD_ADJ = 12,   /* adjoint: x forward + RK4 stage transpose */
+    TG_KERNEL_COUNT = 13
 };
 int tg_profile_enable(tg_plan *plan, int on);
 int tg_profile_reset(tg_plan *plan);
 /* Synchronises the stream; total milliseconds and launch count of a class. */
 int tg_profile_read(tg_plan *plan, int kernel, double *total_ms, int64_t *launches);
 const char *tg_kernel_name(int kernel);
+
+/* Adjoint (no counterpart in the reference, which has no adjoint).  The
+ * inner product is the real dot product of the stored half-spectrum
+ * coefficients (Re, Im), so a gradient dL/dV has components dL/dRe + i dL/dIm.
+ * Supported: no forcing (fluid->force_kind 0) and strong_reality 0 (the
+ * forward map is then the plain Lawson RK4 of tg_rk4_step).
+ *
+ * tg_rhs_vjp: out = f'(V)^T lam for f(V) = -P NL(D V), the unforced
+ * right-hand side of tg_compute_rhs.  V and lam are not modified. */
+int tg_rhs_vjp(tg_plan *plan, double *const V_dev[3], double *const lam_dev[3], double *const out_dev[3]);
+/* Forward sweep with a tape: nsteps RK4 steps of V in place (as tg_rk4_step
+ * with flags 0), keeping a checkpoint of V every checkpoint_every steps and
+ * the real-space stage velocities of the last segment.  Device memory held
+ * until tg_adjoint_free or plan destruction: ceil(nsteps/every) checkpoints
+ * of 3 FP64 spectral components plus 12*every real fields in the transform
+ * precision. */
+int tg_adjoint_forward(tg_plan *plan, double *const V_dev[3], const tg_fluid *fluid, int nsteps,
+                       int checkpoint_every);
+/* Backward sweep over the steps of the last tg_adjoint_forward: lam holds
+ * dL/dV_n on entry and dL/dV_0 on return.  Earlier segments are recomputed
+ * from their checkpoints. */
+int tg_adjoint_backward(tg_plan *plan, double *const lam_dev[3]);
+int tg_adjoint_free(tg_plan *plan);
 
 /* Seeded solenoidal initial field (this library's own generator; the
  * reference's Init_cond_energy_helicity_spectrum, init_cond_energy.cc:67-176,

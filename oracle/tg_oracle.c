@@ -854,6 +854,291 @@ void or_flux_mhd(const or_grid *g, double *V[3], double *W[3], const double *rad
 }
 
 /* ------------------------------------------------------------------------ */
+/* Tangent and adjoint of the NS right-hand side and of the Lawson RK4 step  */
+/* (no counterpart in the reference; this build's FP64 CPU adjoint)          */
+/* ------------------------------------------------------------------------ */
+/*
+ * f(V) = -P NL(D V), NL(U)_i = sum_j i K_j F[I(U_j) I(U_i)] (or_compute_nlin +
+ * or_compute_rhs without forcing).  Inner product: the real dot product of
+ * the stored half-spectrum coefficients (Re, Im).  With c = 1 on the kz = 0
+ * and kz = Nz/2 planes and 2 elsewhere, the transposes of the transforms are
+ * I^T = N^3 diag(c) F and F^T = N^-3 I diag(1/c); (i K)^T = -i K; P and D are
+ * symmetric.  Hence
+ *   J^T L = -D diag(c) F[ sum_j u_j I(diag(1/c)(-i K_m M_j - i K_j M_m)) ]_m,
+ *   M = P L, u = I(D V).
+ */
+
+static double cweight(const or_grid *g, int lz) { return (lz == 0 || lz == g->nz / 2) ? 1.0 : 2.0; }
+
+static void project(const or_grid *g, cplx *a0, cplx *a1, cplx *a2)
+{
+    const int nx = g->nx, ny = g->ny, nzc = g->nz / 2 + 1;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int lx = 0; lx < nx; lx++)
+        for (int ly = 0; ly < ny; ly++)
+            for (int lz = 0; lz < nzc; lz++) {
+                size_t i = cidx(g, lx, ly, lz);
+                double kx = get_kx(g, lx) * g->kfactor[0], ky = get_ky(g, ly) * g->kfactor[1], kz = lz * g->kfactor[2];
+                double ks = ksqr(g, lx, ly, lz);
+                if (ks == 0.0) continue;
+                cplx d = kx * a0[i] + ky * a1[i] + kz * a2[i];
+                a0[i] -= kx * d / ks;
+                a1[i] -= ky * d / ks;
+                a2[i] -= kz * d / ks;
+            }
+}
+
+/* out = J(V) dV (tangent of the right-hand side) */
+void or_rhs_jvp(const or_grid *g, double *const V[3], double *const dV[3], double *out[3])
+{
+    const size_t nr = nreal(g), nc = ncplx(g);
+    double *u[3], *du[3], *p = (double *)malloc(sizeof(double) * nr);
+    double *tmp = (double *)malloc(sizeof(cplx) * nc);
+    for (int c = 0; c < 3; c++) {
+        u[c] = (double *)malloc(sizeof(double) * nr);
+        du[c] = (double *)malloc(sizeof(double) * nr);
+        memcpy(tmp, V[c], sizeof(cplx) * nc); or_dealias(g, tmp); or_inverse(g, tmp, u[c]);
+        memcpy(tmp, dV[c], sizeof(cplx) * nc); or_dealias(g, tmp); or_inverse(g, tmp, du[c]);
+        memset(out[c], 0, sizeof(cplx) * nc);
+    }
+    cplx *X = (cplx *)malloc(sizeof(cplx) * nc);
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+#pragma omp parallel for schedule(static)
+            for (size_t q = 0; q < nr; q++) p[q] = du[j][q] * u[i][q] + u[j][q] * du[i][q];
+            or_forward(g, p, (double *)X);
+            deriv(g, j, X, (cplx *)out[i], 1);
+        }
+    project(g, (cplx *)out[0], (cplx *)out[1], (cplx *)out[2]);
+    for (int c = 0; c < 3; c++) {
+        cplx *o = (cplx *)out[c];
+        for (size_t q = 0; q < nc; q++) o[q] = -o[q];
+    }
+    free(X); free(p); free(tmp);
+    for (int c = 0; c < 3; c++) { free(u[c]); free(du[c]); }
+}
+
+/* out = J(V)^T L (adjoint of the right-hand side) */
+void or_rhs_vjp(const or_grid *g, double *const V[3], double *const Lam[3], double *out[3])
+{
+    const int nx = g->nx, ny = g->ny, nzc = g->nz / 2 + 1;
+    const size_t nr = nreal(g), nc = ncplx(g);
+    cplx *M[3];
+    double *u[3], *sig[6], *q = (double *)malloc(sizeof(double) * nr);
+    double *tmp = (double *)malloc(sizeof(cplx) * nc);
+    for (int c = 0; c < 3; c++) {
+        M[c] = (cplx *)malloc(sizeof(cplx) * nc);
+        memcpy(M[c], Lam[c], sizeof(cplx) * nc);
+        u[c] = (double *)malloc(sizeof(double) * nr);
+        memcpy(tmp, V[c], sizeof(cplx) * nc); or_dealias(g, tmp); or_inverse(g, tmp, u[c]);
+    }
+    project(g, M[0], M[1], M[2]);
+    /* sigma_jm = I(diag(1/c)(-i K_m M_j - i K_j M_m)), j <= m */
+    int k = 0;
+    int jj[6], mm[6];
+    for (int j = 0; j < 3; j++)
+        for (int m = j; m < 3; m++) {
+            jj[k] = j; mm[k] = m;
+            cplx *S = (cplx *)tmp;
+#pragma omp parallel for collapse(2) schedule(static)
+            for (int lx = 0; lx < nx; lx++)
+                for (int ly = 0; ly < ny; ly++)
+                    for (int lz = 0; lz < nzc; lz++) {
+                        size_t i = cidx(g, lx, ly, lz);
+                        double K[3] = {get_kx(g, lx) * g->kfactor[0], get_ky(g, ly) * g->kfactor[1], lz * g->kfactor[2]};
+                        S[i] = (-I * K[m] * M[j][i] - I * K[j] * M[m][i]) / cweight(g, lz);
+                    }
+            sig[k] = (double *)malloc(sizeof(double) * nr);
+            or_inverse(g, tmp, sig[k]);
+            k++;
+        }
+    for (int m = 0; m < 3; m++) {
+#pragma omp parallel for schedule(static)
+        for (size_t r = 0; r < nr; r++) {
+            double acc = 0.0;
+            for (int t = 0; t < 6; t++) {
+                if (mm[t] == m) acc += u[jj[t]][r] * sig[t][r];
+                else if (jj[t] == m) acc += u[mm[t]][r] * sig[t][r];
+            }
+            q[r] = acc;
+        }
+        or_forward(g, q, out[m]);
+        cplx *o = (cplx *)out[m];
+#pragma omp parallel for collapse(2) schedule(static)
+        for (int lx = 0; lx < nx; lx++)
+            for (int ly = 0; ly < ny; ly++)
+                for (int lz = 0; lz < nzc; lz++) {
+                    size_t i = cidx(g, lx, ly, lz);
+                    o[i] = dealias_zero(g, lx, ly, lz) ? 0 : -cweight(g, lz) * o[i];
+                }
+    }
+    free(q); free(tmp);
+    for (int c = 0; c < 3; c++) { free(M[c]); free(u[c]); }
+    for (int t = 0; t < 6; t++) free(sig[t]);
+}
+
+static void scale_add(const or_grid *g, cplx *y, const cplx *x, double a, const double *fac)
+{
+    /* y += a * fac(K) * x, fac = NULL for 1 */
+    const int nx = g->nx, ny = g->ny, nzc = g->nz / 2 + 1;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int lx = 0; lx < nx; lx++)
+        for (int ly = 0; ly < ny; ly++)
+            for (int lz = 0; lz < nzc; lz++) {
+                size_t i = cidx(g, lx, ly, lz);
+                double f = fac ? exp(fac[0] * ksqr(g, lx, ly, lz)) : 1.0;
+                y[i] += a * f * x[i];
+            }
+}
+
+/* Adjoint of one Lawson RK4 step without forcing and without the reality
+ * condition: lam <- (dV_{n+1}/dV_n)^T lam, linearised at V (= V_n). */
+void or_rk4_step_vjp(const or_grid *g, const or_fluid *f, double *const V[3], double *lam[3])
+{
+    const size_t nc = ncplx(g);
+    const double dt = f->dt;
+    double fh = -f->nu * 0.5 * dt, f1 = -f->nu * 1.0 * dt;
+    double *St[4][3], *G[3], *lg[3], *lv[3][3];
+    /* stage inputs V0..V3 */
+    for (int s = 0; s < 4; s++)
+        for (int c = 0; c < 3; c++) St[s][c] = (double *)malloc(sizeof(cplx) * nc);
+    for (int c = 0; c < 3; c++) {
+        memcpy(St[0][c], V[c], sizeof(cplx) * nc);
+        G[c] = (double *)malloc(sizeof(cplx) * nc);
+        lg[c] = (double *)malloc(sizeof(cplx) * nc);
+        for (int s = 0; s < 3; s++) lv[s][c] = (double *)calloc(nc, sizeof(cplx));
+    }
+    {
+        double *W[3];
+        for (int c = 0; c < 3; c++) { W[c] = (double *)malloc(sizeof(cplx) * nc); memcpy(W[c], V[c], sizeof(cplx) * nc); }
+        /* V1 = Eh (D V0 + dt/2 f(V0)) */
+        or_compute_nlin(g, W, G); or_compute_rhs(g, G, NULL);
+        for (int c = 0; c < 3; c++) {
+            cplx *v1 = (cplx *)St[1][c];
+            memcpy(v1, W[c], sizeof(cplx) * nc);  /* W is D V0 after compute_nlin */
+            scale_add(g, v1, (cplx *)G[c], 0.5 * dt, NULL);
+            exp_ksqr(g, St[1][c], fh);
+        }
+        /* V2 = Eh V0 + dt/2 f(V1) */
+        for (int c = 0; c < 3; c++) memcpy(W[c], St[1][c], sizeof(cplx) * nc);
+        or_compute_nlin(g, W, G); or_compute_rhs(g, G, NULL);
+        for (int c = 0; c < 3; c++) {
+            memcpy(St[2][c], V[c], sizeof(cplx) * nc);
+            exp_ksqr(g, St[2][c], fh);
+            scale_add(g, (cplx *)St[2][c], (cplx *)G[c], 0.5 * dt, NULL);
+        }
+        /* V3 = Eh (Eh V0 + dt f(V2)) */
+        for (int c = 0; c < 3; c++) memcpy(W[c], St[2][c], sizeof(cplx) * nc);
+        or_compute_nlin(g, W, G); or_compute_rhs(g, G, NULL);
+        for (int c = 0; c < 3; c++) {
+            memcpy(St[3][c], V[c], sizeof(cplx) * nc);
+            exp_ksqr(g, St[3][c], fh);
+            scale_add(g, (cplx *)St[3][c], (cplx *)G[c], dt, NULL);
+            exp_ksqr(g, St[3][c], fh);
+        }
+        for (int c = 0; c < 3; c++) free(W[c]);
+    }
+    /* backward */
+    double *l4[3];
+    for (int c = 0; c < 3; c++) { l4[c] = (double *)malloc(sizeof(cplx) * nc); memcpy(l4[c], lam[c], sizeof(cplx) * nc); }
+    /* lam_V3 = dt/6 J3^T lam4 */
+    for (int c = 0; c < 3; c++) { memcpy(lg[c], l4[c], sizeof(cplx) * nc); }
+    or_rhs_vjp(g, St[3], lg, lv[2]);
+    for (int c = 0; c < 3; c++) { cplx *x = (cplx *)lv[2][c]; for (size_t i = 0; i < nc; i++) x[i] *= dt / 6; }
+    /* lam_g2 = dt/3 Eh lam4 + dt Eh lam_V3 ; lam_V2 = J2^T lam_g2 */
+    for (int c = 0; c < 3; c++) {
+        memset(lg[c], 0, sizeof(cplx) * nc);
+        scale_add(g, (cplx *)lg[c], (cplx *)l4[c], dt / 3, &fh);
+        scale_add(g, (cplx *)lg[c], (cplx *)lv[2][c], dt, &fh);
+    }
+    or_rhs_vjp(g, St[2], lg, lv[1]);
+    /* lam_g1 = dt/3 Eh lam4 + dt/2 lam_V2 ; lam_V1 = J1^T lam_g1 */
+    for (int c = 0; c < 3; c++) {
+        memset(lg[c], 0, sizeof(cplx) * nc);
+        scale_add(g, (cplx *)lg[c], (cplx *)l4[c], dt / 3, &fh);
+        scale_add(g, (cplx *)lg[c], (cplx *)lv[1][c], dt / 2, NULL);
+    }
+    or_rhs_vjp(g, St[1], lg, lv[0]);
+    /* lam_g0 = dt/6 E1 lam4 + dt/2 Eh lam_V1 */
+    for (int c = 0; c < 3; c++) {
+        memset(lg[c], 0, sizeof(cplx) * nc);
+        scale_add(g, (cplx *)lg[c], (cplx *)l4[c], dt / 6, &f1);
+        scale_add(g, (cplx *)lg[c], (cplx *)lv[0][c], dt / 2, &fh);
+    }
+    or_rhs_vjp(g, St[0], lg, G);
+    /* lam_V0 = E1 lam4 + Eh^2 lam_V3 + Eh lam_V2 + D Eh lam_V1 + J0^T lam_g0 */
+    for (int c = 0; c < 3; c++) {
+        cplx *out = (cplx *)lam[c];
+        memset(out, 0, sizeof(cplx) * nc);
+        scale_add(g, out, (cplx *)l4[c], 1.0, &f1);
+        double f2 = 2 * fh;
+        scale_add(g, out, (cplx *)lv[2][c], 1.0, &f2);
+        scale_add(g, out, (cplx *)lv[1][c], 1.0, &fh);
+        double *t = (double *)malloc(sizeof(cplx) * nc);
+        memcpy(t, lv[0][c], sizeof(cplx) * nc);
+        or_dealias(g, t);
+        scale_add(g, out, (cplx *)t, 1.0, &fh);
+        free(t);
+        scale_add(g, out, (cplx *)G[c], 1.0, NULL);
+    }
+    for (int s = 0; s < 4; s++)
+        for (int c = 0; c < 3; c++) free(St[s][c]);
+    for (int c = 0; c < 3; c++) {
+        free(G[c]); free(lg[c]); free(l4[c]);
+        for (int s = 0; s < 3; s++) free(lv[s][c]);
+    }
+}
+
+/* Loss = energy of the final field in the shell r_lo <= K < r_hi (energy
+ * weights 1/2 on kz = 0), after nsteps RK4 steps from V0; grad = dLoss/dV0
+ * (complex: dL/dRe + i dL/dIm of the stored coefficients).  Forcing and the
+ * reality condition are off.  Stores every state (CPU reference; sizes are
+ * small). */
+double or_adjoint_shell_energy(const or_grid *g0, const or_fluid *f, double *const V0[3], int nsteps, double r_lo,
+                               double r_hi, double *grad[3])
+{
+    or_grid gg = *g0;
+    gg.strong_reality = 0;
+    const or_grid *g = &gg;
+    const size_t nc = ncplx(g);
+    const int nzc = g->nz / 2 + 1;
+    double **hist = (double **)malloc(sizeof(double *) * 3 * (size_t)(nsteps + 1));
+    for (int c = 0; c < 3; c++) { hist[c] = (double *)malloc(sizeof(cplx) * nc); memcpy(hist[c], V0[c], sizeof(cplx) * nc); }
+    or_fluid fl = *f;
+    fl.force_kind = 0;
+    for (int n = 1; n <= nsteps; n++) {
+        double *W[3];
+        for (int c = 0; c < 3; c++) {
+            W[c] = hist[3 * n + c] = (double *)malloc(sizeof(cplx) * nc);
+            memcpy(W[c], hist[3 * (n - 1) + c], sizeof(cplx) * nc);
+        }
+        or_rk4_step(g, &fl, W);
+    }
+    double loss = 0.0;
+    for (int c = 0; c < 3; c++) {
+        const cplx *A = (const cplx *)hist[3 * nsteps + c];
+        cplx *gr = (cplx *)grad[c];
+        for (int lx = 0; lx < g->nx; lx++)
+            for (int ly = 0; ly < g->ny; ly++)
+                for (int lz = 0; lz < nzc; lz++) {
+                    size_t i = cidx(g, lx, ly, lz);
+                    double K = or_kmag(g, lx, ly, lz);
+                    double w = lz == 0 ? 0.5 : 1.0;
+                    if (K >= r_lo && K < r_hi) {
+                        loss += w * (creal(A[i]) * creal(A[i]) + cimag(A[i]) * cimag(A[i]));
+                        gr[i] = 2 * w * A[i];
+                    } else {
+                        gr[i] = 0;
+                    }
+                }
+    }
+    for (int n = nsteps - 1; n >= 0; n--) or_rk4_step_vjp(g, &fl, &hist[3 * n], grad);
+    for (int q = 0; q < 3 * (nsteps + 1); q++) free(hist[q]);
+    free(hist);
+    return loss;
+}
+
+/* ------------------------------------------------------------------------ */
 /* Diagnostics                                                              */
 /* ------------------------------------------------------------------------ */
 

@@ -81,6 +81,13 @@ double or_cross_energy(const or_grid *g, const double *A, const double *B);
 void or_flux_mhd(const or_grid *g, double *V[3], double *W[3], const double *radii, int nspheres, double *u2b,
                  double *b2b);
 
+/* ---- tangent / adjoint (no counterpart in the reference) ---- */
+void or_rhs_jvp(const or_grid *g, double *const V[3], double *const dV[3], double *out[3]);
+void or_rhs_vjp(const or_grid *g, double *const V[3], double *const Lam[3], double *out[3]);
+void or_rk4_step_vjp(const or_grid *g, const or_fluid *f, double *const V[3], double *lam[3]);
+double or_adjoint_shell_energy(const or_grid *g, const or_fluid *f, double *const V0[3], int nsteps, double r_lo,
+                               double r_hi, double *grad[3]);
+
 /* ---- diagnostics ---- */
 double or_energy(const or_grid *g, const double *A);
 double or_Sn(const or_grid *g, const double *A, int n);

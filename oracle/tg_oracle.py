@@ -82,6 +82,10 @@ def lib():
             "or_rk4_step_mhd": (None, [gp, fp, d, pp, pp]),
             "or_cross_energy": (d, [gp, p, p]),
             "or_flux_mhd": (None, [gp, pp, pp, p, i, p, p]),
+            "or_rhs_jvp": (None, [gp, pp, pp, pp]),
+            "or_rhs_vjp": (None, [gp, pp, pp, pp]),
+            "or_rk4_step_vjp": (None, [gp, fp, pp, pp]),
+            "or_adjoint_shell_energy": (d, [gp, fp, pp, i, d, d, pp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(_lib, name)
@@ -245,6 +249,27 @@ class Oracle:
         b2b = np.zeros(len(radii))
         self.L.or_flux_mhd(ctypes.byref(self.g), _pp(V), _pp(W), _ptr(radii), len(radii) - 1, _ptr(u2b), _ptr(b2b))
         return u2b, b2b
+
+    def rhs_jvp(self, V, dV):
+        out = [self.czeros() for _ in range(3)]
+        self.L.or_rhs_jvp(ctypes.byref(self.g), _pp(V), _pp(dV), _pp(out))
+        return out
+
+    def rhs_vjp(self, V, Lam):
+        out = [self.czeros() for _ in range(3)]
+        self.L.or_rhs_vjp(ctypes.byref(self.g), _pp(V), _pp(Lam), _pp(out))
+        return out
+
+    def rk4_step_vjp(self, fluid, V, lam):
+        lam = [l.copy() for l in lam]
+        self.L.or_rk4_step_vjp(ctypes.byref(self.g), ctypes.byref(fluid), _pp(V), _pp(lam))
+        return lam
+
+    def adjoint_shell_energy(self, fluid, V0, nsteps, r_lo, r_hi):
+        grad = [self.czeros() for _ in range(3)]
+        loss = self.L.or_adjoint_shell_energy(ctypes.byref(self.g), ctypes.byref(fluid), _pp(V0), nsteps, r_lo, r_hi,
+                                              _pp(grad))
+        return loss, grad
 
     def init_field(self, kind=0, k0=4.0, kmin=2.0, kmax=170.0, energy=0.5, seed=1):
         V = [self.czeros() for _ in range(3)]

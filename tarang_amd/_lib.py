@@ -68,6 +68,10 @@ SIGNATURES = {
     "tg_init_field": (_I, [_P, _I, _D, _D, _D, _D, ctypes.c_uint64, _PP]),
     "tg_compute_nlin_mhd": (_I, [_P, _PP, _PP, _PP, _PP]),
     "tg_rk4_step_mhd": (_I, [_P, _PP, _PP, ctypes.POINTER(TgFluid), _D, _I]),
+    "tg_rhs_vjp": (_I, [_P, _PP, _PP, _PP]),
+    "tg_adjoint_forward": (_I, [_P, _PP, ctypes.POINTER(TgFluid), _I, _I]),
+    "tg_adjoint_backward": (_I, [_P, _PP]),
+    "tg_adjoint_free": (_I, [_P]),
     "tg_cross_helicity": (_I, [_P, _PP, _PP, _DP]),
     "tg_flux_mhd": (_I, [_P, _PP, _PP, _DP, _I, _DP, _DP]),
     "tg_profile_enable": (_I, [_P, _I]),
@@ -77,7 +81,7 @@ SIGNATURES = {
 }
 
 KERNELS = ["x_inv", "y_fft", "z_products", "x_nlin_rk", "x_nlin_giver", "reality", "z_r2c", "z_c2r", "x_fwd",
-           "reduce"]
+           "reduce", "x_inv_sym", "z_adj", "x_fwd_adj"]
 
 
 def header_symbols():

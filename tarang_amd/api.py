@@ -27,7 +27,7 @@ from ._lib import (DEALIAS_NONE, DEALIAS_REF_CORNER, DEALIAS_TWOTHIRDS, PREC_FP3
                    TgConfig, TgFluid, check, ptr_array)
 
 __all__ = ["FFF_Basis", "FluidVF", "Nlin_incompress", "Time_advance_incompress", "Time_advance_incompress_MHD",
-           "Correlation", "EnergyTr",
+           "Adjoint_incompress", "Correlation", "EnergyTr",
            "DEALIAS_NONE", "DEALIAS_REF_CORNER", "DEALIAS_TWOTHIRDS", "PREC_FP64", "PREC_FP32"]
 
 
@@ -256,6 +256,43 @@ class Time_advance_incompress_MHD:
         b = self.U.basis
         check("tg_rk4_step_mhd", b.L.tg_rk4_step_mhd(b.plan, self.U.ptrs(), self.W.ptrs(), ctypes.byref(self.fluid),
                                                      self.W.dissipation_coefficient, int(nsteps)))
+
+
+class Adjoint_incompress:
+    """Adjoint of the unforced Lawson RK4 map of U (no counterpart in the
+    reference).  Forward() advances U.cvf in place, keeping a checkpoint every
+    checkpoint_every steps and the real-space stage velocities of the last
+    segment; Backward(lam) maps dL/dV_n to dL/dV_0 in place, recomputing the
+    earlier segments.  Gradients of complex coefficients are dL/dRe + i dL/dIm
+    (the convention of torch.autograd for complex tensors).  The plan must have
+    strong_reality off."""
+
+    def __init__(self, U, dt, checkpoint_every=20):
+        self.U = U
+        self.fluid = TgFluid(U.dissipation_coefficient, float(dt), 0, 0.0, 0.0, 0.0)
+        self.checkpoint_every = int(checkpoint_every)
+
+    def Forward(self, nsteps):
+        b = self.U.basis
+        check("tg_adjoint_forward", b.L.tg_adjoint_forward(b.plan, self.U.ptrs(), ctypes.byref(self.fluid),
+                                                           int(nsteps), self.checkpoint_every))
+
+    def Backward(self, lam):
+        b = self.U.basis
+        check("tg_adjoint_backward", b.L.tg_adjoint_backward(b.plan, ptr_array([x.data_ptr() for x in lam])))
+        return lam
+
+    def Rhs_vjp(self, lam):
+        """f'(V)^T lam for f(V) = -P NL(D V) (the unforced Compute_rhs) at V = U.cvf."""
+        b = self.U.basis
+        out = [torch.empty_like(self.U.cvf[0]) for _ in range(3)]
+        check("tg_rhs_vjp", b.L.tg_rhs_vjp(b.plan, self.U.ptrs(), ptr_array([x.data_ptr() for x in lam]),
+                                           ptr_array([x.data_ptr() for x in out])))
+        return out
+
+    def Free(self):
+        b = self.U.basis
+        check("tg_adjoint_free", b.L.tg_adjoint_free(b.plan))
 
 
 class EnergyTr:

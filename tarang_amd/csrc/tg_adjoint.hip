@@ -1,0 +1,297 @@
+// tg_adjoint.hip -- adjoint of the NS right-hand side f(V) = -P NL(D V),
+// NL_i = sum_j i K_j F[u_j u_i], u = I(D V), and of the Lawson RK4 step.
+// (No counterpart in the reference.)
+//
+// Inner product: the real dot product of the stored half-spectrum (Re, Im).
+// With c = 1 on the kz = 0 and kz = Nz/2 planes and 2 elsewhere the
+// transposes of the transforms are I^T = N^3 diag(c) F and
+// F^T = N^-3 I diag(1/c); (i K)^T = -i K; P and D are symmetric.  So
+//   f'(V)^T G = -D diag(c) F[q],  q_m = sum_j u_j sigma_jm,
+//   sigma_jm = I(diag(1/c)(-i K_m M_j - i K_j M_m)),  M = P G,
+// which needs the stage's u (taped by the forward sweep), six inverse and
+// three forward transforms: x-inverse of the symmetrised fields (built from
+// M in registers), y-inverse, z c2r + products with the tape + r2c,
+// y-forward, x-forward with the stage epilogue.
+#include "tg_adjoint.h"
+#include "tg_device.h"
+#include "tg_pass.h"
+
+namespace tg {
+
+struct XsymArgs {
+    const double *G[3];
+    void *S[6];
+    const void *tw;
+    Grid g;
+};
+
+template <typename T, int L, int CB>
+__global__ void __launch_bounds__(CB *(L / 8)) k_xinv_sym(XsymArgs a)
+{
+    constexpr int TPL = L / 8;
+    constexpr int LP = lpad_len(L);
+    __shared__ __attribute__((aligned(16))) char raw[CB * LP * sizeof(cpx<double>)];
+    __shared__ int colw[CB], cols[CB];
+    cpx<double> *ldd = (cpx<double> *)raw;
+    cpx<T> *ldt = (cpx<T> *)raw;
+    const Grid &g = a.g;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    if (tile >= x_tiles(g, CB, false)) return;
+    tile_columns<CB>(colw, cols, g, false, tile);
+    __syncthreads();
+    const int c = threadIdx.x / TPL, j = threadIdx.x % TPL;
+    const int cw = colw[c];
+    const int ly = cw >= 0 ? cw / g.nzp : 0, lz = cw >= 0 ? cw % g.nzp : 0;
+    const double Ky = kwave(ly, g.ny) * g.kf[1], Kz = lz * g.kf[2];
+    const cpx<T> *tw = (const cpx<T> *)a.tw;
+    const int ncw = g.ny * g.nzp, ncs = g.ny * g.nzc;
+    cpx<double> m[3][8];
+#pragma unroll
+    for (int comp = 0; comp < 3; comp++) {
+        stage_in<L, CB>(ldd, (const cpx<double> *)a.G[comp], cols, ncs);
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < 8; s++) m[comp][s] = ldd[c * LP + lpad(j + TPL * s)];
+        __syncthreads();
+    }
+    // M = P G (Pressure.cc:56-72 form: G - K (K.G)/K^2, identity at K = 0)
+#pragma unroll
+    for (int s = 0; s < 8; s++) {
+        const double Kx = kwave(j + TPL * s, g.nx) * g.kf[0];
+        const double ks = (Kx * Kx + Ky * Ky) + Kz * Kz;
+        if (ks > 0.0) {
+            cpx<double> d = Kx * m[0][s];
+            d = d + Ky * m[1][s];
+            d = d + Kz * m[2][s];
+            const cpx<double> e = mk<double>(d.x / ks, d.y / ks);
+            m[0][s] = m[0][s] - Kx * e;
+            m[1][s] = m[1][s] - Ky * e;
+            m[2][s] = m[2][s] - Kz * e;
+        }
+    }
+    const double w = (lz == 0 || lz == g.nz / 2) ? 1.0 : 0.5;
+    constexpr int pa[6] = {0, 1, 2, 0, 1, 2};
+    constexpr int pb[6] = {0, 1, 2, 1, 2, 0};
+#pragma unroll
+    for (int t = 0; t < 6; t++) {
+        cpx<T> v[8];
+#pragma unroll
+        for (int s = 0; s < 8; s++) {
+            const double Kx = kwave(j + TPL * s, g.nx) * g.kf[0];
+            const double K[3] = {Kx, Ky, Kz};
+            const cpx<double> A = m[pa[t]][s], B = m[pb[t]][s];
+            // -i K_b A - i K_a B, times 1/c
+            const double re = K[pb[t]] * A.y + K[pa[t]] * B.y;
+            const double im = -(K[pb[t]] * A.x + K[pa[t]] * B.x);
+            v[s] = (cw >= 0) ? mk<T>((T)(w * re), (T)(w * im)) : mk<T>(0, 0);
+        }
+        fft_line<L, +1>(v, ldt + c * LP, tw, j);
+#pragma unroll
+        for (int s = 0; s < 8; s++) ldt[c * LP + lpad(j + TPL * s)] = v[s];
+        __syncthreads();
+        stage_out<L, CB>(ldt, (cpx<T> *)a.S[t], colw, ncw);
+        __syncthreads();
+    }
+}
+
+// q += u * v on the packed real points of this thread (u from the tape line).
+template <typename T, int M>
+__device__ __forceinline__ void tape_mac(cpx<T> *q, const cpx<T> *v, const cpx<T> *__restrict__ u, int j)
+{
+    constexpr int TPL = M / 8;
+#pragma unroll
+    for (int s = 0; s < 8; s++) {
+        const cpx<T> A = u[j + TPL * s];
+        q[s] = q[s] + mk<T>(A.x * v[s].x, A.y * v[s].y);
+    }
+}
+
+// sigma lines in a.in[]: 0 xx, 1 yy, 2 zz, 3 xy, 4 yz, 5 zx.  One
+// accumulator at a time: q_m = sum_j u_j sigma_jm over three c2r, then its
+// r2c (nine c2r per line; the off-diagonal sigma lines are read twice, the
+// second time from cache).  With more accumulators live the register
+// allocation of the line transforms grows past the register file.
+template <typename T, int M>
+__global__ void __launch_bounds__(256) k_z_adj(ZArgs a)
+{
+    constexpr int TPL = M / 8;
+    constexpr int LPB = z_lines_per_block<M>();
+    constexpr int LP = lpad_len(M);
+    __shared__ cpx<T> lds[LPB * LP];
+    const int li = threadIdx.x / TPL, j = threadIdx.x % TPL;
+    const size_t line = (size_t)blockIdx.x * LPB + li;
+    const size_t nlines = (size_t)a.g.nx * a.g.ny;
+    const bool valid = line < nlines;
+    const size_t lsafe = valid ? line : nlines - 1;
+    const cpx<T> *twm = (const cpx<T> *)a.twm, *twn = (const cpx<T> *)a.twn;
+    cpx<T> *L = lds + li * LP;
+    const size_t off = lsafe * a.g.nzp;
+    constexpr int sig[3][3] = {{0, 3, 5}, {3, 1, 4}, {5, 4, 2}};  // sigma_jm -> line
+#pragma unroll 1
+    for (int m = 0; m < 3; m++) {
+        cpx<T> q[8], v[8];
+#pragma unroll
+        for (int s = 0; s < 8; s++) q[s] = mk<T>(0, 0);
+#pragma unroll
+        for (int jj = 0; jj < 3; jj++) {
+            const int t = m == 0 ? sig[0][jj] : (m == 1 ? sig[1][jj] : sig[2][jj]);
+            const cpx<T> *X = (const cpx<T> *)(t == 0 ? a.in[0] : t == 1 ? a.in[1] : t == 2 ? a.in[2] :
+                                                t == 3 ? a.in[3] : t == 4 ? a.in[4] : a.in[5]);
+            zline_c2r<T, M>(v, X + off, L, twm, twn, j);
+            line_sync<M / 8>();
+            tape_mac<T, M>(q, v, (const cpx<T> *)a.tape[jj] + lsafe * M, j);
+        }
+        cpx<T> *Q = (cpx<T> *)(m == 0 ? a.out[0] : m == 1 ? a.out[1] : a.out[2]);
+        zline_r2c_post<T, M>(q, Q + off, L, twm, twn, j, valid);
+    }
+}
+
+struct XadjArgs {
+    void *Q[3];
+    double *lam[3], *acc[3], *lg[3];
+    const void *tw;
+    Grid g;
+    AdjParams ap;
+};
+
+template <typename T, int L, int CB>
+__global__ void __launch_bounds__(CB *(L / 8)) k_xfwd_adj(XadjArgs a)
+{
+    constexpr int TPL = L / 8;
+    constexpr int LP = lpad_len(L);
+    __shared__ __attribute__((aligned(16))) char raw[CB * LP * sizeof(cpx<double>)];
+    __shared__ int colw[CB], cols[CB];
+    cpx<double> *ldd = (cpx<double> *)raw;
+    cpx<T> *ldt = (cpx<T> *)raw;
+    const Grid &g = a.g;
+    const AdjParams &ap = a.ap;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    if (tile >= x_tiles(g, CB, false)) return;
+    tile_columns<CB>(colw, cols, g, false, tile);
+    __syncthreads();
+    const int c = threadIdx.x / TPL, j = threadIdx.x % TPL;
+    const cpx<T> *tw = (const cpx<T> *)a.tw;
+    const int ncw = g.ny * g.nzp, ncs = g.ny * g.nzc;
+    const double c_sixth = (1.0 / 6) * ap.dt, c_third = (1.0 / 3) * ap.dt, c_half = 0.5 * ap.dt;
+    for (int comp = 0; comp < 3; comp++) {
+        stage_in<L, CB>(ldt, (const cpx<T> *)a.Q[comp], colw, ncw);
+        __syncthreads();
+        cpx<T> v[8];
+#pragma unroll
+        for (int s = 0; s < 8; s++) v[s] = ldt[c * LP + lpad(j + TPL * s)];
+        __syncthreads();
+        fft_line<L, -1>(v, ldt + c * LP, tw, j);
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < 8; s++) ldd[c * LP + lpad(j + TPL * s)] = ap.scale * cvt<double>(v[s]);
+        __syncthreads();
+        cpx<double> *lam = (cpx<double> *)a.lam[comp], *acc = (cpx<double> *)a.acc[comp];
+        cpx<double> *lg = (cpx<double> *)a.lg[comp];
+        for (int e = threadIdx.x; e < L * CB; e += CB * TPL) {
+            const int lx = e / CB, cc = e % CB, col = cols[cc];
+            if (col < 0) continue;
+            const int ly = col / g.nzc, lz = col % g.nzc;
+            const size_t i = (size_t)lx * ncs + col;
+            cpx<double> r = ldd[cc * LP + lpad(lx)];
+            const double cw = (lz == 0 || lz == g.nz / 2) ? -1.0 : -2.0;
+            r = dealias_zero(g, lx, ly, lz) ? mk<double>(0, 0) : cw * r;
+            if (ap.mode == ADJ_RHS) {
+                lg[i] = r;
+                continue;
+            }
+            const double Kx = kwave(lx, g.nx) * g.kf[0], Ky = kwave(ly, g.ny) * g.kf[1], Kz = lz * g.kf[2];
+            const double ks = (Kx * Kx + Ky * Ky) + Kz * Kz;
+            const cpx<double> l4 = lam[i];
+            if (ap.mode == ADJ_S3) {
+                const double Eh = exp(ap.fh * ks);
+                const cpx<double> lv = c_sixth * r;
+                acc[i] = (Eh * Eh) * lv;
+                lg[i] = c_third * (Eh * l4) + ap.dt * (Eh * lv);
+            } else if (ap.mode == ADJ_S2) {
+                const double Eh = exp(ap.fh * ks);
+                acc[i] = acc[i] + Eh * r;
+                lg[i] = c_third * (Eh * l4) + c_half * r;
+            } else if (ap.mode == ADJ_S1) {
+                const double Eh = exp(ap.fh * ks), E1 = exp(ap.f1 * ks);
+                acc[i] = acc[i] + Eh * r;
+                lg[i] = c_sixth * (E1 * l4) + c_half * (Eh * r);
+            } else {
+                const double E1 = exp(ap.f1 * ks);
+                lam[i] = (E1 * l4 + acc[i]) + r;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template <typename T>
+hipError_t launch_xinv_sym(const Grid &g, double *const *G, void *const *S, const void *twx, hipStream_t st)
+{
+    XsymArgs a{};
+    for (int i = 0; i < 3; i++) a.G[i] = G[i];
+    for (int i = 0; i < 6; i++) a.S[i] = S[i];
+    a.tw = twx;
+    a.g = g;
+    TG_DISPATCH_L(g.nx, {
+        constexpr int CB = col_block<T>(LL, sizeof(cpx<double>));
+        hipLaunchKernelGGL((k_xinv_sym<T, LL, CB>), dim3(x_grid(x_tiles(g, CB, false))), dim3(CB * (LL / 8)), 0, st,
+                           a);
+    });
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_z_adj(const Grid &g, void *const *S, void *const *Q, void *const *tape, const void *twm,
+                        const void *twn, hipStream_t st)
+{
+    ZArgs a{};
+    for (int i = 0; i < 6; i++) a.in[i] = S[i];
+    for (int i = 0; i < 3; i++) {
+        a.out[i] = Q[i];
+        a.tape[i] = tape[i];
+    }
+    a.twm = twm;
+    a.twn = twn;
+    a.g = g;
+    const size_t nlines = (size_t)g.nx * g.ny;
+    TG_DISPATCH_M(g.nz / 2, {
+        constexpr int LPB = z_lines_per_block<MM>();
+        dim3 grid((unsigned)((nlines + LPB - 1) / LPB));
+        hipLaunchKernelGGL((k_z_adj<T, MM>), grid, dim3(LPB * (MM / 8)), 0, st, a);
+    });
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_xfwd_adj(const Grid &g, const AdjParams &ap, void *const *Q, double *const *lam,
+                           double *const *acc, double *const *lg, const void *twx, hipStream_t st)
+{
+    XadjArgs a{};
+    for (int i = 0; i < 3; i++) {
+        a.Q[i] = Q[i];
+        a.lam[i] = lam ? lam[i] : nullptr;
+        a.acc[i] = acc ? acc[i] : nullptr;
+        a.lg[i] = lg ? lg[i] : nullptr;
+    }
+    a.tw = twx;
+    a.g = g;
+    a.ap = ap;
+    TG_DISPATCH_L(g.nx, {
+        constexpr int CB = col_block<T>(LL, sizeof(cpx<double>));
+        hipLaunchKernelGGL((k_xfwd_adj<T, LL, CB>), dim3(x_grid(x_tiles(g, CB, false))), dim3(CB * (LL / 8)), 0, st,
+                           a);
+    });
+    return hipGetLastError();
+}
+
+#define TG_ADJ_INST(T)                                                                                           \
+    template hipError_t launch_xinv_sym<T>(const Grid &, double *const *, void *const *, const void *, hipStream_t); \
+    template hipError_t launch_z_adj<T>(const Grid &, void *const *, void *const *, void *const *, const void *,  \
+                                        const void *, hipStream_t);                                              \
+    template hipError_t launch_xfwd_adj<T>(const Grid &, const AdjParams &, void *const *, double *const *,       \
+                                           double *const *, double *const *, const void *, hipStream_t);
+
+TG_ADJ_INST(double)
+TG_ADJ_INST(float)
+
+}  // namespace tg

@@ -1,6 +1,7 @@
 // tg_api.hip -- plan object and the extern "C" boundary (include/tarang_gpu.h).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -11,6 +12,7 @@
 #include <vector>
 
 #include "../../include/tarang_gpu.h"
+#include "tg_adjoint.h"
 #include "tg_diag.h"
 #include "tg_kernels.h"
 #include "tg_mhd.h"
@@ -73,6 +75,19 @@ struct tg_plan {
     // MHD buffers (FP64, boundary layout), allocated on first use:
     // [0..2] accW, [3..5] N+, [6..8] N-, [9..11] Z+, [12..14] Z-
     double *mhd[15] = {};
+    // adjoint: checkpoints (FP64 spectral, 3 components each), tape of the
+    // real-space stage velocities (every x 4 stages x 3 components, packed
+    // z lines in the transform precision), recompute state, cotangent and
+    // accumulator (FP64 spectral)
+    struct Adjoint {
+        int nsteps = 0, every = 0, nsegs_used = 0, cached_seg = -1;
+        int nseg = 0, tape_steps = 0;  // allocated capacity (checkpoints, tape steps)
+        tg_fluid fluid{};
+        double *ckpt = nullptr;
+        void *tape = nullptr;
+        double *vw = nullptr, *lg = nullptr, *acc = nullptr;
+        bool recorded = false;
+    } adj;
     // per-kernel HIP event timing (tg_profile_*)
     bool prof_on = false;
     struct ProfRec { int kid; hipEvent_t a, b; };
@@ -177,6 +192,9 @@ void free_plan(tg_plan *p)
     for (void *q : p->P)
         if (q) (void)hipFree(q);
     for (void *q : p->mhd)
+        if (q) (void)hipFree(q);
+    void *aptrs[] = {p->adj.ckpt, p->adj.tape, p->adj.vw, p->adj.lg, p->adj.acc};
+    for (void *q : aptrs)
         if (q) (void)hipFree(q);
     for (auto &r : p->prof_recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
     for (auto e : p->prof_pool) (void)hipEventDestroy(e);
@@ -447,6 +465,211 @@ int run_nlin_mhd(tg_plan *p, double *const V[3], double *const W[3], double *con
     if (rc) return rc;
     HIPCHK(launch_mhd_combine(p->g, NU, NW, p->stream));
     return TG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint
+// ---------------------------------------------------------------------------
+size_t tape_field_elems(const tg_plan *p) { return (size_t)p->g.nx * p->g.ny * (p->g.nz / 2); }
+
+void *tape_ptr(tg_plan *p, int step, int stage, int comp)
+{
+    const size_t f = tape_field_elems(p) * p->elem;
+    return (char *)p->adj.tape + ((size_t)(step * 4 + stage) * 3 + comp) * f;
+}
+
+void adj_free(tg_plan *p)
+{
+    auto &a = p->adj;
+    const size_t nc = p->nc;
+    void *ptrs[] = {a.ckpt, a.tape, a.vw, a.lg, a.acc};
+    for (void *q : ptrs)
+        if (q) (void)hipFree(q);
+    p->dev_bytes -= (int64_t)((size_t)a.nseg * 3 * nc * 16 + (size_t)a.tape_steps * 12 * tape_field_elems(p) * p->elem +
+                              (a.vw ? 9 * nc * 16 : 0));
+    a = tg_plan::Adjoint{};
+}
+
+// Adjoint buffers for at least nseg checkpoints and tape_steps steps of tape
+// (kept when the current ones are large enough).
+int adj_alloc(tg_plan *p, int nseg, int tape_steps)
+{
+    auto &a = p->adj;
+    if (a.vw && a.nseg >= nseg && a.tape_steps >= tape_steps) return TG_OK;
+    if (a.vw) {
+        nseg = std::max(nseg, a.nseg);
+        tape_steps = std::max(tape_steps, a.tape_steps);
+    }
+    (void)hipStreamSynchronize(p->stream);
+    adj_free(p);
+    const size_t nc = p->nc;
+    int rc = dmalloc(p, (void **)&a.ckpt, (size_t)nseg * 3 * nc * 16);
+    if (!rc) rc = dmalloc(p, &a.tape, (size_t)tape_steps * 12 * tape_field_elems(p) * p->elem);
+    if (!rc) rc = dmalloc(p, (void **)&a.vw, 3 * nc * 16);
+    if (!rc) rc = dmalloc(p, (void **)&a.lg, 3 * nc * 16);
+    if (!rc) rc = dmalloc(p, (void **)&a.acc, 3 * nc * 16);
+    a.nseg = nseg;
+    a.tape_steps = tape_steps;
+    if (rc) {
+        adj_free(p);
+        return rc;
+    }
+    return TG_OK;
+}
+
+// nsteps unforced RK4 steps of V (no reality condition), storing the
+// real-space velocity of every stage input in tape step slots 0..nsteps-1.
+template <typename T>
+int record_steps(tg_plan *p, double *const V[3], const tg_fluid *f, int nsteps)
+{
+    Band nb{};
+    p->band = nb;
+    {
+        Prof pr(p, TG_KERNEL_X_INV);
+        HIPCHK(launch_xinv_state<T>(p->g, nb, V, 3, p->U, nullptr, 1, p->twx, p->stream));
+    }
+    for (int step = 0; step < nsteps; step++)
+        for (int s = 1; s <= 4; s++) {
+            void *tp[3] = {tape_ptr(p, step, s - 1, 0), tape_ptr(p, step, s - 1, 1), tape_ptr(p, step, s - 1, 2)};
+            { Prof pr(p, TG_KERNEL_Y_FFT); HIPCHK(launch_colfft<T>(p->g, 1, +1, p->U, 3, p->twy, p->stream)); }
+            {
+                Prof pr(p, TG_KERNEL_Z_PRODUCTS);
+                HIPCHK(launch_z_products<T>(p->g, p->U, p->P, 0, p->twm, p->twn, p->stream, tp));
+            }
+            { Prof pr(p, TG_KERNEL_Y_FFT); HIPCHK(launch_colfft<T>(p->g, 1, -1, p->P, 6, p->twy, p->stream)); }
+            RkParams rk = rk_params(p, f, s);
+            Prof pr(p, TG_KERNEL_X_NLIN_RK);
+            HIPCHK(launch_x_nlin<T>(p->g, nb, rk, p->P, V, p->acc, p->U, nullptr, nullptr, nullptr, p->twx,
+                                    p->stream, p->x_tile_columns));
+        }
+    p->unext_valid = false;
+    return TG_OK;
+}
+
+// One transposed right-hand side application with the stage epilogue:
+// cotangent G, taped velocity tp, outputs per AdjParams.
+template <typename T>
+int adj_stage(tg_plan *p, double *const G[3], void *const tp[3], const AdjParams &ap, double *const lam[3],
+              double *const acc[3], double *const lg[3])
+{
+    void *S[6] = {p->P[0], p->P[1], p->P[2], p->P[3], p->P[4], p->P[5]};
+    void *Q[3] = {p->P[6], p->P[7], p->P[8]};
+    { Prof pr(p, TG_KERNEL_X_INV_SYM); HIPCHK(launch_xinv_sym<T>(p->g, G, S, p->twx, p->stream)); }
+    { Prof pr(p, TG_KERNEL_Y_FFT); HIPCHK(launch_colfft<T>(p->g, 1, +1, S, 6, p->twy, p->stream)); }
+    { Prof pr(p, TG_KERNEL_Z_ADJ); HIPCHK(launch_z_adj<T>(p->g, S, Q, tp, p->twm, p->twn, p->stream)); }
+    { Prof pr(p, TG_KERNEL_Y_FFT); HIPCHK(launch_colfft<T>(p->g, 1, -1, Q, 3, p->twy, p->stream)); }
+    Prof pr(p, TG_KERNEL_X_FWD_ADJ);
+    HIPCHK(launch_xfwd_adj<T>(p->g, ap, Q, lam, acc, lg, p->twx, p->stream));
+    return TG_OK;
+}
+
+AdjParams adj_params(const tg_plan *p, const tg_fluid *f, int mode)
+{
+    AdjParams ap{};
+    ap.mode = mode;
+    ap.dt = f ? f->dt : 0.0;
+    ap.fh = f ? -f->nu * 0.5 * f->dt : 0.0;
+    ap.f1 = f ? -f->nu * 1.0 * f->dt : 0.0;
+    ap.scale = 1.0 / ((double)p->g.nx * p->g.ny * p->g.nz);
+    return ap;
+}
+
+// lam <- (dV_{n+1}/dV_n)^T lam for the taped step slot `step`.
+template <typename T>
+int adj_step(tg_plan *p, int step, double *const lam[3])
+{
+    const size_t nc = p->nc;
+    double *lg[3] = {p->adj.lg, p->adj.lg + 2 * nc, p->adj.lg + 4 * nc};
+    double *acc[3] = {p->adj.acc, p->adj.acc + 2 * nc, p->adj.acc + 4 * nc};
+    const tg_fluid *f = &p->adj.fluid;
+    for (int s = 3; s >= 0; s--) {
+        void *tp[3] = {tape_ptr(p, step, s, 0), tape_ptr(p, step, s, 1), tape_ptr(p, step, s, 2)};
+        int rc = adj_stage<T>(p, s == 3 ? lam : lg, tp, adj_params(p, f, s), lam, acc, lg);
+        if (rc) return rc;
+    }
+    return TG_OK;
+}
+
+int seg_steps(const tg_plan *p, int seg)
+{
+    const int left = p->adj.nsteps - seg * p->adj.every;
+    return left < p->adj.every ? left : p->adj.every;
+}
+
+template <typename T>
+int run_adjoint_forward(tg_plan *p, double *const V[3], const tg_fluid *f, int nsteps, int every)
+{
+    if (every > nsteps) every = nsteps;
+    const int nseg = (nsteps + every - 1) / every;
+    int rc = adj_alloc(p, nseg, every);
+    if (rc) return rc;
+    auto &a = p->adj;
+    a.nsteps = nsteps;
+    a.every = every;
+    a.nsegs_used = nseg;
+    a.fluid = *f;
+    a.recorded = false;
+    const size_t nc = p->nc;
+    for (int seg = 0; seg < nseg; seg++) {
+        for (int c = 0; c < 3; c++)
+            HIPCHK(hipMemcpyAsync(a.ckpt + ((size_t)seg * 3 + c) * 2 * nc, V[c], nc * 16, hipMemcpyDeviceToDevice,
+                                  p->stream));
+        const int n = seg_steps(p, seg);
+        if (seg == nseg - 1) rc = record_steps<T>(p, V, f, n);
+        else rc = run_rk4<T>(p, V, f, n, 0);
+        if (rc) return rc;
+    }
+    a.cached_seg = nseg - 1;
+    a.recorded = true;
+    p->unext_valid = false;
+    return TG_OK;
+}
+
+template <typename T>
+int run_adjoint_backward(tg_plan *p, double *const lam[3])
+{
+    auto &a = p->adj;
+    const size_t nc = p->nc;
+    for (int seg = a.nsegs_used - 1; seg >= 0; seg--) {
+        const int n = seg_steps(p, seg);
+        if (a.cached_seg != seg) {
+            double *vw[3] = {a.vw, a.vw + 2 * nc, a.vw + 4 * nc};
+            for (int c = 0; c < 3; c++)
+                HIPCHK(hipMemcpyAsync(vw[c], a.ckpt + ((size_t)seg * 3 + c) * 2 * nc, nc * 16,
+                                      hipMemcpyDeviceToDevice, p->stream));
+            int rc = record_steps<T>(p, vw, &a.fluid, n);
+            if (rc) return rc;
+            a.cached_seg = seg;
+        }
+        for (int st = n - 1; st >= 0; st--) {
+            int rc = adj_step<T>(p, st, lam);
+            if (rc) return rc;
+        }
+    }
+    return TG_OK;
+}
+
+template <typename T>
+int run_rhs_vjp(tg_plan *p, double *const V[3], double *const lam[3], double *const out[3])
+{
+    // one tape step slot for u = I(D V)
+    if (!p->adj.tape) {
+        int rc = adj_alloc(p, 1, 1);
+        if (rc) return rc;
+        p->adj.recorded = false;
+    }
+    Band nb{};
+    void *tp[3] = {tape_ptr(p, 0, 0, 0), tape_ptr(p, 0, 0, 1), tape_ptr(p, 0, 0, 2)};
+    { Prof pr(p, TG_KERNEL_X_INV); HIPCHK(launch_xinv_state<T>(p->g, nb, V, 3, p->U, nullptr, 1, p->twx, p->stream)); }
+    { Prof pr(p, TG_KERNEL_Y_FFT); HIPCHK(launch_colfft<T>(p->g, 1, +1, p->U, 3, p->twy, p->stream)); }
+    {
+        Prof pr(p, TG_KERNEL_Z_PRODUCTS);
+        HIPCHK(launch_z_products<T>(p->g, p->U, p->P, 0, p->twm, p->twn, p->stream, tp));
+    }
+    p->unext_valid = false;
+    // the tape slot now differs from any recorded segment
+    if (p->adj.cached_seg >= 0) p->adj.cached_seg = -1;
+    return adj_stage<T>(p, lam, tp, adj_params(p, nullptr, ADJ_RHS), nullptr, nullptr, out);
 }
 
 int reduce_to_host(tg_plan *p, int mode, const double *const *A, const double *const *B, int ncomp, int n,
@@ -841,6 +1064,50 @@ int tg_flux_mhd(tg_plan *plan, double *const V_dev[3], double *const W_dev[3], c
     return TG_OK;
 }
 
+int tg_rhs_vjp(tg_plan *plan, double *const V_dev[3], double *const lam_dev[3], double *const out_dev[3])
+{
+    if (!plan) return fail(TG_ERR_ARG, "null plan");
+    int rc = check_ptrs((const void *const *)V_dev, 3, "V");
+    if (!rc) rc = check_ptrs((const void *const *)lam_dev, 3, "lam");
+    if (!rc) rc = check_ptrs((const void *const *)out_dev, 3, "out");
+    if (!rc) rc = set_device(plan);
+    if (rc) return rc;
+    return DISPATCH_PREC(plan, run_rhs_vjp)(plan, V_dev, lam_dev, out_dev);
+}
+
+int tg_adjoint_forward(tg_plan *plan, double *const V_dev[3], const tg_fluid *fluid, int nsteps, int checkpoint_every)
+{
+    if (!plan || !fluid) return fail(TG_ERR_ARG, "null argument");
+    if (nsteps < 1 || checkpoint_every < 1) return fail(TG_ERR_ARG, "need nsteps >= 1 and checkpoint_every >= 1");
+    if (!(fluid->dt > 0) || !(fluid->nu >= 0)) return fail(TG_ERR_ARG, "need dt > 0 and nu >= 0");
+    if (fluid->force_kind != 0) return fail(TG_ERR_ARG, "the adjoint supports force_kind 0 only");
+    if (plan->cfg.strong_reality) return fail(TG_ERR_ARG, "the adjoint needs a plan with strong_reality 0");
+    int rc = check_ptrs((const void *const *)V_dev, 3, "V");
+    if (!rc) rc = set_device(plan);
+    if (rc) return rc;
+    return DISPATCH_PREC(plan, run_adjoint_forward)(plan, V_dev, fluid, nsteps, checkpoint_every);
+}
+
+int tg_adjoint_backward(tg_plan *plan, double *const lam_dev[3])
+{
+    if (!plan) return fail(TG_ERR_ARG, "null plan");
+    if (!plan->adj.recorded) return fail(TG_ERR_ARG, "no tg_adjoint_forward sweep recorded on this plan");
+    int rc = check_ptrs((const void *const *)lam_dev, 3, "lam");
+    if (!rc) rc = set_device(plan);
+    if (rc) return rc;
+    return DISPATCH_PREC(plan, run_adjoint_backward)(plan, lam_dev);
+}
+
+int tg_adjoint_free(tg_plan *plan)
+{
+    if (!plan) return fail(TG_ERR_ARG, "null plan");
+    int rc = set_device(plan);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(plan->stream));
+    adj_free(plan);
+    return TG_OK;
+}
+
 int tg_profile_enable(tg_plan *plan, int on)
 {
     if (!plan) return fail(TG_ERR_ARG, "null plan");
@@ -872,7 +1139,8 @@ int tg_profile_read(tg_plan *plan, int kernel, double *total_ms, int64_t *launch
 const char *tg_kernel_name(int kernel)
 {
     static const char *names[TG_KERNEL_COUNT] = {"x_inv", "y_fft", "z_products", "x_nlin_rk", "x_nlin_giver",
-                                                 "reality", "z_r2c", "z_c2r", "x_fwd", "reduce"};
+                                                 "reality", "z_r2c", "z_c2r", "x_fwd", "reduce", "x_inv_sym",
+                                                 "z_adj", "x_fwd_adj"};
     return (kernel >= 0 && kernel < TG_KERNEL_COUNT) ? names[kernel] : "unknown";
 }
 

@@ -56,9 +56,11 @@ hipError_t launch_z_r2c(const Grid &g, const double *const *R, int ncomp, void *
 // z pass of the NS nonlinear term: 3 velocity lines in, 6 product lines out
 // (order xx, yy, zz, xy, yz, zx).  giver != 0: 6 lines in (u then w),
 // 9 products out (u_j w_i, order j-major: (x,x),(x,y),(x,z),(y,x)...).
+// tape (NS form only): also store the three real velocity lines, each line as
+// Nz/2 packed points (x[2n] + i x[2n+1]) at line*(Nz/2).
 template <typename T>
 hipError_t launch_z_products(const Grid &g, void *const *U, void *const *P, int giver, const void *twm,
-                             const void *twn, hipStream_t st);
+                             const void *twn, hipStream_t st, void *const *tape = nullptr);
 
 struct RkParams {
     int stage;            // 1..4; 0 = write the nonlinear term only; -1 = giver

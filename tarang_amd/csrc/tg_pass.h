@@ -1,0 +1,235 @@
+// tg_pass.h -- device helpers shared by the x, y and z passes (tiling of the
+// x pass through LDS, z-line real<->complex transforms, wavenumbers and the
+// dealias mask) and the size dispatch of the launchers.
+#pragma once
+#include "tg_device.h"
+#include "tg_kernels.h"
+
+namespace tg {
+
+static constexpr int kLdsBudget = 73728;  // bytes; two blocks fit the 160 KB LDS of a CU
+
+__device__ __forceinline__ int kwave(int l, int n) { return l <= n / 2 ? l : l - n; }
+
+__device__ __forceinline__ bool dealias_zero(const Grid &g, int lx, int ly, int lz)
+{
+    if (g.dealias == 1) {
+        // FFF_PENCIL::Dealias (FFF_pencil_basic.cc:87-90) via Assign_sub_array
+        // (:228-259): intersection of lx in [Nx/3+1, 2Nx/3-1],
+        // ly in [Ny/3+1, 2Ny/3-1], lz in [Nz/3+1, Nz/2]; an empty y/z range is
+        // clamped to its first index.
+        int ylo = g.ny / 3 + 1, yhi = 2 * g.ny / 3 - 1;
+        if (yhi < ylo) yhi = ylo;
+        int zlo = g.nz / 3 + 1, zhi = g.nz / 2;
+        if (zhi < zlo) zhi = zlo;
+        return (lx >= g.nx / 3 + 1 && lx <= 2 * g.nx / 3 - 1) && (ly >= ylo && ly <= yhi) && (lz >= zlo && lz <= zhi);
+    }
+    if (g.dealias == 2) {
+        int kx = abs(kwave(lx, g.nx)), ky = abs(kwave(ly, g.ny));
+        return kx > g.nx / 3 || ky > g.ny / 3 || lz > g.nz / 3;
+    }
+    return false;
+}
+
+template <typename T>
+__host__ __device__ constexpr int col_block(int L, int elem_bytes)
+{
+    int tpl = L / 8;
+    int cb = 16;
+    while (cb > 1 && (cb * tpl > 1024 || cb * lpad_len(L) * elem_bytes > kLdsBudget)) cb /= 2;
+    return cb;
+}
+
+template <typename A, typename B>
+__device__ __forceinline__ cpx<A> cvt(cpx<B> v) { return mk<A>((A)v.x, (A)v.y); }
+
+// ---------------------------------------------------------------------------
+// x pass helpers: staging of an (Nx x CB) column tile through LDS.
+// ---------------------------------------------------------------------------
+template <int L, int CB, typename S, typename G>
+__device__ __forceinline__ void stage_in(cpx<S> *lds, const cpx<G> *__restrict__ f, const int *cols, int ncol_total)
+{
+    constexpr int TPL = L / 8;
+    constexpr int LP = lpad_len(L);
+    for (int e = threadIdx.x; e < L * CB; e += CB * TPL) {
+        const int r = e / CB, c = e % CB, col = cols[c];
+        cpx<S> v = mk<S>(0, 0);
+        if (col >= 0) v = cvt<S>(f[(size_t)r * ncol_total + col]);
+        lds[c * LP + lpad(r)] = v;
+    }
+}
+
+template <int L, int CB, typename S, typename G>
+__device__ __forceinline__ void stage_out(const cpx<S> *lds, cpx<G> *__restrict__ f, const int *cols, int ncol_total)
+{
+    constexpr int TPL = L / 8;
+    constexpr int LP = lpad_len(L);
+    for (int e = threadIdx.x; e < L * CB; e += CB * TPL) {
+        const int r = e / CB, c = e % CB, col = cols[c];
+        if (col >= 0) f[(size_t)r * ncol_total + col] = cvt<G>(lds[c * LP + lpad(r)]);
+    }
+}
+
+// Tiles of the x pass.  Work fields store each ky row of kz with the padded
+// stride nzp (a multiple of 128 bytes); the FP64 state in the boundary layout
+// uses nzc = Nz/2+1.  A tile is CB consecutive columns of the padded (ly,lz)
+// space (pad columns lz >= nzc are skipped), or -- patch tiles -- the
+// kz = 0 and kz = Nz/2 columns of every ky.  colw = ly*nzp + lz indexes the
+// work fields, cols = ly*nzc + lz the state.
+//
+// Blocks are dispatched to the 8 XCDs round-robin; xcd_tile() hands each XCD
+// a contiguous range of tiles so that neighbouring tiles (which share the
+// 128-byte lines of the unpadded state rows) run on the same L2.
+__device__ __forceinline__ int xcd_tile(int b, int nb)
+{
+    const int per = nb / 8;
+    return (b % 8) * per + b / 8;
+}
+
+__host__ __device__ inline int x_tiles(const Grid &g, int CB, bool patch)
+{
+    return patch ? (2 * g.ny + CB - 1) / CB : (g.ny * g.nzp + CB - 1) / CB;
+}
+
+__host__ inline int x_grid(int ntiles) { return 8 * ((ntiles + 7) / 8); }
+
+template <int CB>
+__device__ __forceinline__ void tile_columns(int *colw, int *cols, const Grid &g, bool patch, int tile)
+{
+    if (threadIdx.x < CB) {
+        const int c = threadIdx.x;
+        int w = -1, st = -1;
+        if (!patch) {
+            const int cw = tile * CB + c;
+            const int ly = cw / g.nzp, lz = cw % g.nzp;
+            if (cw < g.ny * g.nzp && lz < g.nzc) { w = cw; st = ly * g.nzc + lz; }
+        } else {
+            const int p = tile * CB + c;
+            if (p < 2 * g.ny) {
+                const int ly = p >> 1, lz = (p & 1) ? g.nz / 2 : 0;
+                w = ly * g.nzp + lz;
+                st = ly * g.nzc + lz;
+            }
+        }
+        colw[c] = w;
+        cols[c] = st;
+    }
+}
+
+__device__ __forceinline__ int band_slot(const Band &b, int kx, int ky, int kz)
+{
+    if (kx < -b.kb || kx > b.kb || ky < -b.kb || ky > b.kb || kz > b.kb) return -1;
+    return ((kx + b.kb) * (2 * b.kb + 1) + (ky + b.kb)) * (b.kb + 1) + kz;
+}
+
+__device__ __forceinline__ double kmag(const Grid &g, int kx, int ky, int kz)
+{
+    const double a = kx * g.kf[0], b = ky * g.kf[1], c = kz * g.kf[2];
+    return sqrt(a * a + b * b + c * c);
+}
+
+// ---------------------------------------------------------------------------
+// z-line transforms.  A real line x[0..N) (N = 2M) is packed as
+// z[n] = x[2n] + i x[2n+1]; a complex line holds X[0..M].
+// c2r: Z'[k] = (X[k] + conj X[M-k]) + i W^-k (X[k] - conj X[M-k]), with the
+//      imaginary parts of X[0], X[M] dropped; z = IDFT_M(Z') (unscaled), so
+//      x = sum_k X_herm[k] e^{+2 pi i k n / N}.
+// r2c: Z = DFT_M(z); X[k] = ((Z[k] + conj Z[M-k]) - i W^k (Z[k] - conj Z[M-k]))/2.
+// W = exp(-2 pi i / N); twn holds W^k for k = 0..M.
+// ---------------------------------------------------------------------------
+template <typename T, int M>
+__device__ __forceinline__ void zline_c2r(cpx<T> *v, const cpx<T> *__restrict__ X, cpx<T> *lds,
+                                          const cpx<T> *__restrict__ twm, const cpx<T> *__restrict__ twn, int j)
+{
+    constexpr int TPL = M / 8;
+    // stage the line (M+1 points) in LDS
+#pragma unroll
+    for (int s = 0; s < 8; s++) lds[lpad(j + TPL * s)] = X[j + TPL * s];
+    if (j == 0) lds[lpad(M)] = X[M];
+    line_sync<M / 8>();
+#pragma unroll
+    for (int s = 0; s < 8; s++) {
+        const int k = j + TPL * s;
+        cpx<T> xk = lds[lpad(k)];
+        cpx<T> xm = lds[lpad(M - k)];
+        if (k == 0) { xk.y = 0; xm.y = 0; }
+        const cpx<T> a = xk + conj(xm);
+        const cpx<T> d = xk - conj(xm);
+        const cpx<T> w = twn[k];  // W^k ; we need W^-k = conj(w)
+        const cpx<T> e = mk<T>(d.x * w.x + d.y * w.y, d.y * w.x - d.x * w.y);
+        v[s] = a + mk<T>(-e.y, e.x);
+    }
+    line_sync<M / 8>();
+    fft_line<M, +1>(v, lds, twm, j);
+}
+
+template <typename T, int M>
+__device__ __forceinline__ void zline_r2c_post(cpx<T> *v, cpx<T> *__restrict__ X, cpx<T> *lds,
+                                               const cpx<T> *__restrict__ twm, const cpx<T> *__restrict__ twn, int j,
+                                               bool store)
+{
+    constexpr int TPL = M / 8;
+    fft_line<M, -1>(v, lds, twm, j);
+#pragma unroll
+    for (int s = 0; s < 8; s++) lds[lpad(j + TPL * s)] = v[s];
+    line_sync<M / 8>();
+    const T h = (T)0.5;
+#pragma unroll
+    for (int s = 0; s < 8; s++) {
+        const int k = j + TPL * s;
+        const cpx<T> zk = v[s];
+        const cpx<T> zm = lds[lpad((M - k) & (M - 1))];
+        const cpx<T> a = zk + conj(zm);
+        const cpx<T> d = zk - conj(zm);
+        const cpx<T> w = twn[k];
+        const cpx<T> e = d * w;  // W^k d
+        if (store) X[k] = mk<T>(h * (a.x + e.y), h * (a.y - e.x));
+    }
+    if (store && j == 0) X[M] = mk<T>(v[0].x - v[0].y, (T)0);
+    line_sync<M / 8>();
+}
+
+struct ZArgs {
+    void *in[6];
+    void *out[9];
+    void *tape[3];  // optional: real-space velocity lines (Nx*Ny lines of Nz/2 packed points)
+    const void *twm, *twn;
+    Grid g;
+    int ncomp;
+};
+
+template <int M>
+__host__ __device__ constexpr int z_lines_per_block()
+{
+    return (M / 8) >= 256 ? 1 : 256 / (M / 8);
+}
+
+// ---------------------------------------------------------------------------
+// Size dispatch of the launchers (transform lengths are compile-time).
+// ---------------------------------------------------------------------------
+#define TG_DISPATCH_L(Lval, CALL) \
+    switch (Lval) {               \
+        case 8: { constexpr int LL = 8; CALL; } break;     \
+        case 16: { constexpr int LL = 16; CALL; } break;   \
+        case 32: { constexpr int LL = 32; CALL; } break;   \
+        case 64: { constexpr int LL = 64; CALL; } break;   \
+        case 128: { constexpr int LL = 128; CALL; } break; \
+        case 256: { constexpr int LL = 256; CALL; } break; \
+        case 512: { constexpr int LL = 512; CALL; } break; \
+        case 1024: { constexpr int LL = 1024; CALL; } break; \
+        default: return hipErrorInvalidValue;              \
+    }
+
+#define TG_DISPATCH_M(Mval, CALL) \
+    switch (Mval) {               \
+        case 8: { constexpr int MM = 8; CALL; } break;     \
+        case 16: { constexpr int MM = 16; CALL; } break;   \
+        case 32: { constexpr int MM = 32; CALL; } break;   \
+        case 64: { constexpr int MM = 64; CALL; } break;   \
+        case 128: { constexpr int MM = 128; CALL; } break; \
+        case 256: { constexpr int MM = 256; CALL; } break; \
+        case 512: { constexpr int MM = 512; CALL; } break; \
+        default: return hipErrorInvalidValue;              \
+    }
+
+}  // namespace tg

@@ -58,7 +58,12 @@ def test_null_arguments_are_errors():
     assert L.tg_forward(None, 0, None, None) == -1
     assert L.tg_rk4_step(None, None, None, 1, 0) == -1
     assert L.tg_plan_destroy(None) == 0
+    assert L.tg_rhs_vjp(None, None, None, None) == -1
+    assert L.tg_adjoint_forward(None, None, None, 1, 1) == -1
+    assert L.tg_adjoint_backward(None, None) == -1
+    assert L.tg_adjoint_free(None) == -1
     assert L.tg_kernel_name(3) == b"x_nlin_rk"
+    assert L.tg_kernel_name(11) == b"z_adj"
     assert L.tg_version().startswith(b"tarang_amd")
 
 
