@@ -48,6 +48,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-extra", action="store_true", help="headline FP64 number only")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint cost measurement")
+    ap.add_argument("--adjoint-only", action="store_true", help="run only the adjoint cost measurement")
     ap.add_argument("--copy-calib", action="store_true", help="run a plain 1 GiB device copy (counter calibration)")
     return ap.parse_args()
 
@@ -98,6 +99,10 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         print(json.dumps({"copy_calib_bytes": a.numel() * 8, "launches": 3}))
+        return
+
+    if args.adjoint_only:
+        print(json.dumps({"adjoint": adjoint_cost(api, torch, np)}))
         return
 
     N = (args.n, args.n, args.n)
@@ -151,7 +156,7 @@ def main():
         "kernel": dom, "bound": "hbm", "achieved": round(dom_achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
         "frac": round(dom_achieved / HBM_PEAK_GBS, 4),
         "bytes_per_launch": int(dom_bytes_step / dom_launch), "avg_launch_ms": round(dom_ms_step / dom_launch, 4),
-        "traffic": None,
+        "traffic": measured_traffic(dom, N, "fp64"),
     }
     per_kernel = {k: {"ms_per_step": round(v[0], 3), "launches_per_step": v[1],
                       "GBps": round(algorithmic_bytes(k, N, 8) / (v[0] * 1e-3) / 1e9, 1)
@@ -235,6 +240,22 @@ def main():
     print(json.dumps(out))
 
 
+def measured_traffic(kernel_class, N, precision):
+    """HBM bytes per launch of a kernel class from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE passes (scaled by the copy calibration), or None
+    when no pass of this kernel, grid and precision is on record."""
+    path = os.path.join(HERE, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f)
+    except (OSError, ValueError):
+        return None
+    for r in rec.values():
+        if r.get("kernel_class") == kernel_class and r.get("grid") == list(N) and r.get("precision") == precision:
+            return r["traffic_bytes_per_launch"]
+    return None
+
+
 def adjoint_cost(api, torch, np, n=256, nsteps=200, every=20, nu=1e-3, dt=1e-3, r_lo=8.0, r_hi=16.0):
     N = (n, n, n)
     b = api.FFF_Basis(N, precision=api.PREC_FP64, dealias=api.DEALIAS_REF_CORNER, strong_reality=False)
@@ -268,6 +289,23 @@ def adjoint_cost(api, torch, np, n=256, nsteps=200, every=20, nu=1e-3, dt=1e-3, 
     t_adj = time.perf_counter() - t0
     loss = float(sum((m * (v.real ** 2 + v.imag ** 2)).sum().item() for v in U.cvf))
     gnorm = float(sum((x.real ** 2 + x.imag ** 2).sum().item() for x in lam)) ** 0.5
+    # per-kernel HIP-event times of one segment (forward with tape + backward), per step
+    import ctypes
+    from tarang_amd._lib import KERNELS, check
+    L = b.L
+    U.set_field(V0)
+    check("tg_profile_reset", L.tg_profile_reset(b.plan))
+    check("tg_profile_enable", L.tg_profile_enable(b.plan, 1))
+    adj.Forward(every)
+    lam = [2.0 * m * v for v in U.cvf]
+    adj.Backward(lam)
+    check("tg_profile_enable", L.tg_profile_enable(b.plan, 0))
+    kern = {}
+    for k, name in enumerate(KERNELS):
+        ms, cnt = ctypes.c_double(), ctypes.c_int64()
+        check("tg_profile_read", L.tg_profile_read(b.plan, k, ctypes.byref(ms), ctypes.byref(cnt)))
+        if cnt.value:
+            kern[name] = {"ms_per_step": round(ms.value / every, 3), "launches_per_step": cnt.value // every}
     nc = n * n * (n // 2 + 1)
     tape_bytes = (nsteps + every - 1) // every * 3 * nc * 16 + every * 12 * n * n * (n // 2) * 16
     adj.Free()
@@ -277,7 +315,7 @@ def adjoint_cost(api, torch, np, n=256, nsteps=200, every=20, nu=1e-3, dt=1e-3, 
                         f"loss = E in {r_lo:g} <= |k| < {r_hi:g}",
             "forward_s": round(t_fwd, 4), "forward_and_backward_s": round(t_adj, 4),
             "cost_ratio": round(t_adj / t_fwd, 3), "target_ratio": 3.0, "loss": loss, "grad_norm": gnorm,
-            "checkpoint_and_tape_GB": round(tape_bytes / 1e9, 2)}
+            "checkpoint_and_tape_GB": round(tape_bytes / 1e9, 2), "segment_kernels": kern}
 
 
 def cpu_baseline(N, nu, dt, force, V_dev):

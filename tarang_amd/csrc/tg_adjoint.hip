@@ -132,14 +132,15 @@ __global__ void __launch_bounds__(256) k_z_adj(ZArgs a)
         cpx<T> q[8], v[8];
 #pragma unroll
         for (int s = 0; s < 8; s++) q[s] = mk<T>(0, 0);
-#pragma unroll
+#pragma unroll 1
         for (int jj = 0; jj < 3; jj++) {
             const int t = m == 0 ? sig[0][jj] : (m == 1 ? sig[1][jj] : sig[2][jj]);
             const cpx<T> *X = (const cpx<T> *)(t == 0 ? a.in[0] : t == 1 ? a.in[1] : t == 2 ? a.in[2] :
                                                 t == 3 ? a.in[3] : t == 4 ? a.in[4] : a.in[5]);
             zline_c2r<T, M>(v, X + off, L, twm, twn, j);
             line_sync<M / 8>();
-            tape_mac<T, M>(q, v, (const cpx<T> *)a.tape[jj] + lsafe * M, j);
+            const cpx<T> *u = (const cpx<T> *)(jj == 0 ? a.tape[0] : jj == 1 ? a.tape[1] : a.tape[2]);
+            tape_mac<T, M>(q, v, u + lsafe * M, j);
         }
         cpx<T> *Q = (cpx<T> *)(m == 0 ? a.out[0] : m == 1 ? a.out[1] : a.out[2]);
         zline_r2c_post<T, M>(q, Q + off, L, twm, twn, j, valid);

@@ -145,23 +145,6 @@ __device__ __forceinline__ void line_sync()
     }
 }
 
-// The same pointer, hidden from the optimiser: loads through the result are
-// not merged with loads through earlier copies, so tables read by several
-// line transforms of one kernel are re-read (cache hits) where they are used
-// instead of being held in registers across the whole kernel.
-template <typename P>
-__device__ __forceinline__ P *opaque_uniform(P *p)
-{
-    asm volatile("" : "+s"(p));
-    return p;
-}
-template <typename P>
-__device__ __forceinline__ P *opaque_lane(P *p)
-{
-    asm volatile("" : "+v"(p));
-    return p;
-}
-
 // LDS position with one pad entry every 16 points (breaks the stride-8
 // write pattern of the first radix-8 pass across banks).
 __device__ __forceinline__ int lpad(int i) { return i + (i >> 4); }
