@@ -35,6 +35,7 @@
  * vector from a running reference pins any of these.
  */
 #include "tg_oracle.h"
+#include "tg_oracle_int.h"
 
 #include <complex.h>
 #include <math.h>
@@ -161,51 +162,54 @@ static inline size_t cidx(const or_grid *g, int lx, int ly, int lz)
     return ((size_t)lx * g->ny + ly) * (size_t)(g->nz / 2 + 1) + lz;
 }
 
-/* complex-to-complex transforms along x and y of a (Nx,Ny,Nzc) array */
-static void fft_xy(const or_grid *g, cplx *A, int sign)
+/* complex-to-complex transform along y of a (Nx,Ny,Nzc) array, unnormalised */
+void or_int_fft_y(const or_grid *g, cplx *A, int sign)
 {
     const int nx = g->nx, ny = g->ny, nzc = g->nz / 2 + 1;
-    if (ny > 1) {
+    if (ny <= 1) return;
 #pragma omp parallel
-        {
-            fftplan p;
-            plan_init(&p, ny);
-            cplx *line = (cplx *)malloc(sizeof(cplx) * (size_t)ny);
-            cplx *work = (cplx *)malloc(sizeof(cplx) * (size_t)ny);
+    {
+        fftplan p;
+        plan_init(&p, ny);
+        cplx *line = (cplx *)malloc(sizeof(cplx) * (size_t)ny);
+        cplx *work = (cplx *)malloc(sizeof(cplx) * (size_t)ny);
 #pragma omp for collapse(2) schedule(static)
-            for (int lx = 0; lx < nx; lx++)
-                for (int lz = 0; lz < nzc; lz++) {
-                    for (int ly = 0; ly < ny; ly++) line[ly] = A[cidx(g, lx, ly, lz)];
-                    fft_exec(&p, line, work, sign);
-                    for (int ly = 0; ly < ny; ly++) A[cidx(g, lx, ly, lz)] = line[ly];
-                }
-            free(line); free(work); plan_free(&p);
-        }
-    }
-    if (nx > 1) {
-#pragma omp parallel
-        {
-            fftplan p;
-            plan_init(&p, nx);
-            cplx *line = (cplx *)malloc(sizeof(cplx) * (size_t)nx);
-            cplx *work = (cplx *)malloc(sizeof(cplx) * (size_t)nx);
-#pragma omp for collapse(2) schedule(static)
-            for (int ly = 0; ly < ny; ly++)
-                for (int lz = 0; lz < nzc; lz++) {
-                    for (int lx = 0; lx < nx; lx++) line[lx] = A[cidx(g, lx, ly, lz)];
-                    fft_exec(&p, line, work, sign);
-                    for (int lx = 0; lx < nx; lx++) A[cidx(g, lx, ly, lz)] = line[lx];
-                }
-            free(line); free(work); plan_free(&p);
-        }
+        for (int lx = 0; lx < nx; lx++)
+            for (int lz = 0; lz < nzc; lz++) {
+                for (int ly = 0; ly < ny; ly++) line[ly] = A[cidx(g, lx, ly, lz)];
+                fft_exec(&p, line, work, sign);
+                for (int ly = 0; ly < ny; ly++) A[cidx(g, lx, ly, lz)] = line[ly];
+            }
+        free(line); free(work); plan_free(&p);
     }
 }
 
-/* Forward_transform (FFF_pencil_tr.cc:43-51): r2c along z, then y, x; 1/N^3. */
-void or_forward(const or_grid *g, const double *real_in, double *spec_out)
+/* complex-to-complex transform along x, unnormalised */
+static void fft_x(const or_grid *g, cplx *A, int sign)
+{
+    const int nx = g->nx, ny = g->ny, nzc = g->nz / 2 + 1;
+    if (nx <= 1) return;
+#pragma omp parallel
+    {
+        fftplan p;
+        plan_init(&p, nx);
+        cplx *line = (cplx *)malloc(sizeof(cplx) * (size_t)nx);
+        cplx *work = (cplx *)malloc(sizeof(cplx) * (size_t)nx);
+#pragma omp for collapse(2) schedule(static)
+        for (int ly = 0; ly < ny; ly++)
+            for (int lz = 0; lz < nzc; lz++) {
+                for (int lx = 0; lx < nx; lx++) line[lx] = A[cidx(g, lx, ly, lz)];
+                fft_exec(&p, line, work, sign);
+                for (int lx = 0; lx < nx; lx++) A[cidx(g, lx, ly, lz)] = line[lx];
+            }
+        free(line); free(work); plan_free(&p);
+    }
+}
+
+/* r2c along z of every (x, y) line of a real (Nx,Ny,Nz+2) array, unnormalised */
+void or_int_z_r2c(const or_grid *g, const double *real_in, cplx *A)
 {
     const int nx = g->nx, ny = g->ny, nz = g->nz, nzc = nz / 2 + 1;
-    cplx *A = (cplx *)spec_out;
 #pragma omp parallel
     {
         fftplan p;
@@ -222,22 +226,12 @@ void or_forward(const or_grid *g, const double *real_in, double *spec_out)
             }
         free(line); free(work); plan_free(&p);
     }
-    fft_xy(g, A, -1);
-    const double s = 1.0 / ((double)nx * (double)ny * (double)nz);
-    const size_t tot = (size_t)nx * ny * nzc;
-#pragma omp parallel for schedule(static)
-    for (size_t i = 0; i < tot; i++) A[i] *= s;
 }
 
-/* Inverse_transform (FFF_pencil_tr.cc:57-66): input preserved (the reference
- * copies to X_transform first, :59); x, y inverse, then c2r along z. */
-void or_inverse(const or_grid *g, const double *spec_in, double *real_out)
+/* c2r along z of every line (imaginary parts of kz = 0 and Nz/2 ignored), unnormalised */
+void or_int_z_c2r(const or_grid *g, const cplx *A, double *real_out)
 {
-    const int nx = g->nx, ny = g->ny, nz = g->nz, nzc = nz / 2 + 1;
-    const size_t tot = (size_t)nx * ny * nzc;
-    cplx *A = (cplx *)malloc(sizeof(cplx) * tot);
-    memcpy(A, spec_in, sizeof(cplx) * tot);
-    fft_xy(g, A, +1);
+    const int nx = g->nx, ny = g->ny, nz = g->nz;
 #pragma omp parallel
     {
         fftplan p;
@@ -262,6 +256,32 @@ void or_inverse(const or_grid *g, const double *spec_in, double *real_out)
             }
         free(line); free(work); plan_free(&p);
     }
+}
+
+/* Forward_transform (FFF_pencil_tr.cc:43-51): r2c along z, then y, x; 1/N^3. */
+void or_forward(const or_grid *g, const double *real_in, double *spec_out)
+{
+    const int nx = g->nx, ny = g->ny, nz = g->nz, nzc = nz / 2 + 1;
+    cplx *A = (cplx *)spec_out;
+    or_int_z_r2c(g, real_in, A);
+    or_int_fft_y(g, A, -1);
+    fft_x(g, A, -1);
+    const double s = 1.0 / ((double)nx * (double)ny * (double)nz);
+    const size_t tot = (size_t)nx * ny * nzc;
+#pragma omp parallel for schedule(static)
+    for (size_t i = 0; i < tot; i++) A[i] *= s;
+}
+
+/* Inverse_transform (FFF_pencil_tr.cc:57-66): input preserved (the reference
+ * copies to X_transform first, :59); x, y inverse, then c2r along z. */
+void or_inverse(const or_grid *g, const double *spec_in, double *real_out)
+{
+    const size_t tot = (size_t)g->nx * g->ny * (g->nz / 2 + 1);
+    cplx *A = (cplx *)malloc(sizeof(cplx) * tot);
+    memcpy(A, spec_in, sizeof(cplx) * tot);
+    fft_x(g, A, +1);
+    or_int_fft_y(g, A, +1);
+    or_int_z_c2r(g, A, real_out);
     free(A);
 }
 

@@ -81,6 +81,24 @@ double or_cross_energy(const or_grid *g, const double *A, const double *B);
 void or_flux_mhd(const or_grid *g, double *V[3], double *W[3], const double *radii, int nspheres, double *u2b,
                  double *b2b);
 
+/* ---- SFF basis and Rayleigh-Benard convection (tg_oracle_sff.c) ---- */
+enum { OR_SIN = 0, OR_COS = 1 };
+typedef struct or_rbc {
+    double Ra, Pr;       /* PRLARGE / ULARGE: nu = sqrt(Pr/Ra), kappa = 1/sqrt(Pr Ra) */
+    double dt;
+    int reality;         /* after each RK4 step: 0 none, 1 strong, 2 weak */
+} or_rbc;
+void or_sff_forward(const or_grid *g, int parity, const double *real_in, double *spec_out);
+void or_sff_inverse(const or_grid *g, int parity, const double *spec_in, double *real_out);
+void or_sff_dealias(const or_grid *g, double *spec);
+void or_sff_reality(const or_grid *g, double *spec, int mode);
+double or_sff_energy(const or_grid *g, const double *A, const double *B);
+double or_sff_kmag(const or_grid *g, int lx, int ly, int lz);
+void or_compute_nlin_rbc(const or_grid *g, double *V[3], double *T, double *N[3], double *NT);
+void or_compute_rhs_rbc(const or_grid *g, const double *Fu1, const double *FT, double *N[3], double *NT);
+void or_rk4_step_rbc(const or_grid *g, const or_rbc *p, double *V[3], double *T);
+double or_nusselt(const or_grid *g, const or_rbc *p, const double *V1, const double *T);
+
 /* ---- tangent / adjoint (no counterpart in the reference) ---- */
 void or_rhs_jvp(const or_grid *g, double *const V[3], double *const dV[3], double *out[3]);
 void or_rhs_vjp(const or_grid *g, double *const V[3], double *const Lam[3], double *out[3]);

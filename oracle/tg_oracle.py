@@ -32,6 +32,15 @@ class Fluid(ctypes.Structure):
                 ("force_r_out", ctypes.c_double)]
 
 
+class Rbc(ctypes.Structure):
+    """PRLARGE / ULARGE Rayleigh-Benard parameters; reality 0 none, 1 strong, 2 weak."""
+    _fields_ = [("Ra", ctypes.c_double), ("Pr", ctypes.c_double), ("dt", ctypes.c_double),
+                ("reality", ctypes.c_int)]
+
+
+SIN, COS = 0, 1
+
+
 def build():
     subprocess.check_call(["make", "-s", "-C", _HERE])
 
@@ -86,6 +95,16 @@ def lib():
             "or_rhs_vjp": (None, [gp, pp, pp, pp]),
             "or_rk4_step_vjp": (None, [gp, fp, pp, pp]),
             "or_adjoint_shell_energy": (d, [gp, fp, pp, i, d, d, pp]),
+            "or_sff_forward": (None, [gp, i, p, p]),
+            "or_sff_inverse": (None, [gp, i, p, p]),
+            "or_sff_dealias": (None, [gp, p]),
+            "or_sff_reality": (None, [gp, p, i]),
+            "or_sff_energy": (d, [gp, p, p]),
+            "or_sff_kmag": (d, [gp, i, i, i]),
+            "or_compute_nlin_rbc": (None, [gp, pp, p, pp, p]),
+            "or_compute_rhs_rbc": (None, [gp, p, p, pp, p]),
+            "or_rk4_step_rbc": (None, [gp, ctypes.POINTER(Rbc), pp, p]),
+            "or_nusselt": (d, [gp, ctypes.POINTER(Rbc), p, p]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(_lib, name)
@@ -270,6 +289,46 @@ class Oracle:
         loss = self.L.or_adjoint_shell_energy(ctypes.byref(self.g), ctypes.byref(fluid), _pp(V0), nsteps, r_lo, r_hi,
                                               _pp(grad))
         return loss, grad
+
+    # ---- SFF basis / Rayleigh-Benard ----
+    def sff_forward(self, parity, real):
+        real = np.ascontiguousarray(real, np.float64)
+        out = self.czeros()
+        self.L.or_sff_forward(ctypes.byref(self.g), parity, _ptr(real), _ptr(out))
+        return out
+
+    def sff_inverse(self, parity, spec):
+        spec = np.ascontiguousarray(spec, np.complex128)
+        out = np.zeros(self.rshape, np.float64)
+        self.L.or_sff_inverse(ctypes.byref(self.g), parity, _cptr(spec), _ptr(out))
+        return out
+
+    def sff_dealias(self, A):
+        self.L.or_sff_dealias(ctypes.byref(self.g), _cptr(A))
+
+    def sff_reality(self, A, mode):
+        self.L.or_sff_reality(ctypes.byref(self.g), _cptr(A), mode)
+
+    def sff_energy(self, A, B=None):
+        return self.L.or_sff_energy(ctypes.byref(self.g), _cptr(A), _cptr(A if B is None else B))
+
+    def sff_kmag(self, lx, ly, lz):
+        return self.L.or_sff_kmag(ctypes.byref(self.g), lx, ly, lz)
+
+    def compute_nlin_rbc(self, V, T):
+        N = [self.czeros() for _ in range(3)]
+        NT = self.czeros()
+        self.L.or_compute_nlin_rbc(ctypes.byref(self.g), _pp(V), _cptr(T), _pp(N), _cptr(NT))
+        return N, NT
+
+    def compute_rhs_rbc(self, Fu1, FT, N, NT):
+        self.L.or_compute_rhs_rbc(ctypes.byref(self.g), _cptr(Fu1), _cptr(FT), _pp(N), _cptr(NT))
+
+    def rk4_step_rbc(self, prm, V, T):
+        self.L.or_rk4_step_rbc(ctypes.byref(self.g), ctypes.byref(prm), _pp(V), _cptr(T))
+
+    def nusselt(self, prm, V1, T):
+        return self.L.or_nusselt(ctypes.byref(self.g), ctypes.byref(prm), _cptr(V1), _cptr(T))
 
     def init_field(self, kind=0, k0=4.0, kmin=2.0, kmax=170.0, energy=0.5, seed=1):
         V = [self.czeros() for _ in range(3)]
