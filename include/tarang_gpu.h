@@ -38,7 +38,14 @@ extern "C" {
 #define TG_ERR_NAN (-5)
 #define TG_ERR_STATE (-6)
 
-enum { TG_BASIS_FFF = 0 };
+/* Bases: FFF (Fourier x, y, z) and SFF (sin/cos along x, Fourier y, z;
+ * SFF_pencil.cc).  For SFF, kfactor[0] = pi / L_x and strong_reality selects
+ * the reality condition applied after each RK4 step: 0 none, 1 strong, 2 weak
+ * (apply_strong/weak_realitycond_alltime_switch, Time_advance.cc:85-93). */
+enum { TG_BASIS_FFF = 0, TG_BASIS_SFF = 1 };
+/* sincos_code of tg_forward / tg_inverse (global.program.sincostr_switch[0]):
+ * TG_SC_FOURIER on FFF plans; TG_SC_SIN ('S') or TG_SC_COS ('C') on SFF plans. */
+enum { TG_SC_FOURIER = 0, TG_SC_SIN = 1, TG_SC_COS = 2 };
 enum { TG_PREC_FP64 = 0, TG_PREC_FP32 = 1 };
 /* Dealias modes: reference corner box (FFF_pencil_basic.cc:87-90) or the
  * two-thirds rule |kx|<=Nx/3, |ky|<=Ny/3, kz<=Nz/3. */
@@ -46,7 +53,7 @@ enum { TG_DEALIAS_NONE = 0, TG_DEALIAS_REF_CORNER = 1, TG_DEALIAS_TWOTHIRDS = 2 
 enum { TG_STEP_CONTINUE = 1 };
 
 typedef struct tg_config {
-    int basis;          /* TG_BASIS_FFF */
+    int basis;          /* TG_BASIS_FFF or TG_BASIS_SFF */
     int nx, ny, nz;     /* grid; powers of two, nx,ny in [8,1024], nz in [16,1024] */
     double kfactor[3];  /* 2 pi / L per axis (global.field.kfactor[1..3]) */
     int precision;      /* TG_PREC_FP64 or TG_PREC_FP32 (transform/product precision) */
@@ -64,6 +71,15 @@ typedef struct tg_fluid {
     double force_r_in;  /* forcing band (r_in, r_out] */
     double force_r_out;
 } tg_fluid;
+
+/* Rayleigh-Benard parameters, Pr_option PRLARGE with Uscaling ULARGE
+ * (Iscalar_main.cc:66-68: nu = sqrt(Pr/Ra), kappa = 1/sqrt(Pr Ra);
+ * compute_force_RBC.cc:95-125: F_u = (theta, 0, 0), F_theta = u_x). */
+typedef struct tg_rbc {
+    double Ra;
+    double Pr;
+    double dt;
+} tg_rbc;
 
 typedef struct tg_plan tg_plan;
 
@@ -83,17 +99,26 @@ int tg_get_shapes(const tg_plan *plan, int64_t complex_shape[3], int64_t real_sh
 int64_t tg_plan_device_bytes(const tg_plan *plan);
 
 /* FFF_PENCIL::Forward_transform -> fftk.Forward_transform_3d("FFF", Ar, A)
- * (FFF_pencil_tr.cc:43-51): r2c, scaled by 1/(Nx Ny Nz).  sincos_code is
- * accepted for signature parity with the basis interface and must be 0. */
+ * (FFF_pencil_tr.cc:43-51): r2c, scaled by 1/(Nx Ny Nz); sincos_code
+ * TG_SC_FOURIER.  SFF_PENCIL::Forward_transform (SFF_pencil_tr.cc:43-52):
+ * sin ('S') or cos ('C') transform along x on the cell-centred samples
+ * x_j = (j + 1/2) L_x / Nx, A_m = (1/Nx) sum_j f_j phi_m(x_j) (the sine mode
+ * m = Nx dropped, A_0 = 0 for 'S'), r2c along z and FFT along y scaled
+ * 1/(Ny Nz). */
 int tg_forward(tg_plan *plan, int sincos_code, const double *real_dev, double *spec_dev);
 /* FFF_PENCIL::Inverse_transform (FFF_pencil_tr.cc:57-66): c2r, unscaled,
- * input preserved. */
+ * input preserved.  SFF (SFF_pencil_tr.cc:58-67): f_j = sum_m w_m A_m
+ * phi_m(x_j), w_0 = 1, w_m = 2. */
 int tg_inverse(tg_plan *plan, int sincos_code, const double *spec_dev, double *real_dev);
 
 /* FFF_PENCIL::Dealias (FFF_pencil_basic.cc:87-90) in place, per the plan's mode. */
 int tg_dealias(tg_plan *plan, double *const V_dev[], int ncomp);
-/* Satisfy_strong_reality_condition_in_Array (FFF_pencil_basic.cc:139-179). */
+/* Satisfy_strong_reality_condition_in_Array (FFF_pencil_basic.cc:139-179;
+ * SFF: SFF_pencil_basic.cc:157-182). */
 int tg_strong_reality(tg_plan *plan, double *const V_dev[], int ncomp);
+/* Satisfy_weak_reality_condition_in_Array (FFF_pencil_basic.cc:181-186,
+ * SFF_pencil_basic.cc:185-189): the kz = Nz/2 plane is zeroed. */
+int tg_weak_reality(tg_plan *plan, double *const V_dev[], int ncomp);
 
 /* Nlin_incompress::Compute_nlin(FluidVF&) (compute_nlin.cc:58-109):
  * dealiases V in place, nlin_i = FT[d_j(u_j u_i)]. */
@@ -130,8 +155,28 @@ int tg_cross_helicity(tg_plan *plan, double *const V_dev[3], double *const W_dev
 int tg_flux_mhd(tg_plan *plan, double *const V_dev[3], double *const W_dev[3], const double *radii, int nspheres,
                 double *u2b, double *b2b);
 
+/* Rayleigh-Benard convection on an SFF plan (state u_x 'S', u_y, u_z 'C',
+ * theta 'S'; parities from basis_table, Global.cc:138-146).
+ * Nlin_incompress::Compute_nlin(FluidVF& U, FluidSF& T) ->
+ * Compute_nlin_scalar (compute_nlin_SF.cc:75-142): dealiases V and T in
+ * place; nlin_i = FT[d_j(u_j u_i)], nlinT = FT[d_j(u_j theta)]. */
+int tg_compute_nlin_rbc(tg_plan *plan, double *const V_dev[3], double *T_dev, double *const nlin_dev[3],
+                        double *nlinT_dev);
+/* Compute_force_TO_rhs_RBC (Tadvance_functions.cc:477-489): the force from
+ * V and T as given, then the nonlinear terms (V, T dealiased in place),
+ * pressure and signs: rhs = -(N - F + grad p), rhsT = -(N_T - u_x). */
+int tg_compute_rhs_rbc(tg_plan *plan, double *const V_dev[3], double *T_dev, const tg_rbc *rbc,
+                       double *const rhs_dev[3], double *rhsT_dev);
+/* Time_advance_incompress::RK4(U, T, P, F) (RK_SF.cc:97-215) followed by the
+ * plan's reality condition: nsteps Lawson RK4 steps of (V, T) in place. */
+int tg_rk4_step_rbc(tg_plan *plan, double *const V_dev[3], double *T_dev, const tg_rbc *rbc, int nsteps);
+/* Correlation::Get_Nusselt_no (Correlation.cc:144-160), PRLARGE / ULARGE:
+ * Nu = 1 + 2 sqrt(Ra Pr) E(V1, T) with the SFF pair energy. */
+int tg_nusselt(tg_plan *plan, const double *V1_dev, const double *T_dev, const tg_rbc *rbc, double *nu);
+
 /* Universal::Get_total_energy (universal_energy.cc:63-71) of one component:
- * sum |A|^2 - sum_{kz=0} |A|^2 / 2. */
+ * FFF sum |A|^2 - sum_{kz=0} |A|^2 / 2; SFF (SFF_pencil_energy.cc:88-104)
+ * 2 sum |A|^2 - sum_{kz=0} - sum_{kx=0} + sum_{kx=0,kz=0} / 2. */
 int tg_energy(tg_plan *plan, const double *A_dev, double *energy);
 /* Universal::Get_total_Sn (universal_energy.cc:106-141): sum m K^n |A|^2. */
 int tg_Sn(tg_plan *plan, const double *A_dev, int n, double *Sn);
@@ -162,7 +207,10 @@ enum {
     TG_KERNEL_X_INV_SYM = 10,   /* adjoint: x-inverse of the symmetrised cotangent */
     TG_KERNEL_Z_ADJ = 11,       /* adjoint: z c2r + products with the velocity tape + r2c */
     TG_KERNEL_X_FWD_ADJ = 12,   /* adjoint: x forward + RK4 stage transpose */
-    TG_KERNEL_COUNT = 13
+    TG_KERNEL_X_SC = 13,        /* SFF sin/cos x passes (transforms, stage-1 input, patches) */
+    TG_KERNEL_Z_RBC = 14,       /* RBC z pass: 4 c2r, 9 products, 9 r2c */
+    TG_KERNEL_X_RBC = 15,       /* RBC x pass: sin/cos forward + N, N_T + force + projection + RK4 stage */
+    TG_KERNEL_COUNT = 16
 };
 int tg_profile_enable(tg_plan *plan, int on);
 int tg_profile_reset(tg_plan *plan);

@@ -29,6 +29,14 @@ class TgFluid(ctypes.Structure):
                 ("force_r_out", ctypes.c_double)]
 
 
+BASIS_FFF, BASIS_SFF = 0, 1
+SC_FOURIER, SC_SIN, SC_COS = 0, 1, 2
+
+
+class TgRbc(ctypes.Structure):
+    _fields_ = [("Ra", ctypes.c_double), ("Pr", ctypes.c_double), ("dt", ctypes.c_double)]
+
+
 class TarangGpuError(RuntimeError):
     def __init__(self, fn, code, msg):
         super().__init__(f"{fn} failed with code {code}: {msg}")
@@ -68,6 +76,11 @@ SIGNATURES = {
     "tg_init_field": (_I, [_P, _I, _D, _D, _D, _D, ctypes.c_uint64, _PP]),
     "tg_compute_nlin_mhd": (_I, [_P, _PP, _PP, _PP, _PP]),
     "tg_rk4_step_mhd": (_I, [_P, _PP, _PP, ctypes.POINTER(TgFluid), _D, _I]),
+    "tg_weak_reality": (_I, [_P, _PP, _I]),
+    "tg_compute_nlin_rbc": (_I, [_P, _PP, _P, _PP, _P]),
+    "tg_compute_rhs_rbc": (_I, [_P, _PP, _P, ctypes.POINTER(TgRbc), _PP, _P]),
+    "tg_rk4_step_rbc": (_I, [_P, _PP, _P, ctypes.POINTER(TgRbc), _I]),
+    "tg_nusselt": (_I, [_P, _P, _P, ctypes.POINTER(TgRbc), ctypes.POINTER(_D)]),
     "tg_rhs_vjp": (_I, [_P, _PP, _PP, _PP]),
     "tg_adjoint_forward": (_I, [_P, _PP, ctypes.POINTER(TgFluid), _I, _I]),
     "tg_adjoint_backward": (_I, [_P, _PP]),
@@ -81,7 +94,7 @@ SIGNATURES = {
 }
 
 KERNELS = ["x_inv", "y_fft", "z_products", "x_nlin_rk", "x_nlin_giver", "reality", "z_r2c", "z_c2r", "x_fwd",
-           "reduce", "x_inv_sym", "z_adj", "x_fwd_adj"]
+           "reduce", "x_inv_sym", "z_adj", "x_fwd_adj", "x_sc", "z_rbc", "x_rbc"]
 
 
 def header_symbols():

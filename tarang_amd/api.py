@@ -24,10 +24,11 @@ import torch
 
 from . import _lib
 from ._lib import (DEALIAS_NONE, DEALIAS_REF_CORNER, DEALIAS_TWOTHIRDS, PREC_FP32, PREC_FP64, STEP_CONTINUE,
-                   TgConfig, TgFluid, check, ptr_array)
+                   TgConfig, TgFluid, check, ptr_array, BASIS_SFF, SC_COS, SC_SIN, TgRbc)
 
-__all__ = ["FFF_Basis", "FluidVF", "Nlin_incompress", "Time_advance_incompress", "Time_advance_incompress_MHD",
-           "Adjoint_incompress", "Correlation", "EnergyTr",
+__all__ = ["FFF_Basis", "SFF_Basis", "FluidVF", "FluidSF", "Nlin_incompress", "Time_advance_incompress",
+           "Time_advance_incompress_MHD", "Time_advance_incompress_RBC", "Adjoint_incompress", "Correlation",
+           "EnergyTr", "SC_SIN", "SC_COS",
            "DEALIAS_NONE", "DEALIAS_REF_CORNER", "DEALIAS_TWOTHIRDS", "PREC_FP64", "PREC_FP32"]
 
 
@@ -37,6 +38,8 @@ def _vp(t):
 
 class FFF_Basis:
     """Periodic Fourier basis on an (Nx, Ny, Nz) grid, one plan per instance."""
+
+    basis_code = 0
 
     def __init__(self, N, kfactor=(1.0, 1.0, 1.0), precision=PREC_FP64, dealias=DEALIAS_REF_CORNER,
                  strong_reality=True, device=0):
@@ -48,8 +51,8 @@ class FFF_Basis:
         self.dealias_mode = dealias
         self.device = torch.device("cuda", device)
         self.L = _lib.load()
-        cfg = TgConfig(0, self.Nx, self.Ny, self.Nz, (ctypes.c_double * 3)(*self.kfactor), precision, dealias,
-                       1 if strong_reality else 0, device)
+        cfg = TgConfig(self.basis_code, self.Nx, self.Ny, self.Nz, (ctypes.c_double * 3)(*self.kfactor), precision,
+                       dealias, int(strong_reality), device)
         h = ctypes.c_void_p()
         check("tg_plan_create", self.L.tg_plan_create(ctypes.byref(cfg), ctypes.byref(h)))
         self.plan = h
@@ -146,6 +149,49 @@ class FFF_Basis:
         return r
 
 
+class SFF_Basis(FFF_Basis):
+    """sin/cos along x (walls at x = 0 and L_x), Fourier along y and z
+    (SFF_pencil.cc).  Default kfactor is the reference's RBC box
+    (pi, pi/sqrt 2, pi/sqrt 2) (SFF_pencil.cc:67-72); reality 0 none, 1 strong,
+    2 weak (the reference's RBC example applies the weak condition)."""
+
+    basis_code = BASIS_SFF
+
+    def __init__(self, N, kfactor=(np.pi, np.pi / np.sqrt(2.0), np.pi / np.sqrt(2.0)), precision=PREC_FP64,
+                 dealias=DEALIAS_REF_CORNER, reality=2, device=0):
+        super().__init__(N, kfactor=kfactor, precision=precision, dealias=dealias, strong_reality=reality,
+                         device=device)
+
+    # SFF_pencil_tr.cc:43-67; sincos SC_SIN ('S') or SC_COS ('C')
+    def Forward_transform(self, Ar, A, sincos=SC_SIN):
+        check("tg_forward", self.L.tg_forward(self.plan, sincos, _vp(Ar), _vp(A)))
+
+    def Inverse_transform(self, A, Ar, sincos=SC_SIN):
+        check("tg_inverse", self.L.tg_inverse(self.plan, sincos, _vp(A), _vp(Ar)))
+
+    def Satisfy_weak_reality_condition_in_Array(self, *arrays):
+        check("tg_weak_reality",
+              self.L.tg_weak_reality(self.plan, ptr_array([a.data_ptr() for a in arrays]), len(arrays)))
+
+
+class FluidSF:
+    """Scalar field (temperature): csf.F, nlin, diffusion coefficient."""
+
+    def __init__(self, basis, diffusion_coefficient=0.0, name="T"):
+        self.basis = basis
+        self.name = name
+        self.diffusion_coefficient = float(diffusion_coefficient)
+        self.F = basis.complex_array()
+        self.nlin = basis.complex_array()
+
+    def set_field(self, F):
+        self.F.copy_(torch.as_tensor(F, dtype=torch.complex128))
+
+    def Compute_total_energy(self):
+        self.total_energy = self.basis.Get_total_energy(self.F)
+        return self.total_energy
+
+
 class FluidVF:
     """Solver-level vector field: cvf (V1, V2, V3), nlin1..3, dissipation."""
 
@@ -213,6 +259,15 @@ class Nlin_incompress:
 
 class Correlation:
     @staticmethod
+    def Get_Nusselt_no(U, T, Ra, Pr):
+        """Correlation.cc:144-160 (PRLARGE / ULARGE)."""
+        out = ctypes.c_double()
+        b = U.basis
+        prm = TgRbc(float(Ra), float(Pr), 1.0)
+        check("tg_nusselt", b.L.tg_nusselt(b.plan, _vp(U.cvf[0]), _vp(T.F), ctypes.byref(prm), ctypes.byref(out)))
+        return out.value
+
+    @staticmethod
     def Get_cross_helicity(U, W):
         out = ctypes.c_double()
         b = U.basis
@@ -256,6 +311,34 @@ class Time_advance_incompress_MHD:
         b = self.U.basis
         check("tg_rk4_step_mhd", b.L.tg_rk4_step_mhd(b.plan, self.U.ptrs(), self.W.ptrs(), ctypes.byref(self.fluid),
                                                      self.W.dissipation_coefficient, int(nsteps)))
+
+
+class Time_advance_incompress_RBC:
+    """Rayleigh-Benard convection on an SFF basis, PRLARGE / ULARGE
+    (nu = sqrt(Pr/Ra), kappa = 1/sqrt(Pr Ra)): RK4 of U and T (RK_SF.cc:97-215)."""
+
+    def __init__(self, U, T, Ra, Pr, dt):
+        self.U, self.T = U, T
+        self.prm = TgRbc(float(Ra), float(Pr), float(dt))
+        U.dissipation_coefficient = float(np.sqrt(Pr / Ra))
+        T.diffusion_coefficient = float(1.0 / np.sqrt(Pr * Ra))
+
+    def Compute_nlin(self):
+        """Nlin_incompress::Compute_nlin(U, T) into U.nlin, T.nlin (dealiases U, T)."""
+        b = self.U.basis
+        check("tg_compute_nlin_rbc", b.L.tg_compute_nlin_rbc(b.plan, self.U.ptrs(), _vp(self.T.F),
+                                                             self.U.ptrs("nlin"), _vp(self.T.nlin)))
+
+    def Compute_force_TO_rhs(self):
+        b = self.U.basis
+        check("tg_compute_rhs_rbc", b.L.tg_compute_rhs_rbc(b.plan, self.U.ptrs(), _vp(self.T.F),
+                                                           ctypes.byref(self.prm), self.U.ptrs("nlin"),
+                                                           _vp(self.T.nlin)))
+
+    def Time_advance_step(self, nsteps=1):
+        b = self.U.basis
+        check("tg_rk4_step_rbc", b.L.tg_rk4_step_rbc(b.plan, self.U.ptrs(), _vp(self.T.F), ctypes.byref(self.prm),
+                                                     int(nsteps)))
 
 
 class Adjoint_incompress:

@@ -15,7 +15,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libtarang_amd.so")
 DRIVER = os.path.join(HERE, "tarang_ifluid")
-SOURCES = ["tg_kernels.hip", "tg_diag.hip", "tg_mhd.hip", "tg_adjoint.hip", "tg_api.hip"]
+SOURCES = ["tg_kernels.hip", "tg_diag.hip", "tg_mhd.hip", "tg_adjoint.hip", "tg_sff.hip", "tg_api.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("TG_OFFLOAD_ARCH", "gfx950")
 FLAGS = [

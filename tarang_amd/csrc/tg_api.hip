@@ -16,6 +16,7 @@
 #include "tg_diag.h"
 #include "tg_kernels.h"
 #include "tg_mhd.h"
+#include "tg_sff.h"
 
 using namespace tg;
 
@@ -72,6 +73,12 @@ struct tg_plan {
     // host-side band config for the current step
     Band band{};
     int x_tile_columns = 0;  // fused x-pass tile width (0 = LDS-budget width)
+    // SFF: quarter-wave twiddles exp(-i pi k / (2 Nx)), k < Nx
+    void *twq = nullptr;
+    // RBC buffers, allocated on first use: [0] theta accumulator (FP64 padded
+    // work layout), [1..4] force inputs theta, u_x of two stage parities
+    // (FP64 boundary layout)
+    double *rbc[5] = {};
     // MHD buffers (FP64, boundary layout), allocated on first use:
     // [0..2] accW, [3..5] N+, [6..8] N-, [9..11] Z+, [12..14] Z-
     double *mhd[15] = {};
@@ -183,8 +190,10 @@ int dmalloc(tg_plan *p, void **ptr, size_t bytes)
 void free_plan(tg_plan *p)
 {
     if (!p) return;
-    void *ptrs[] = {p->twx, p->twy, p->twm, p->twn, p->acc[0], p->acc[1], p->acc[2], p->slots[0],
+    void *ptrs[] = {p->twx, p->twy, p->twm, p->twn, p->twq, p->acc[0], p->acc[1], p->acc[2], p->slots[0],
                     p->slots[1], p->partial, p->small};
+    for (void *q : p->rbc)
+        if (q) (void)hipFree(q);
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     for (void *q : p->U)
@@ -245,26 +254,39 @@ int ensure_slots(tg_plan *p, const tg_fluid *f)
     return TG_OK;
 }
 
+// sincos: 0 Fourier (FFF), 1 sine, 2 cosine (SFF)
 template <typename T>
-int run_forward(tg_plan *p, const double *real_dev, double *spec_dev)
+int run_forward(tg_plan *p, const double *real_dev, double *spec_dev, int sincos)
 {
     const double *R[1] = {real_dev};
     void *W[1] = {p->U[0]};
     { Prof pr(p, TG_KERNEL_Z_R2C); HIPCHK(launch_z_r2c<T>(p->g, R, 1, W, p->twm, p->twn, p->stream)); }
     { Prof pr(p, TG_KERNEL_Y_FFT); HIPCHK(launch_colfft<T>(p->g, 1, -1, W, 1, p->twy, p->stream)); }
     double *O[1] = {spec_dev};
+    if (p->g.basis == 1) {
+        const double scale = 1.0 / (2.0 * p->g.nx * (double)p->g.ny * p->g.nz);
+        Prof pr(p, TG_KERNEL_X_SC);
+        HIPCHK(launch_xfwd_sc<T>(p->g, W, 1, sincos == TG_SC_SIN ? 1 : 0, O, scale, p->twx, p->twq, p->stream));
+        return TG_OK;
+    }
     const double scale = 1.0 / ((double)p->g.nx * p->g.ny * p->g.nz);
     { Prof pr(p, TG_KERNEL_X_FWD); HIPCHK(launch_xfwd_spec<T>(p->g, W, 1, O, scale, p->twx, p->stream)); }
     return TG_OK;
 }
 
 template <typename T>
-int run_inverse(tg_plan *p, const double *spec_dev, double *real_dev)
+int run_inverse(tg_plan *p, const double *spec_dev, double *real_dev, int sincos)
 {
     double *V[1] = {const_cast<double *>(spec_dev)};
     void *W[1] = {p->U[0]};
     Band nb{};
-    { Prof pr(p, TG_KERNEL_X_INV); HIPCHK(launch_xinv_state<T>(p->g, nb, V, 1, W, nullptr, 0, p->twx, p->stream)); }
+    if (p->g.basis == 1) {
+        Prof pr(p, TG_KERNEL_X_SC);
+        HIPCHK(launch_xinv_sc<T>(p->g, V, 1, sincos == TG_SC_SIN ? 1 : 0, W, 0, p->twx, p->twq, p->stream));
+    } else {
+        Prof pr(p, TG_KERNEL_X_INV);
+        HIPCHK(launch_xinv_state<T>(p->g, nb, V, 1, W, nullptr, 0, p->twx, p->stream));
+    }
     { Prof pr(p, TG_KERNEL_Y_FFT); HIPCHK(launch_colfft<T>(p->g, 1, +1, W, 1, p->twy, p->stream)); }
     double *R[1] = {real_dev};
     { Prof pr(p, TG_KERNEL_Z_C2R); HIPCHK(launch_z_c2r<T>(p->g, W, 1, R, p->twm, p->twn, p->stream)); }
@@ -672,6 +694,115 @@ int run_rhs_vjp(tg_plan *p, double *const V[3], double *const lam[3], double *co
     return adj_stage<T>(p, lam, tp, adj_params(p, nullptr, ADJ_RHS), nullptr, nullptr, out);
 }
 
+// ---------------------------------------------------------------------------
+// Rayleigh-Benard convection on the SFF basis
+// ---------------------------------------------------------------------------
+int ensure_rbc(tg_plan *p)
+{
+    if (!p->rbc[0]) {
+        int rc = dmalloc(p, (void **)&p->rbc[0], p->ncw * 16);
+        if (rc) return rc;
+    }
+    for (int i = 1; i < 5; i++)
+        if (!p->rbc[i]) {
+            int rc = dmalloc(p, (void **)&p->rbc[i], p->nc * 16);
+            if (rc) return rc;
+        }
+    return TG_OK;
+}
+
+RbcParams rbc_params(const tg_plan *p, const tg_rbc *r, int stage)
+{
+    RbcParams rp{};
+    rp.stage = stage;
+    const double nu = r ? std::sqrt(r->Pr / r->Ra) : 0.0, kappa = r ? 1.0 / std::sqrt(r->Pr * r->Ra) : 0.0;
+    const double dt = r ? r->dt : 0.0;
+    rp.dt = dt;
+    rp.fh_u = -nu * 0.5 * dt;
+    rp.f1_u = -nu * 1.0 * dt;
+    rp.fh_t = -kappa * 0.5 * dt;
+    rp.f1_t = -kappa * 1.0 * dt;
+    rp.scale = 1.0 / (2.0 * p->g.nx * (double)p->g.ny * p->g.nz);
+    return rp;
+}
+
+// y-inverse of U[0..3], z products, y-forward of P[0..8]
+template <typename T>
+int rbc_middle(tg_plan *p)
+{
+    { Prof pr(p, TG_KERNEL_Y_FFT); HIPCHK(launch_colfft<T>(p->g, 1, +1, p->U, 4, p->twy, p->stream)); }
+    { Prof pr(p, TG_KERNEL_Z_RBC); HIPCHK(launch_z_rbc<T>(p->g, p->U, p->P, p->twm, p->twn, p->stream)); }
+    { Prof pr(p, TG_KERNEL_Y_FFT); HIPCHK(launch_colfft<T>(p->g, 1, -1, p->P, 9, p->twy, p->stream)); }
+    return TG_OK;
+}
+
+template <typename T>
+int run_nlin_rbc(tg_plan *p, double *const V[3], double *T_, double *const out[3], double *outT, const tg_rbc *r)
+{
+    int rc = ensure_rbc(p);
+    if (rc) return rc;
+    double *S[4] = {V[0], V[1], V[2], T_};
+    const double *f_in[2] = {p->rbc[1], p->rbc[2]};
+    if (r) {
+        // the force sees the fields before Compute_nlin dealiases them
+        HIPCHK(hipMemcpyAsync(p->rbc[1], T_, p->nc * 16, hipMemcpyDeviceToDevice, p->stream));
+        HIPCHK(hipMemcpyAsync(p->rbc[2], V[0], p->nc * 16, hipMemcpyDeviceToDevice, p->stream));
+    }
+    if (p->g.dealias) {
+        HIPCHK(launch_dealias(p->g, S, 3, p->stream));
+        HIPCHK(launch_dealias(p->g, S + 3, 1, p->stream));
+    }
+    {
+        Prof pr(p, TG_KERNEL_X_SC);
+        HIPCHK(launch_xinv_sc<T>(p->g, S, 4, kRbcStateParity, p->U, 0, p->twx, p->twq, p->stream));
+    }
+    rc = rbc_middle<T>(p);
+    if (rc) return rc;
+    RbcParams rp = rbc_params(p, r, 0);
+    rp.write_rhs = r ? 1 : 0;
+    double *O[4] = {out[0], out[1], out[2], outT};
+    Prof pr(p, TG_KERNEL_X_RBC);
+    HIPCHK(launch_x_rbc<T>(p->g, rp, p->P, nullptr, nullptr, nullptr, f_in, nullptr, O, p->twx, p->twq, p->stream));
+    p->unext_valid = false;
+    return TG_OK;
+}
+
+template <typename T>
+int run_rk4_rbc(tg_plan *p, double *const V[3], double *T_, const tg_rbc *r, int nsteps)
+{
+    int rc = ensure_rbc(p);
+    if (rc) return rc;
+    double *S[4] = {V[0], V[1], V[2], T_};
+    double *acc[4] = {p->acc[0], p->acc[1], p->acc[2], p->rbc[0]};
+    const int reality = p->cfg.strong_reality;
+    p->unext_valid = false;
+    {
+        Prof pr(p, TG_KERNEL_X_SC);
+        HIPCHK(launch_xinv_sc<T>(p->g, S, 4, kRbcStateParity, p->U, 1, p->twx, p->twq, p->stream));
+    }
+    for (int step = 0; step < nsteps; step++) {
+        for (int s = 1; s <= 4; s++) {
+            rc = rbc_middle<T>(p);
+            if (rc) return rc;
+            // force of stage s: the state (stage 1) or the undealiased stage
+            // input written by stage s-1, ping-pong by stage parity
+            const int rd = (s - 1) & 1, wr = s & 1;
+            const double *f_in[2] = {s == 1 ? T_ : p->rbc[1 + 2 * rd], s == 1 ? V[0] : p->rbc[2 + 2 * rd]};
+            double *f_out[2] = {p->rbc[1 + 2 * wr], p->rbc[2 + 2 * wr]};
+            RbcParams rp = rbc_params(p, r, s);
+            Prof pr(p, TG_KERNEL_X_RBC);
+            HIPCHK(launch_x_rbc<T>(p->g, rp, p->P, S, acc, p->U, f_in, f_out, nullptr, p->twx, p->twq, p->stream));
+        }
+        if (reality == 1 || reality == 2) {
+            Prof pr(p, TG_KERNEL_REALITY);
+            HIPCHK(launch_sff_reality(p->g, S, 4, reality, p->stream));
+            // the next stage-1 input: redo the x-inverse of the kz = 0 and Nz/2 columns
+            HIPCHK(launch_xinv_sc<T>(p->g, S, 4, kRbcStateParity, p->U, 1 | 4, p->twx, p->twq, p->stream));
+        }
+    }
+    return TG_OK;
+}
+
 int reduce_to_host(tg_plan *p, int mode, const double *const *A, const double *const *B, int ncomp, int n,
                    const double *radii_host, int nbins, int nq, int kcut, double r_lo, double r_hi, double *out)
 {
@@ -720,7 +851,10 @@ int tg_plan_create(const tg_config *cfg, tg_plan **plan)
 {
     if (!cfg || !plan) return fail(TG_ERR_ARG, "null argument");
     *plan = nullptr;
-    if (cfg->basis != TG_BASIS_FFF) return fail(TG_ERR_ARG, "unsupported basis %d", cfg->basis);
+    if (cfg->basis != TG_BASIS_FFF && cfg->basis != TG_BASIS_SFF)
+        return fail(TG_ERR_ARG, "unsupported basis %d", cfg->basis);
+    if (cfg->strong_reality < 0 || cfg->strong_reality > (cfg->basis == TG_BASIS_SFF ? 2 : 1))
+        return fail(TG_ERR_ARG, "strong_reality %d out of range for this basis", cfg->strong_reality);
     if (!pow2_in(cfg->nx, 8, 1024) || !pow2_in(cfg->ny, 8, 1024) || !pow2_in(cfg->nz, 16, 1024))
         return fail(TG_ERR_SHAPE, "grid %dx%dx%d unsupported (powers of two, nx,ny in [8,1024], nz in [16,1024])",
                     cfg->nx, cfg->ny, cfg->nz);
@@ -741,6 +875,7 @@ int tg_plan_create(const tg_config *cfg, tg_plan **plan)
     p->g.nzc = cfg->nz / 2 + 1;
     for (int i = 0; i < 3; i++) p->g.kf[i] = cfg->kfactor[i];
     p->g.dealias = cfg->dealias;
+    p->g.basis = cfg->basis == TG_BASIS_SFF ? 1 : 0;
     p->prec = cfg->precision;
     p->elem = p->prec == TG_PREC_FP32 ? 8 : 16;
     {
@@ -757,11 +892,13 @@ int tg_plan_create(const tg_config *cfg, tg_plan **plan)
         if (!rc) rc = upload_twiddles<float>(p, &p->twy, cfg->ny, cfg->ny);
         if (!rc) rc = upload_twiddles<float>(p, &p->twm, cfg->nz / 2, cfg->nz / 2);
         if (!rc) rc = upload_twiddles<float>(p, &p->twn, cfg->nz, cfg->nz / 2 + 1);
+        if (!rc && p->g.basis == 1) rc = upload_twiddles<float>(p, &p->twq, 4 * cfg->nx, cfg->nx);
     } else {
         rc = upload_twiddles<double>(p, &p->twx, cfg->nx, cfg->nx);
         if (!rc) rc = upload_twiddles<double>(p, &p->twy, cfg->ny, cfg->ny);
         if (!rc) rc = upload_twiddles<double>(p, &p->twm, cfg->nz / 2, cfg->nz / 2);
         if (!rc) rc = upload_twiddles<double>(p, &p->twn, cfg->nz, cfg->nz / 2 + 1);
+        if (!rc && p->g.basis == 1) rc = upload_twiddles<double>(p, &p->twq, 4 * cfg->nx, cfg->nx);
     }
     for (int i = 0; i < 6 && !rc; i++) rc = dmalloc(p, &p->U[i], p->ncw * p->elem);
     for (int i = 0; i < 9 && !rc; i++) rc = dmalloc(p, &p->P[i], p->ncw * p->elem);
@@ -810,24 +947,46 @@ int tg_get_shapes(const tg_plan *plan, int64_t cshape[3], int64_t rshape[3])
 
 int64_t tg_plan_device_bytes(const tg_plan *plan) { return plan ? plan->dev_bytes : 0; }
 
+static int check_sincos(const tg_plan *p, int code)
+{
+    if (p->g.basis == 0 && code != TG_SC_FOURIER) return fail(TG_ERR_ARG, "FFF basis takes sincos_code 0");
+    if (p->g.basis == 1 && code != TG_SC_SIN && code != TG_SC_COS)
+        return fail(TG_ERR_ARG, "SFF basis takes sincos_code TG_SC_SIN or TG_SC_COS");
+    return TG_OK;
+}
+
+static int need_basis(const tg_plan *p, int basis, const char *what)
+{
+    if (p->g.basis != basis)
+        return fail(TG_ERR_ARG, "%s needs a plan with basis %s", what, basis ? "TG_BASIS_SFF" : "TG_BASIS_FFF");
+    return TG_OK;
+}
+
+static int check_rbc(const tg_rbc *r)
+{
+    if (!r) return fail(TG_ERR_ARG, "null tg_rbc");
+    if (!(r->Ra > 0) || !(r->Pr > 0) || !(r->dt > 0)) return fail(TG_ERR_ARG, "need Ra > 0, Pr > 0, dt > 0");
+    return TG_OK;
+}
+
 int tg_forward(tg_plan *plan, int sincos_code, const double *real_dev, double *spec_dev)
 {
     if (!plan || !real_dev || !spec_dev) return fail(TG_ERR_ARG, "null argument");
-    if (sincos_code != 0) return fail(TG_ERR_ARG, "FFF basis takes sincos_code 0");
-    int rc = set_device(plan);
+    int rc = check_sincos(plan, sincos_code);
+    if (!rc) rc = set_device(plan);
     if (rc) return rc;
     plan->unext_valid = false;
-    return DISPATCH_PREC(plan, run_forward)(plan, real_dev, spec_dev);
+    return DISPATCH_PREC(plan, run_forward)(plan, real_dev, spec_dev, sincos_code);
 }
 
 int tg_inverse(tg_plan *plan, int sincos_code, const double *spec_dev, double *real_dev)
 {
     if (!plan || !real_dev || !spec_dev) return fail(TG_ERR_ARG, "null argument");
-    if (sincos_code != 0) return fail(TG_ERR_ARG, "FFF basis takes sincos_code 0");
-    int rc = set_device(plan);
+    int rc = check_sincos(plan, sincos_code);
+    if (!rc) rc = set_device(plan);
     if (rc) return rc;
     plan->unext_valid = false;
-    return DISPATCH_PREC(plan, run_inverse)(plan, spec_dev, real_dev);
+    return DISPATCH_PREC(plan, run_inverse)(plan, spec_dev, real_dev, sincos_code);
 }
 
 int tg_dealias(tg_plan *plan, double *const V_dev[], int ncomp)
@@ -849,13 +1008,82 @@ int tg_strong_reality(tg_plan *plan, double *const V_dev[], int ncomp)
     rc = set_device(plan);
     if (rc) return rc;
     plan->unext_valid = false;
-    HIPCHK(launch_reality(plan->g, V_dev, ncomp, plan->stream));
+    if (plan->g.basis == 1) HIPCHK(launch_sff_reality(plan->g, V_dev, ncomp, 1, plan->stream));
+    else HIPCHK(launch_reality(plan->g, V_dev, ncomp, plan->stream));
+    return TG_OK;
+}
+
+int tg_weak_reality(tg_plan *plan, double *const V_dev[], int ncomp)
+{
+    if (!plan || ncomp < 1 || ncomp > 4) return fail(TG_ERR_ARG, "bad argument");
+    int rc = check_ptrs((const void *const *)V_dev, ncomp, "tg_weak_reality");
+    if (rc) return rc;
+    rc = set_device(plan);
+    if (rc) return rc;
+    plan->unext_valid = false;
+    // the kz = Nz/2 plane is zeroed in both bases
+    HIPCHK(launch_sff_reality(plan->g, V_dev, ncomp, 2, plan->stream));
+    return TG_OK;
+}
+
+int tg_compute_nlin_rbc(tg_plan *plan, double *const V_dev[3], double *T_dev, double *const nlin_dev[3],
+                        double *nlinT_dev)
+{
+    if (!plan || !T_dev || !nlinT_dev) return fail(TG_ERR_ARG, "null argument");
+    int rc = need_basis(plan, 1, "tg_compute_nlin_rbc");
+    if (!rc) rc = check_ptrs((const void *const *)V_dev, 3, "V");
+    if (!rc) rc = check_ptrs((const void *const *)nlin_dev, 3, "nlin");
+    if (!rc) rc = set_device(plan);
+    if (rc) return rc;
+    return DISPATCH_PREC(plan, run_nlin_rbc)(plan, V_dev, T_dev, nlin_dev, nlinT_dev, nullptr);
+}
+
+int tg_compute_rhs_rbc(tg_plan *plan, double *const V_dev[3], double *T_dev, const tg_rbc *rbc,
+                       double *const rhs_dev[3], double *rhsT_dev)
+{
+    if (!plan || !T_dev || !rhsT_dev) return fail(TG_ERR_ARG, "null argument");
+    int rc = need_basis(plan, 1, "tg_compute_rhs_rbc");
+    if (!rc) rc = check_rbc(rbc);
+    if (!rc) rc = check_ptrs((const void *const *)V_dev, 3, "V");
+    if (!rc) rc = check_ptrs((const void *const *)rhs_dev, 3, "rhs");
+    if (!rc) rc = set_device(plan);
+    if (rc) return rc;
+    return DISPATCH_PREC(plan, run_nlin_rbc)(plan, V_dev, T_dev, rhs_dev, rhsT_dev, rbc);
+}
+
+int tg_rk4_step_rbc(tg_plan *plan, double *const V_dev[3], double *T_dev, const tg_rbc *rbc, int nsteps)
+{
+    if (!plan || !T_dev) return fail(TG_ERR_ARG, "null argument");
+    if (nsteps < 0) return fail(TG_ERR_ARG, "nsteps must be >= 0");
+    int rc = need_basis(plan, 1, "tg_rk4_step_rbc");
+    if (!rc) rc = check_rbc(rbc);
+    if (!rc) rc = check_ptrs((const void *const *)V_dev, 3, "V");
+    if (!rc) rc = set_device(plan);
+    if (rc) return rc;
+    return DISPATCH_PREC(plan, run_rk4_rbc)(plan, V_dev, T_dev, rbc, nsteps);
+}
+
+int tg_nusselt(tg_plan *plan, const double *V1_dev, const double *T_dev, const tg_rbc *rbc, double *nu)
+{
+    if (!plan || !V1_dev || !T_dev || !nu) return fail(TG_ERR_ARG, "null argument");
+    int rc = need_basis(plan, 1, "tg_nusselt");
+    if (!rc) rc = check_rbc(rbc);
+    if (!rc) rc = set_device(plan);
+    if (rc) return rc;
+    const double *A[1] = {V1_dev}, *B[1] = {T_dev};
+    double e = 0.0;
+    rc = reduce_to_host(plan, RED_CROSS, A, B, 1, 0, nullptr, 1, 1, 0, 0, 0, &e);
+    if (rc) return rc;
+    // RED_CROSS carries the SFF weights w = (lx?2:1)(lz?2:1)/2 of
+    // SFF_pencil_energy.cc:117-133
+    *nu = 1.0 + 2.0 * std::sqrt(rbc->Ra * rbc->Pr) * e;
     return TG_OK;
 }
 
 int tg_compute_nlin(tg_plan *plan, double *const V_dev[3], double *const nlin_dev[3])
 {
     if (!plan) return fail(TG_ERR_ARG, "null plan");
+    if (int rb_ = need_basis(plan, 0, "tg_compute_nlin")) return rb_;
     int rc = check_ptrs((const void *const *)V_dev, 3, "V");
     if (!rc) rc = check_ptrs((const void *const *)nlin_dev, 3, "nlin");
     if (!rc) rc = set_device(plan);
@@ -866,6 +1094,7 @@ int tg_compute_nlin(tg_plan *plan, double *const V_dev[3], double *const nlin_de
 int tg_compute_rhs(tg_plan *plan, double *const V_dev[3], const tg_fluid *fluid, double *const rhs_dev[3])
 {
     if (!plan || !fluid) return fail(TG_ERR_ARG, "null argument");
+    if (int rb_ = need_basis(plan, 0, "tg_compute_rhs")) return rb_;
     int rc = check_ptrs((const void *const *)V_dev, 3, "V");
     if (!rc) rc = check_ptrs((const void *const *)rhs_dev, 3, "rhs");
     if (!rc) rc = set_device(plan);
@@ -876,6 +1105,7 @@ int tg_compute_rhs(tg_plan *plan, double *const V_dev[3], const tg_fluid *fluid,
 int tg_compute_nlin_giver(tg_plan *plan, double *const V_dev[3], double *const W_dev[3], double *const nlin_dev[3])
 {
     if (!plan) return fail(TG_ERR_ARG, "null plan");
+    if (int rb_ = need_basis(plan, 0, "tg_compute_nlin_giver")) return rb_;
     int rc = check_ptrs((const void *const *)V_dev, 3, "V");
     if (!rc) rc = check_ptrs((const void *const *)W_dev, 3, "W");
     if (!rc) rc = check_ptrs((const void *const *)nlin_dev, 3, "nlin");
@@ -891,6 +1121,7 @@ int tg_compute_nlin_giver(tg_plan *plan, double *const V_dev[3], double *const W
 int tg_rk4_step(tg_plan *plan, double *const V_dev[3], const tg_fluid *fluid, int nsteps, int flags)
 {
     if (!plan || !fluid) return fail(TG_ERR_ARG, "null argument");
+    if (int rb_ = need_basis(plan, 0, "tg_rk4_step")) return rb_;
     if (nsteps < 0) return fail(TG_ERR_ARG, "nsteps must be >= 0");
     if (!(fluid->dt > 0) || !(fluid->nu >= 0)) return fail(TG_ERR_ARG, "need dt > 0 and nu >= 0");
     int rc = check_ptrs((const void *const *)V_dev, 3, "V");
@@ -911,6 +1142,7 @@ int tg_energy(tg_plan *plan, const double *A_dev, double *energy)
 int tg_Sn(tg_plan *plan, const double *A_dev, int n, double *Sn)
 {
     if (!plan || !A_dev || !Sn || n < 0) return fail(TG_ERR_ARG, "bad argument");
+    if (int rb_ = need_basis(plan, 0, "tg_Sn")) return rb_;
     int rc = set_device(plan);
     if (rc) return rc;
     const double *A[1] = {A_dev};
@@ -920,6 +1152,7 @@ int tg_Sn(tg_plan *plan, const double *A_dev, int n, double *Sn)
 int tg_shell_spectrum(tg_plan *plan, const double *A_dev, int n, double *Sk, int nbins)
 {
     if (!plan || !A_dev || !Sk || n < 0) return fail(TG_ERR_ARG, "bad argument");
+    if (int rb_ = need_basis(plan, 0, "tg_shell_spectrum")) return rb_;
     const int kmax = min_radius_outside(plan->g);
     if (nbins < kmax + 1) return fail(TG_ERR_ARG, "nbins must be >= %d", kmax + 1);
     int rc = set_device(plan);
@@ -940,6 +1173,7 @@ int tg_shell_spectrum(tg_plan *plan, const double *A_dev, int n, double *Sk, int
 int tg_flux(tg_plan *plan, double *const V_dev[3], const double *radii, int nspheres, double *flux)
 {
     if (!plan || !radii || !flux || nspheres < 1) return fail(TG_ERR_ARG, "bad argument");
+    if (int rb_ = need_basis(plan, 0, "tg_flux")) return rb_;
     int rc = check_ptrs((const void *const *)V_dev, 3, "V");
     if (!rc) rc = set_device(plan);
     if (rc) return rc;
@@ -962,6 +1196,7 @@ int tg_flux(tg_plan *plan, double *const V_dev[3], const double *radii, int nsph
 int tg_shell_to_shell(tg_plan *plan, double *const V_dev[3], const double *radii, int nshells, double *S)
 {
     if (!plan || !radii || !S || nshells < 1) return fail(TG_ERR_ARG, "bad argument");
+    if (int rb_ = need_basis(plan, 0, "tg_shell_to_shell")) return rb_;
     int rc = check_ptrs((const void *const *)V_dev, 3, "V");
     if (!rc) rc = set_device(plan);
     if (rc) return rc;
@@ -987,6 +1222,7 @@ int tg_compute_nlin_mhd(tg_plan *plan, double *const V_dev[3], double *const W_d
                         double *const nlinW_dev[3])
 {
     if (!plan) return fail(TG_ERR_ARG, "null plan");
+    if (int rb_ = need_basis(plan, 0, "tg_compute_nlin_mhd")) return rb_;
     int rc = check_ptrs((const void *const *)V_dev, 3, "V");
     if (!rc) rc = check_ptrs((const void *const *)W_dev, 3, "W");
     if (!rc) rc = check_ptrs((const void *const *)nlinV_dev, 3, "nlinV");
@@ -1000,6 +1236,7 @@ int tg_rk4_step_mhd(tg_plan *plan, double *const V_dev[3], double *const W_dev[3
                     int nsteps)
 {
     if (!plan || !fluid) return fail(TG_ERR_ARG, "null argument");
+    if (int rb_ = need_basis(plan, 0, "tg_rk4_step_mhd")) return rb_;
     if (nsteps < 0) return fail(TG_ERR_ARG, "nsteps must be >= 0");
     if (!(fluid->dt > 0) || !(fluid->nu >= 0) || !(eta >= 0)) return fail(TG_ERR_ARG, "need dt > 0, nu, eta >= 0");
     if (fluid->force_kind != 0) return fail(TG_ERR_ARG, "forcing is not supported for MHD");
@@ -1013,6 +1250,7 @@ int tg_rk4_step_mhd(tg_plan *plan, double *const V_dev[3], double *const W_dev[3
 int tg_cross_helicity(tg_plan *plan, double *const V_dev[3], double *const W_dev[3], double *hc)
 {
     if (!plan || !hc) return fail(TG_ERR_ARG, "null argument");
+    if (int rb_ = need_basis(plan, 0, "tg_cross_helicity")) return rb_;
     int rc = check_ptrs((const void *const *)V_dev, 3, "V");
     if (!rc) rc = check_ptrs((const void *const *)W_dev, 3, "W");
     if (!rc) rc = set_device(plan);
@@ -1034,6 +1272,7 @@ int tg_flux_mhd(tg_plan *plan, double *const V_dev[3], double *const W_dev[3], c
                 double *u2b, double *b2b)
 {
     if (!plan || !radii || !u2b || !b2b || nspheres < 1) return fail(TG_ERR_ARG, "bad argument");
+    if (int rb_ = need_basis(plan, 0, "tg_flux_mhd")) return rb_;
     int rc = check_ptrs((const void *const *)V_dev, 3, "V");
     if (!rc) rc = check_ptrs((const void *const *)W_dev, 3, "W");
     if (!rc) rc = set_device(plan);
@@ -1067,6 +1306,7 @@ int tg_flux_mhd(tg_plan *plan, double *const V_dev[3], double *const W_dev[3], c
 int tg_rhs_vjp(tg_plan *plan, double *const V_dev[3], double *const lam_dev[3], double *const out_dev[3])
 {
     if (!plan) return fail(TG_ERR_ARG, "null plan");
+    if (int rb_ = need_basis(plan, 0, "tg_rhs_vjp")) return rb_;
     int rc = check_ptrs((const void *const *)V_dev, 3, "V");
     if (!rc) rc = check_ptrs((const void *const *)lam_dev, 3, "lam");
     if (!rc) rc = check_ptrs((const void *const *)out_dev, 3, "out");
@@ -1078,6 +1318,7 @@ int tg_rhs_vjp(tg_plan *plan, double *const V_dev[3], double *const lam_dev[3], 
 int tg_adjoint_forward(tg_plan *plan, double *const V_dev[3], const tg_fluid *fluid, int nsteps, int checkpoint_every)
 {
     if (!plan || !fluid) return fail(TG_ERR_ARG, "null argument");
+    if (int rb_ = need_basis(plan, 0, "tg_adjoint_forward")) return rb_;
     if (nsteps < 1 || checkpoint_every < 1) return fail(TG_ERR_ARG, "need nsteps >= 1 and checkpoint_every >= 1");
     if (!(fluid->dt > 0) || !(fluid->nu >= 0)) return fail(TG_ERR_ARG, "need dt > 0 and nu >= 0");
     if (fluid->force_kind != 0) return fail(TG_ERR_ARG, "the adjoint supports force_kind 0 only");
@@ -1140,7 +1381,7 @@ const char *tg_kernel_name(int kernel)
 {
     static const char *names[TG_KERNEL_COUNT] = {"x_inv", "y_fft", "z_products", "x_nlin_rk", "x_nlin_giver",
                                                  "reality", "z_r2c", "z_c2r", "x_fwd", "reduce", "x_inv_sym",
-                                                 "z_adj", "x_fwd_adj"};
+                                                 "z_adj", "x_fwd_adj", "x_sc", "z_rbc", "x_rbc"};
     return (kernel >= 0 && kernel < TG_KERNEL_COUNT) ? names[kernel] : "unknown";
 }
 
@@ -1148,6 +1389,7 @@ int tg_init_field(tg_plan *plan, int kind, double k0, double kmin, double kmax, 
                   double *const V_dev[3])
 {
     if (!plan || (kind != 0 && kind != 1) || !(energy >= 0)) return fail(TG_ERR_ARG, "bad argument");
+    if (int rb_ = need_basis(plan, 0, "tg_init_field")) return rb_;
     int rc = check_ptrs((const void *const *)V_dev, 3, "V");
     if (!rc) rc = set_device(plan);
     if (rc) return rc;

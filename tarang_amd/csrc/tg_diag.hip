@@ -83,14 +83,16 @@ __global__ void __launch_bounds__(kRedThreads) k_reduce(RedArgs a)
         const int lz = i % g.nzc;
         const int ly = (i / g.nzc) % g.ny;
         const int lx = i / ((size_t)g.nzc * g.ny);
+        // FFF: w = 1/2 on kz = 0 (FFF_pencil_energy.cc:59-68); SFF: 2 sum - kz=0
+        // plane - kx=0 plane + half the kx=0,kz=0 line (SFF_pencil_energy.cc:
+        // 88-104), i.e. w = (lx ? 2 : 1)(lz ? 2 : 1)/2
+        const double w = g.basis ? 0.5 * (lx ? 2.0 : 1.0) * (lz ? 2.0 : 1.0) : (lz == 0 ? 0.5 : 1.0);
         if (MODE == RED_ENERGY) {
             const double *A = a.A[0];
             const double re = A[2 * i], im = A[2 * i + 1];
-            const double w = lz == 0 ? 0.5 : 1.0;
             neumaier(s0, c0, w * (re * re + im * im));
         } else if (MODE == RED_CROSS) {
             const double *A = a.A[0], *B = a.B[0];
-            const double w = lz == 0 ? 0.5 : 1.0;
             neumaier(s0, c0, w * (A[2 * i] * B[2 * i] + A[2 * i + 1] * B[2 * i + 1]));
         } else if (MODE == RED_SN) {
             if (i == 0 && a.n == 0) continue;

@@ -13,6 +13,7 @@ struct Grid {
     int dealias;          // 0 none, 1 reference corner box, 2 two-thirds rule
     int nzp;              // kz stride of work fields and the RK accumulator:
                           // nzc rounded up to a multiple of 128 bytes
+    int basis;            // 0 FFF; 1 SFF (sin/cos along x: kx = lx, K_x = lx*kf[0])
 };
 
 // Forcing band slots: modes with |kx|,|ky| <= kb and 0 <= kz <= kb.

@@ -13,6 +13,20 @@ __device__ __forceinline__ int kwave(int l, int n) { return l <= n / 2 ? l : l -
 
 __device__ __forceinline__ bool dealias_zero(const Grid &g, int lx, int ly, int lz)
 {
+    if (g.basis == 1) {
+        // SFF_PENCIL::Dealias (SFF_pencil_basic.cc:94-97): lx in [2Nx/3+1, Nx-1],
+        // ly in [Ny/3+1, 2Ny/3-1], lz in [Nz/3+1, Nz/2] (empty y/z ranges
+        // clamped); the 2/3 mode zeroes lx > 2Nx/3, |ky| > Ny/3, lz > Nz/3.
+        if (g.dealias == 1) {
+            int ylo = g.ny / 3 + 1, yhi = 2 * g.ny / 3 - 1;
+            if (yhi < ylo) yhi = ylo;
+            int zlo = g.nz / 3 + 1, zhi = g.nz / 2;
+            if (zhi < zlo) zhi = zlo;
+            return lx >= 2 * g.nx / 3 + 1 && (ly >= ylo && ly <= yhi) && (lz >= zlo && lz <= zhi);
+        }
+        if (g.dealias == 2) return lx > 2 * g.nx / 3 || abs(kwave(ly, g.ny)) > g.ny / 3 || lz > g.nz / 3;
+        return false;
+    }
     if (g.dealias == 1) {
         // FFF_PENCIL::Dealias (FFF_pencil_basic.cc:87-90) via Assign_sub_array
         // (:228-259): intersection of lx in [Nx/3+1, 2Nx/3-1],
