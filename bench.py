@@ -21,6 +21,12 @@ Besides the headline FP64 number the line reports:
   cpu_baseline   the FP64 CPU oracle (restatement of the reference
              algorithm, own FFT, OpenMP) on one RK4 step of the same
              512^3 workload, on the host's cores;
+  rbc        Rayleigh-Benard convection on the SFF basis (sin/cos along x):
+             512 x 1024 x 1024 FP64, Ra = 1e8, Pr = 1 (nu = kappa = 1e-4),
+             theta = white noise of amplitude 1e-3 in the first 32 modes per
+             axis, u = 0, corner dealias, weak reality; RK4 steps/s, per-kernel
+             times, step roofline (8 (4 + 9) S + 11 * 4 S64 bytes per step) and
+             the Nusselt number;
   adjoint    the adjoint/forward cost ratio on its own workload: 256^3 FP64,
              200 unforced RK4 steps (no reality condition), checkpoints
              every 20 steps, loss = energy in the shell 8 <= |k| < 16 at the
@@ -49,6 +55,8 @@ def parse():
     ap.add_argument("--no-extra", action="store_true", help="headline FP64 number only")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint cost measurement")
     ap.add_argument("--adjoint-only", action="store_true", help="run only the adjoint cost measurement")
+    ap.add_argument("--no-rbc", action="store_true", help="skip the Rayleigh-Benard measurement")
+    ap.add_argument("--rbc-only", action="store_true", help="run only the Rayleigh-Benard measurement")
     ap.add_argument("--copy-calib", action="store_true", help="run a plain 1 GiB device copy (counter calibration)")
     return ap.parse_args()
 
@@ -103,6 +111,9 @@ def main():
 
     if args.adjoint_only:
         print(json.dumps({"adjoint": adjoint_cost(api, torch, np)}))
+        return
+    if args.rbc_only:
+        print(json.dumps({"rbc": rbc_rate(api, torch, np)}))
         return
 
     N = (args.n, args.n, args.n)
@@ -210,6 +221,10 @@ def main():
             del et
             torch.cuda.empty_cache()
             extra["adjoint"] = adjoint_cost(api, torch, np)
+            torch.cuda.empty_cache()
+        if not args.no_rbc:
+            extra["rbc"] = rbc_rate(api, torch, np)
+            torch.cuda.empty_cache()
 
     cpu = None
     if not args.no_cpu:
@@ -238,6 +253,55 @@ def main():
     }
     out.update(extra)
     print(json.dumps(out))
+
+
+def rbc_rate(api, torch, np, N=(512, 1024, 1024), steps=3, warmup=1, Ra=1e8, Pr=1.0, dt=1e-3, seed=6):
+    import ctypes
+    from tarang_amd._lib import KERNELS, check
+    b = api.SFF_Basis(N, precision=api.PREC_FP64, dealias=api.DEALIAS_REF_CORNER, reality=2)
+    U, T = api.FluidVF(b), api.FluidSF(b)
+    nx, ny, nzc = b.shape_complex_array
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(seed)
+    m = 32
+    rows = torch.cat([torch.arange(0, m), torch.arange(ny - m + 1, ny)]).to("cuda")
+    shape = (m - 1, rows.numel(), m)
+    noise = torch.complex(torch.randn(shape, generator=gen, device="cuda", dtype=torch.float64),
+                          torch.randn(shape, generator=gen, device="cuda", dtype=torch.float64))
+    T.F[1:m, rows, :m] = 1e-3 * noise
+    b.Satisfy_strong_reality_condition_in_Array(T.F)
+    ta = api.Time_advance_incompress_RBC(U, T, Ra, Pr, dt)
+    ta.Time_advance_step(warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ta.Time_advance_step(steps)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    L = b.L
+    check("tg_profile_reset", L.tg_profile_reset(b.plan))
+    check("tg_profile_enable", L.tg_profile_enable(b.plan, 1))
+    ta.Time_advance_step(1)
+    check("tg_profile_enable", L.tg_profile_enable(b.plan, 0))
+    kern = {}
+    for k, name in enumerate(KERNELS):
+        kms, cnt = ctypes.c_double(), ctypes.c_int64()
+        check("tg_profile_read", L.tg_profile_read(b.plan, k, ctypes.byref(kms), ctypes.byref(cnt)))
+        if cnt.value:
+            kern[name] = {"ms_per_step": round(kms.value, 3), "launches_per_step": cnt.value}
+    nu = api.Correlation.Get_Nusselt_no(U, T, Ra, Pr)
+    S = nx * ny * nzc * 16
+    model = 8 * (4 + 9) * S + 11 * 4 * S
+    print(f"[bench] rbc {N}: {ms:.1f} ms/step, Nu {nu:.12g}", file=sys.stderr, flush=True)
+    out = {"workload": f"SFF Rayleigh-Benard {N[0]}x{N[1]}x{N[2]} FP64, Ra={Ra:g}, Pr={Pr:g}, corner dealias, "
+                       "weak reality, theta noise 1e-3 in the first 32 modes",
+           "steps_per_s": round(1e3 / ms, 4), "ms_per_step": round(ms, 2), "steps": steps,
+           "step_roofline": {"bytes_model": model, "achieved": round(model / (ms * 1e-3) / 1e9, 1),
+                             "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                             "frac": round(model / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+           "kernels": kern, "nusselt": nu, "energy_theta": T.Compute_total_energy(),
+           "plan_GB": round(b.device_bytes() / 1e9, 1)}
+    del ta, U, T, b
+    return out
 
 
 def measured_traffic(kernel_class, N, precision):
