@@ -902,6 +902,12 @@ int tg_plan_create(const tg_config *cfg, tg_plan **plan)
     }
     for (int i = 0; i < 6 && !rc; i++) rc = dmalloc(p, &p->U[i], p->ncw * p->elem);
     for (int i = 0; i < 9 && !rc; i++) rc = dmalloc(p, &p->P[i], p->ncw * p->elem);
+    // the pad columns kz >= Nz/2+1 of the work fields are never written; the x
+    // passes load them with their tiles (and discard the results): keep them zero
+    for (int i = 0; i < 6 && !rc; i++)
+        if (hipMemset(p->U[i], 0, p->ncw * p->elem) != hipSuccess) rc = fail(TG_ERR_HIP, "hipMemset failed");
+    for (int i = 0; i < 9 && !rc; i++)
+        if (hipMemset(p->P[i], 0, p->ncw * p->elem) != hipSuccess) rc = fail(TG_ERR_HIP, "hipMemset failed");
     // accumulator: FP64, padded work layout (also holds giver outputs in the boundary layout)
     for (int i = 0; i < 3 && !rc; i++) rc = dmalloc(p, (void **)&p->acc[i], p->ncw * 16);
     p->red_blocks = (int)std::min<size_t>(2048, (p->nc + 1023) / 1024);

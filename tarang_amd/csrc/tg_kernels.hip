@@ -347,15 +347,19 @@ __global__ void __launch_bounds__(CB *(L / 8), 2) k_x_nlin(XnlArgs a)
 {
     constexpr int TPL = L / 8;
     constexpr int LP = lpad_len(L);
-    // two FP64 tile buffers: A stages inputs and hosts the FFT exchanges, B
-    // holds the state/accumulator tile of the RK update
-    __shared__ __attribute__((aligned(16))) char raw[2 * CB * LP * sizeof(cpx<double>)];
+    // two FP64 tile buffers (separate LDS objects, so that the compiler can
+    // tell accesses to one from the direct-to-LDS loads into the other): A
+    // hosts the FFT exchanges and stages state inputs; B receives the next
+    // product while the current one is transformed, then holds the
+    // state/accumulator tile of the RK update
+    __shared__ __attribute__((aligned(16))) char rawA[CB * LP * sizeof(cpx<double>)];
+    __shared__ __attribute__((aligned(16))) char rawB[CB * LP * sizeof(cpx<double>)];
     __shared__ int colw[CB], cols[CB];
     // twiddles are read from LDS: kept out of registers across the product loop
     __shared__ cpx<T> twl[L];
-    cpx<double> *ldd = (cpx<double> *)raw;
-    cpx<T> *ldt = (cpx<T> *)raw;
-    cpx<double> *ldb = (cpx<double> *)raw + CB * LP;
+    cpx<double> *ldd = (cpx<double> *)rawA;
+    cpx<T> *ldt = (cpx<T> *)rawA;
+    cpx<double> *ldb = (cpx<double> *)rawB;
     const Grid &g = a.g;
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     if (tile >= x_tiles(g, CB, false)) return;
@@ -381,6 +385,9 @@ __global__ void __launch_bounds__(CB *(L / 8), 2) k_x_nlin(XnlArgs a)
 #pragma unroll
     for (int s = 0; s < 8; s++) N[0][s] = N[1][s] = N[2][s] = mk<T>(0, 0);
     constexpr int NPROD = GIVER ? 9 : 6;
+    constexpr bool DMA = tile_dma_ok<T, L, CB>();
+    const int cw0 = tile * CB;  // first padded work column of the tile
+    if constexpr (DMA) dma_tile<T, L, CB>((const cpx<T> *)a.P[GIVER ? kGiverOrder[0] : kNsOrder[0]], cw0, ncw, rawB);
 #pragma unroll 1
     for (int pi = 0; pi < NPROD; pi++) {
         const int p = GIVER ? kGiverOrder[pi] : kNsOrder[pi];
@@ -400,16 +407,34 @@ __global__ void __launch_bounds__(CB *(L / 8), 2) k_x_nlin(XnlArgs a)
             sel1 = kNsSel[p][1];
             sel2 = kNsSel[p][2];
         }
-        stage_in<L, CB>(ldt, (const cpx<T> *)a.P[p], colw, ncw);
-        __syncthreads();
+        // the line index, hidden from loop-invariant code motion: the LDS
+        // addresses of the FFT passes are recomputed each product (a few
+        // integer ops) rather than held in registers across the loop, where
+        // their spill reloads would wait for the in-flight prefetch
+        int jj = j;
+        asm volatile("" : "+v"(jj));
         cpx<T> v[8];
+        if constexpr (DMA) {
+            // product p has landed in B; read it, then start the next one
+            dma_wait();
+            __syncthreads();
 #pragma unroll
-        for (int s = 0; s < 8; s++) v[s] = ldt[c * LP + lpad(j + TPL * s)];
-        __syncthreads();
-        fft_line<L, -1>(v, ldt + c * LP, tw, j);
+            for (int s = 0; s < 8; s++) v[s] = dma_tile_read<T, L, CB>(rawB, jj + TPL * s, c);
+            __syncthreads();
+            if (pi + 1 < NPROD)
+                dma_tile<T, L, CB>((const cpx<T> *)a.P[GIVER ? kGiverOrder[pi + 1] : kNsOrder[pi + 1]], cw0, ncw,
+                                   rawB);
+        } else {
+            stage_in<L, CB>(ldt, (const cpx<T> *)a.P[p], colw, ncw);
+            __syncthreads();
+#pragma unroll
+            for (int s = 0; s < 8; s++) v[s] = ldt[c * LP + lpad(j + TPL * s)];
+            __syncthreads();
+        }
+        fft_line<L, -1>(v, ldt + c * LP, tw, jj);
 #pragma unroll
         for (int s = 0; s < 8; s++) {
-            const T Kx = (T)(kwave(j + TPL * s, g.nx) * g.kf[0]);
+            const T Kx = (T)(kwave(jj + TPL * s, g.nx) * g.kf[0]);
             const cpx<T> q = sc * v[s];
             const T K0 = sel0 == 1 ? Kx : (sel0 == 2 ? Kyt : (sel0 == 3 ? Kzt : (T)0));
             const T K1 = sel1 == 1 ? Kx : (sel1 == 2 ? Kyt : (sel1 == 3 ? Kzt : (T)0));
@@ -418,8 +443,10 @@ __global__ void __launch_bounds__(CB *(L / 8), 2) k_x_nlin(XnlArgs a)
             N[1][s] = N[1][s] + mul_i(K1, q);
             N[2][s] = N[2][s] + mul_i(K2, q);
         }
-        __syncthreads();
+        if constexpr (!DMA) __syncthreads();
     }
+    // the FFT exchanges in A are done before A and B are reused below
+    if constexpr (DMA) __syncthreads();
 
     const int stage = a.rk.stage;
     const bool want_rhs = stage > 0 || a.rk.write_rhs;

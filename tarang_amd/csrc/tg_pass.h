@@ -130,6 +130,64 @@ __device__ __forceinline__ void tile_columns(int *colw, int *cols, const Grid &g
     }
 }
 
+// ---------------------------------------------------------------------------
+// Direct-to-LDS staging of an x tile (global_load_lds_dwordx4): the L x CB
+// tile of consecutive padded work columns cw0 .. cw0+CB-1 is moved in 1 KB
+// chunks, one per wave instruction (64 lanes x 16 bytes, lane l landing at
+// chunk + 16 l).  Within a chunk the layout is column-major, so the FFT's
+// reads of one column are consecutive:
+//   FP64 (16-byte elements): chunk = RPC = 64/CB rows x CB columns, lane
+//        c*RPC + r%RPC holds element (r, c);
+//   FP32 (8-byte elements): chunk = RPC2 = 128/CB rows x CB columns, lane
+//        (c/2)*RPC2 + r%RPC2 holds the pair (r, c & ~1), (r, c | 1).
+// Lanes of one instruction read CB adjacent columns of RPC (RPC2) rows: the
+// same 64-byte row segments as a row-major staging.  The tile must fill whole
+// waves and whole chunks (tile_dma_ok); the columns are always inside the
+// padded rows, so pad columns are loaded too (and ignored).
+// ---------------------------------------------------------------------------
+template <typename T, int L, int CB>
+__host__ __device__ constexpr bool tile_dma_ok()
+{
+    return (CB * (L / 8)) % 64 == 0 && (L * CB * (int)sizeof(cpx<T>)) % 1024 == 0 &&
+           (sizeof(T) == 8 || CB % 2 == 0);
+}
+
+template <typename T, int L, int CB>
+__device__ __forceinline__ void dma_tile(const cpx<T> *__restrict__ f, int cw0, int ncol_total, char *lds)
+{
+    constexpr int NWAVE = CB * (L / 8) / 64;
+    constexpr int NCHUNK = L * CB * (int)sizeof(cpx<T>) / 1024;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int k = wave; k < NCHUNK; k += NWAVE) {
+        const cpx<T> *src;
+        if constexpr (sizeof(T) == 8) {
+            constexpr int RPC = 64 / CB;
+            src = f + (size_t)(k * RPC + lane % RPC) * ncol_total + cw0 + lane / RPC;
+        } else {
+            constexpr int RPC2 = 128 / CB;
+            src = f + (size_t)(k * RPC2 + lane % RPC2) * ncol_total + cw0 + 2 * (lane / RPC2);
+        }
+        __builtin_amdgcn_global_load_lds((const void *)src, (__attribute__((address_space(3))) void *)(lds + k * 1024),
+                                         16, 0, 0);
+    }
+}
+
+template <typename T, int L, int CB>
+__device__ __forceinline__ cpx<T> dma_tile_read(const char *lds, int r, int c)
+{
+    if constexpr (sizeof(T) == 8) {
+        constexpr int RPC = 64 / CB;
+        return ((const cpx<T> *)lds)[(r / RPC) * 64 + c * RPC + (r % RPC)];
+    } else {
+        constexpr int RPC2 = 128 / CB;
+        return *(const cpx<T> *)(lds + (r / RPC2) * 1024 + ((c / 2) * RPC2 + (r % RPC2)) * 16 + (c % 2) * 8);
+    }
+}
+
+// Waits for this wave's direct-to-LDS loads (the barrier that follows makes
+// every wave's loads visible).
+__device__ __forceinline__ void dma_wait() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+
 __device__ __forceinline__ int band_slot(const Band &b, int kx, int ky, int kz)
 {
     if (kx < -b.kb || kx > b.kb || ky < -b.kb || ky > b.kb || kz > b.kb) return -1;

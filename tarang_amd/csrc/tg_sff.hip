@@ -28,22 +28,13 @@ __device__ __forceinline__ int mk_perm(int n, int N) { return n < N / 2 ? 2 * n 
 template <typename T>
 __device__ __forceinline__ cpx<T> times_i(cpx<T> z) { return mk<T>(-z.y, z.x); }
 
-// Forward sin/cos transform of the line held in natural order in the LDS
-// line buffer `col`; on return v[s] = 2N A_k at k = j + TPL*s.  The caller
-// synchronises before (col written by other threads) and after (col reused).
+// The core: v[s] holds the reordered (and, for sine, sign-alternated) input
+// at Makhoul position j + TPL*s; `col` is a free LDS line buffer.
 template <int L, typename T>
-__device__ __forceinline__ void sc_forward(cpx<T> *v, cpx<T> *col, const cpx<T> *tw, const cpx<T> *twq, int j,
-                                           bool sine)
+__device__ __forceinline__ void sc_forward_core(cpx<T> *v, cpx<T> *col, const cpx<T> *tw, const cpx<T> *twq, int j,
+                                                bool sine)
 {
     constexpr int TPL = L / 8;
-#pragma unroll
-    for (int s = 0; s < 8; s++) {
-        const int n = j + TPL * s;
-        cpx<T> x = col[lpad(mk_perm(n, L))];
-        if (sine && n >= L / 2) x = mk<T>(-x.x, -x.y);
-        v[s] = x;
-    }
-    line_sync<TPL>();
     fft_line<L, -1>(v, col, tw, j);
 #pragma unroll
     for (int s = 0; s < 8; s++) col[lpad(j + TPL * s)] = v[s];
@@ -59,6 +50,32 @@ __device__ __forceinline__ void sc_forward(cpx<T> *v, cpx<T> *col, const cpx<T> 
         else v[s] = k == 0 ? mk<T>(0, 0) : times_i(p - q);
     }
     line_sync<TPL>();
+}
+
+// Makhoul input gather of sample mk_perm(n) from `load(sample)`.
+template <int L, typename T, typename Load>
+__device__ __forceinline__ void sc_gather(cpx<T> *v, int j, bool sine, Load load)
+{
+    constexpr int TPL = L / 8;
+#pragma unroll
+    for (int s = 0; s < 8; s++) {
+        const int n = j + TPL * s;
+        cpx<T> x = load(mk_perm(n, L));
+        if (sine && n >= L / 2) x = mk<T>(-x.x, -x.y);
+        v[s] = x;
+    }
+}
+
+// Forward sin/cos transform of the line held in natural order in the LDS
+// line buffer `col`; on return v[s] = 2N A_k at k = j + TPL*s.  The caller
+// synchronises before (col written by other threads) and after (col reused).
+template <int L, typename T>
+__device__ __forceinline__ void sc_forward(cpx<T> *v, cpx<T> *col, const cpx<T> *tw, const cpx<T> *twq, int j,
+                                           bool sine)
+{
+    sc_gather<L, T>(v, j, sine, [&](int r) { return col[lpad(r)]; });
+    line_sync<L / 8>();
+    sc_forward_core<L>(v, col, tw, twq, j, sine);
 }
 
 // Inverse: A_k in v[s] (k = j + TPL*s); on return the samples are in `col`
@@ -344,17 +361,24 @@ __device__ __forceinline__ void rbc_lawson(const Grid &g, const RbcParams &rp, i
     __syncthreads();
 }
 
-template <typename T, int L, int CB>
+// PHASE 0: temperature (N_T, its right-hand side, Lawson for theta);
+// PHASE 1: velocity.  The two launches of a stage touch disjoint data (the
+// force inputs they read are the other field's, of this stage's input).
+template <typename T, int L, int CB, int PHASE>
 __global__ void __launch_bounds__(CB *(L / 8), 2) k_x_rbc(XrbcArgs a)
 {
     constexpr int TPL = L / 8;
     constexpr int LP = lpad_len(L);
-    __shared__ __attribute__((aligned(16))) char raw[2 * CB * LP * sizeof(cpx<double>)];
+    // A: FFT exchanges and state staging; B: products arriving by
+    // direct-to-LDS loads, then the accumulator tile (separate LDS objects)
+    __shared__ __attribute__((aligned(16))) char rawA[CB * LP * sizeof(cpx<double>)];
+    __shared__ __attribute__((aligned(16))) char rawB[CB * LP * sizeof(cpx<double>)];
     __shared__ int colw[CB], cols[CB];
     __shared__ cpx<T> twl[L], twql[L];
-    cpx<double> *ldd = (cpx<double> *)raw;
-    cpx<T> *ldt = (cpx<T> *)raw;
-    cpx<double> *ldb = (cpx<double> *)raw + CB * LP;
+    cpx<double> *ldd = (cpx<double> *)rawA;
+    cpx<T> *ldt = (cpx<T> *)rawA;
+    cpx<double> *ldb = (cpx<double> *)rawB;
+    constexpr bool DMA = tile_dma_ok<T, L, CB>();
     const Grid &g = a.g;
     const RbcParams &rp = a.rp;
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
@@ -374,26 +398,47 @@ __global__ void __launch_bounds__(CB *(L / 8), 2) k_x_rbc(XrbcArgs a)
     const int ncw = g.ny * g.nzp, ncs = g.ny * g.nzc;
     const int stage = rp.stage;
     const bool want_rhs = stage > 0 || rp.write_rhs;
+    const int cw0 = tile * CB;
+    // Product p of a phase into registers v (Makhoul order, sign-alternated
+    // for sine products): from B after its direct-to-LDS load (then the load
+    // of product p_next starts), or staged through A.
+    auto load_product = [&](cpx<T> *v, int p, int p_next, int jj, bool sine) {
+        if constexpr (DMA) {
+            dma_wait();
+            __syncthreads();
+            sc_gather<L, T>(v, jj, sine, [&](int r) { return dma_tile_read<T, L, CB>(rawB, r, c); });
+            __syncthreads();
+            if (p_next >= 0) dma_tile<T, L, CB>((const cpx<T> *)sel_ptr(a.P, p_next, 9), cw0, ncw, rawB);
+        } else {
+            stage_in<L, CB>(ldt, (const cpx<T> *)sel_ptr(a.P, p, 9), colw, ncw);
+            __syncthreads();
+            sc_gather<L, T>(v, jj, sine, [&](int r) { return ldt[c * LP + lpad(r)]; });
+            __syncthreads();
+        }
+    };
 
     // ===== temperature: N_T = -Kx F_C(u_x T) + i Ky F_S(u_y T) + i Kz F_S(u_z T) =====
-    {
+    if constexpr (PHASE == 0) {
         cpx<T> nt[8];
 #pragma unroll
         for (int s = 0; s < 8; s++) nt[s] = mk<T>(0, 0);
+        if constexpr (DMA) dma_tile<T, L, CB>((const cpx<T> *)a.P[6], cw0, ncw, rawB);
 #pragma unroll 1
         for (int p = 6; p < 9; p++) {
-            stage_in<L, CB>(ldt, (const cpx<T> *)sel_ptr(a.P, p, 9), colw, ncw);
-            __syncthreads();
+            int jj = j;  // hidden from loop-invariant code motion (see k_x_nlin)
+            asm volatile("" : "+v"(jj));
             cpx<T> v[8];
-            sc_forward<L>(v, ldt + c * LP, twl, twql, j, p != 6);
+            load_product(v, p, p < 8 ? p + 1 : -1, jj, p != 6);
+            sc_forward_core<L>(v, ldt + c * LP, twl, twql, jj, p != 6);
             const int sel = p == 6 ? 2 : (p == 7 ? 3 : 4);
 #pragma unroll
             for (int s = 0; s < 8; s++) {
-                const T Kx = (T)((j + TPL * s) * g.kf[0]);
+                const T Kx = (T)((jj + TPL * s) * g.kf[0]);
                 nt[s] = nt[s] + rbc_term<T>(sel, Kx, Kyt, Kzt, sc * v[s]);
             }
-            __syncthreads();
+            if constexpr (!DMA) __syncthreads();
         }
+        if constexpr (DMA) __syncthreads();
         cpx<double> r[8];
         if (want_rhs) {
             // RHS_T = -(N_T - F_T), F_T = u_x of the stage input
@@ -421,27 +466,31 @@ __global__ void __launch_bounds__(CB *(L / 8), 2) k_x_rbc(XrbcArgs a)
         }
     }
 
+    if constexpr (PHASE == 0) return;
     // ===== velocity: divergence-form N with the SFF parities =====
     cpx<T> N[3][8];
 #pragma unroll
     for (int s = 0; s < 8; s++) N[0][s] = N[1][s] = N[2][s] = mk<T>(0, 0);
+    if constexpr (DMA) dma_tile<T, L, CB>((const cpx<T> *)a.P[0], cw0, ncw, rawB);
 #pragma unroll 1
     for (int p = 0; p < 6; p++) {
-        stage_in<L, CB>(ldt, (const cpx<T> *)sel_ptr(a.P, p, 9), colw, ncw);
-        __syncthreads();
+        int jj = j;
+        asm volatile("" : "+v"(jj));
         cpx<T> v[8];
-        sc_forward<L>(v, ldt + c * LP, twl, twql, j, p == 3 || p == 5);
+        load_product(v, p, p < 5 ? p + 1 : -1, jj, p == 3 || p == 5);
+        sc_forward_core<L>(v, ldt + c * LP, twl, twql, jj, p == 3 || p == 5);
         const int s0 = rbc_sel(p, 0), s1 = rbc_sel(p, 1), s2 = rbc_sel(p, 2);
 #pragma unroll
         for (int s = 0; s < 8; s++) {
-            const T Kx = (T)((j + TPL * s) * g.kf[0]);
+            const T Kx = (T)((jj + TPL * s) * g.kf[0]);
             const cpx<T> q = sc * v[s];
             N[0][s] = N[0][s] + rbc_term<T>(s0, Kx, Kyt, Kzt, q);
             N[1][s] = N[1][s] + rbc_term<T>(s1, Kx, Kyt, Kzt, q);
             N[2][s] = N[2][s] + rbc_term<T>(s2, Kx, Kyt, Kzt, q);
         }
-        __syncthreads();
+        if constexpr (!DMA) __syncthreads();
     }
+    if constexpr (DMA) __syncthreads();
     if (want_rhs) {
         // nlin -= F (F_x = theta of the stage input); p = div/K^2 with the SFF
         // divergence (+Kx on the 'S' x component); nlin += grad p ('C' parity:
@@ -471,9 +520,19 @@ __global__ void __launch_bounds__(CB *(L / 8), 2) k_x_rbc(XrbcArgs a)
     }
 #pragma unroll 1
     for (int comp = 0; comp < 3; comp++) {
+        // explicit branches: a select on comp would be folded into a
+        // dynamically indexed N[comp][s], which moves N to scratch memory
         cpx<double> r[8];
+        if (comp == 0) {
 #pragma unroll
-        for (int s = 0; s < 8; s++) r[s] = cvt<double>(comp == 0 ? N[0][s] : (comp == 1 ? N[1][s] : N[2][s]));
+            for (int s = 0; s < 8; s++) r[s] = cvt<double>(N[0][s]);
+        } else if (comp == 1) {
+#pragma unroll
+            for (int s = 0; s < 8; s++) r[s] = cvt<double>(N[1][s]);
+        } else {
+#pragma unroll
+            for (int s = 0; s < 8; s++) r[s] = cvt<double>(N[2][s]);
+        }
         if (stage == 0) {
 #pragma unroll
             for (int s = 0; s < 8; s++) ldd[c * LP + lpad(j + TPL * s)] = r[s];
@@ -610,7 +669,9 @@ hipError_t launch_x_rbc(const Grid &g, const RbcParams &rp, void *const *P, doub
     a.rp = rp;
     TG_DISPATCH_L(g.nx, {
         constexpr int CB = col_block<T>(LL, 2 * sizeof(cpx<double>));
-        hipLaunchKernelGGL((k_x_rbc<T, LL, CB>), dim3(x_grid(x_tiles(g, CB, false))), dim3(CB * (LL / 8)), 0, st, a);
+        const dim3 grid(x_grid(x_tiles(g, CB, false)));
+        hipLaunchKernelGGL((k_x_rbc<T, LL, CB, 0>), grid, dim3(CB * (LL / 8)), 0, st, a);
+        hipLaunchKernelGGL((k_x_rbc<T, LL, CB, 1>), grid, dim3(CB * (LL / 8)), 0, st, a);
     });
     return hipGetLastError();
 }
