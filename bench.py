@@ -193,14 +193,28 @@ def main():
             ta32.Time_advance_step(1)
         torch.cuda.synchronize()
         ms32 = (time.perf_counter() - t0) * 1e3 / max(args.steps, 1)
+        L32 = b32.L
+        check("tg_profile_reset", L32.tg_profile_reset(b32.plan))
+        check("tg_profile_enable", L32.tg_profile_enable(b32.plan, 1))
         ta32.Time_advance_step(prof_steps)
+        check("tg_profile_enable", L32.tg_profile_enable(b32.plan, 0))
+        kern32 = {}
+        for k, name in enumerate(KERNELS):
+            kms, cnt = ctypes.c_double(), ctypes.c_int64()
+            check("tg_profile_read", L32.tg_profile_read(b32.plan, k, ctypes.byref(kms), ctypes.byref(cnt)))
+            if cnt.value:
+                kern32[name] = {"ms_per_step": round(kms.value / prof_steps, 3),
+                                "launches_per_step": cnt.value // prof_steps,
+                                "GBps": round(algorithmic_bytes(name, N, 4) / (kms.value / prof_steps * 1e-3) / 1e9, 1)
+                                if algorithmic_bytes(name, N, 4) else None}
         E32 = U32.Compute_total_energy()
         print(f"[bench] fp32 {args.steps} steps: {ms32:.3f} ms/step", file=sys.stderr, flush=True)
         step_model32 = step_bytes_model(N, 4)
         extra["fp32"] = {"steps_per_s": round(1e3 / ms32, 3), "ms_per_step": round(ms32, 3),
                          "speedup_vs_fp64": round(ms_step / ms32, 3),
                          "energy_rel_err_vs_fp64": abs(E32 - E64) / abs(E64), "steps_compared": nsteps_total,
-                         "step_roofline_frac": round(step_model32 / (ms32 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+                         "step_roofline_frac": round(step_model32 / (ms32 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                         "kernels": kern32}
         del ta32, U32, b32
         torch.cuda.empty_cache()
         # energy-transfer diagnostics on the FP64 state (20 spheres / 20 shells)
