@@ -208,11 +208,19 @@ def main():
                                 "GBps": round(algorithmic_bytes(name, N, 4) / (kms.value / prof_steps * 1e-3) / 1e9, 1)
                                 if algorithmic_bytes(name, N, 4) else None}
         E32 = U32.Compute_total_energy()
+        # energy flux through the 20 default spheres, FP32 transforms vs FP64,
+        # after the same steps from the same field
+        radii = b64.default_radii(20)
+        flux64 = api.EnergyTr(b64, radii, radii).Compute_flux(U64)
+        flux32 = api.EnergyTr(b32, radii, radii).Compute_flux(U32)
+        fin = np.isfinite(radii[1:])
+        flux_err = float(np.max(np.abs(flux32[1:][fin] - flux64[1:][fin])) / np.max(np.abs(flux64[1:][fin])))
         print(f"[bench] fp32 {args.steps} steps: {ms32:.3f} ms/step", file=sys.stderr, flush=True)
         step_model32 = step_bytes_model(N, 4)
         extra["fp32"] = {"steps_per_s": round(1e3 / ms32, 3), "ms_per_step": round(ms32, 3),
                          "speedup_vs_fp64": round(ms_step / ms32, 3),
                          "energy_rel_err_vs_fp64": abs(E32 - E64) / abs(E64), "steps_compared": nsteps_total,
+                         "flux_rel_err_vs_fp64": flux_err,
                          "step_roofline_frac": round(step_model32 / (ms32 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                          "kernels": kern32}
         del ta32, U32, b32

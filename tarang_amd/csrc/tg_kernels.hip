@@ -558,7 +558,8 @@ __global__ void __launch_bounds__(CB *(L / 8), 2) k_x_nlin(XnlArgs a)
                 vn = E1 * v0 + ac;
             }
             // accumulator (stages 1-3) or the new state (stage 4) back to B
-            ldb[li] = stage == 4 ? vn : ac;
+            // (component-wise select: selecting the objects keeps them in memory)
+            ldb[li] = mk<double>(stage == 4 ? vn.x : ac.x, stage == 4 ? vn.y : ac.y);
             // next stage input: band slots, dealias
             if (a.b.kind == 1 && cw >= 0) {
                 const int kx = kwave(lx, g.nx);

@@ -185,8 +185,10 @@ __device__ __forceinline__ cpx<T> dma_tile_read(const char *lds, int r, int c)
 }
 
 // Waits for this wave's direct-to-LDS loads (the barrier that follows makes
-// every wave's loads visible).
-__device__ __forceinline__ void dma_wait() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+// every wave's loads visible): s_waitcnt vmcnt(0) with expcnt and lgkmcnt at
+// their maxima.  The builtin (not inline asm) lets the compiler's wait
+// insertion see that the loads have retired.
+__device__ __forceinline__ void dma_wait() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
 __device__ __forceinline__ int band_slot(const Band &b, int kx, int ky, int kz)
 {

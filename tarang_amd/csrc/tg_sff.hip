@@ -15,7 +15,10 @@
 //   cosine inverse   V_k = W^-k (A_k - i A_{N-k}), v = IFFT(V) (unscaled)
 //   sine inverse     V_k = W^-k (A_{N-k} - i A_k), V_0 = 0, then (-1)^j
 // with W = exp(-i pi / (2N)) (table twq) and the (k, N-k) pairs exchanged
-// through the LDS line buffer.  Along y and z the FFF passes are reused.
+// through the LDS line buffer.  A thread holds k = j + TPL*s (s < 8), and
+// every pass forms W^k = W^j W^(TPL s) from those TPL + 8 table entries
+// (QuarterTw), so all sin/cos passes round alike and the fused kernels keep
+// only 8 + TPL entries in LDS.  Along y and z the FFF passes are reused.
 #include "tg_device.h"
 #include "tg_pass.h"
 #include "tg_sff.h"
@@ -28,11 +31,18 @@ __device__ __forceinline__ int mk_perm(int n, int N) { return n < N / 2 ? 2 * n 
 template <typename T>
 __device__ __forceinline__ cpx<T> times_i(cpx<T> z) { return mk<T>(-z.y, z.x); }
 
+// Post-twiddle W^(j + TPL*s) for the sin/cos line transforms.
+template <typename T>
+struct QuarterTw {
+    const cpx<T> *lo;  // W^j, j < TPL
+    const cpx<T> *hi;  // W^(TPL s), s < 8
+    __device__ __forceinline__ cpx<T> operator()(int j, int s) const { return lo[j] * hi[s]; }
+};
+
 // The core: v[s] holds the reordered (and, for sine, sign-alternated) input
 // at Makhoul position j + TPL*s; `col` is a free LDS line buffer.
-template <int L, typename T>
-__device__ __forceinline__ void sc_forward_core(cpx<T> *v, cpx<T> *col, const cpx<T> *tw, const cpx<T> *twq, int j,
-                                                bool sine)
+template <int L, typename T, typename Q>
+__device__ __forceinline__ void sc_forward_core(cpx<T> *v, cpx<T> *col, const cpx<T> *tw, Q twq, int j, bool sine)
 {
     constexpr int TPL = L / 8;
     fft_line<L, -1>(v, col, tw, j);
@@ -43,7 +53,7 @@ __device__ __forceinline__ void sc_forward_core(cpx<T> *v, cpx<T> *col, const cp
     for (int s = 0; s < 8; s++) {
         const int k = j + TPL * s;
         const cpx<T> vn = col[lpad((L - k) & (L - 1))];
-        const cpx<T> w = twq[k];
+        const cpx<T> w = twq(j, s);
         const cpx<T> p = w * v[s];
         const cpx<T> q = conj(w) * vn;
         if (!sine) v[s] = p + q;
@@ -69,9 +79,8 @@ __device__ __forceinline__ void sc_gather(cpx<T> *v, int j, bool sine, Load load
 // Forward sin/cos transform of the line held in natural order in the LDS
 // line buffer `col`; on return v[s] = 2N A_k at k = j + TPL*s.  The caller
 // synchronises before (col written by other threads) and after (col reused).
-template <int L, typename T>
-__device__ __forceinline__ void sc_forward(cpx<T> *v, cpx<T> *col, const cpx<T> *tw, const cpx<T> *twq, int j,
-                                           bool sine)
+template <int L, typename T, typename Q>
+__device__ __forceinline__ void sc_forward(cpx<T> *v, cpx<T> *col, const cpx<T> *tw, Q twq, int j, bool sine)
 {
     sc_gather<L, T>(v, j, sine, [&](int r) { return col[lpad(r)]; });
     line_sync<L / 8>();
@@ -80,9 +89,8 @@ __device__ __forceinline__ void sc_forward(cpx<T> *v, cpx<T> *col, const cpx<T> 
 
 // Inverse: A_k in v[s] (k = j + TPL*s); on return the samples are in `col`
 // in natural order (the caller synchronises before reading them).
-template <int L, typename T>
-__device__ __forceinline__ void sc_inverse(cpx<T> *v, cpx<T> *col, const cpx<T> *tw, const cpx<T> *twq, int j,
-                                           bool sine)
+template <int L, typename T, typename Q>
+__device__ __forceinline__ void sc_inverse(cpx<T> *v, cpx<T> *col, const cpx<T> *tw, Q twq, int j, bool sine)
 {
     constexpr int TPL = L / 8;
 #pragma unroll
@@ -92,9 +100,12 @@ __device__ __forceinline__ void sc_inverse(cpx<T> *v, cpx<T> *col, const cpx<T> 
     for (int s = 0; s < 8; s++) {
         const int k = j + TPL * s;
         const cpx<T> an = k == 0 ? mk<T>(0, 0) : col[lpad(L - k)];
-        const cpx<T> p = sine ? an : v[s];
-        const cpx<T> q = sine ? v[s] : an;
-        cpx<T> z = conj(twq[k]) * mk<T>(p.x + q.y, p.y - q.x);  // W^-k (p - i q)
+        const cpx<T> ak = v[s];
+        // component-wise selects (a select of the two values as objects keeps
+        // v[] out of registers)
+        const cpx<T> p = mk<T>(sine ? an.x : ak.x, sine ? an.y : ak.y);
+        const cpx<T> q = mk<T>(sine ? ak.x : an.x, sine ? ak.y : an.y);
+        cpx<T> z = conj(twq(j, s)) * mk<T>(p.x + q.y, p.y - q.x);  // W^-k (p - i q)
         if (sine && k == 0) z = mk<T>(0, 0);
         v[s] = z;
     }
@@ -132,15 +143,18 @@ __global__ void __launch_bounds__(CB *(L / 8)) k_xinv_sc(XscArgs a)
     cpx<double> *ldd = (cpx<double> *)raw;
     cpx<T> *ldt = (cpx<T> *)raw;
     const Grid &g = a.g;
+    __shared__ cpx<T> qhi[8];
     const bool patch = (a.flags & 4) != 0;
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     if (tile >= x_tiles(g, CB, patch)) return;
     tile_columns<CB>(colw, cols, g, patch, tile);
+    if (threadIdx.x < 8) qhi[threadIdx.x] = ((const cpx<T> *)a.twq)[(L / 8) * threadIdx.x];
     __syncthreads();
     const int c = threadIdx.x / TPL, j = threadIdx.x % TPL;
     const int cw = colw[c];
     const int ly = cw >= 0 ? cw / g.nzp : 0, lz = cw >= 0 ? cw % g.nzp : 0;
-    const cpx<T> *tw = (const cpx<T> *)a.tw, *twq = (const cpx<T> *)a.twq;
+    const cpx<T> *tw = (const cpx<T> *)a.tw;
+    const QuarterTw<T> twq{(const cpx<T> *)a.twq, qhi};
     const int ncw = g.ny * g.nzp, ncs = g.ny * g.nzc;
     for (int comp = 0; comp < a.ncomp; comp++) {
         stage_in<L, CB>(ldd, (const cpx<double> *)a.V[comp], cols, ncs);
@@ -171,12 +185,15 @@ __global__ void __launch_bounds__(CB *(L / 8)) k_xfwd_sc(XscArgs a)
     cpx<double> *ldd = (cpx<double> *)raw;
     cpx<T> *ldt = (cpx<T> *)raw;
     const Grid &g = a.g;
+    __shared__ cpx<T> qhi[8];
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     if (tile >= x_tiles(g, CB, false)) return;
     tile_columns<CB>(colw, cols, g, false, tile);
+    if (threadIdx.x < 8) qhi[threadIdx.x] = ((const cpx<T> *)a.twq)[(L / 8) * threadIdx.x];
     __syncthreads();
     const int c = threadIdx.x / TPL, j = threadIdx.x % TPL;
-    const cpx<T> *tw = (const cpx<T> *)a.tw, *twq = (const cpx<T> *)a.twq;
+    const cpx<T> *tw = (const cpx<T> *)a.tw;
+    const QuarterTw<T> twq{(const cpx<T> *)a.twq, qhi};
     const int ncw = g.ny * g.nzp, ncs = g.ny * g.nzc;
     for (int comp = 0; comp < a.ncomp; comp++) {
         stage_in<L, CB>(ldt, (const cpx<T> *)a.U[comp], colw, ncw);
@@ -267,16 +284,22 @@ struct XrbcArgs {
 
 // Contribution of a transformed product q to one output component:
 // sel 0 none, 1 +Kx q (d/dx of an 'S' product), 2 -Kx q ('C' product),
-// 3 i Ky q, 4 i Kz q (SFF_pencil_tr.cc:77-151).
+// 3 i Ky q, 4 i Kz q (SFF_pencil_tr.cc:77-151).  Branch-free form: the
+// term is a q + i b q with a = ax Kx, b = ay Ky + az Kz and one of ax, ay,
+// az equal to +-1 (the others 0), so every term keeps its exact value.
 template <typename T>
-__device__ __forceinline__ cpx<T> rbc_term(int sel, T Kx, T Ky, T Kz, cpx<T> q)
-{
-    if (sel == 1) return Kx * q;
-    if (sel == 2) return (-Kx) * q;
-    if (sel == 3) return mul_i(Ky, q);
-    if (sel == 4) return mul_i(Kz, q);
-    return mk<T>(0, 0);
-}
+struct RbcTerm {
+    T ax, b;
+    __device__ __forceinline__ RbcTerm(int sel, T Ky, T Kz)
+        : ax(sel == 1 ? T(1) : (sel == 2 ? T(-1) : T(0))), b(sel == 3 ? Ky : (sel == 4 ? Kz : T(0)))
+    {
+    }
+    __device__ __forceinline__ cpx<T> operator()(T Kx, cpx<T> q) const
+    {
+        const T a = ax * Kx;
+        return mk<T>(a * q.x - b * q.y, a * q.y + b * q.x);
+    }
+};
 
 // Selectors of the velocity products (xx C, yy C, zz C, xy S, yz C, zx S)
 // for N_x, N_y, N_z in the reference's accumulation order.
@@ -298,7 +321,7 @@ template <typename T, int L, int CB>
 __device__ __forceinline__ void rbc_lawson(const Grid &g, const RbcParams &rp, int c, int j, int ly, int lz,
                                            double Ky, double Kz, const int *colw, const int *cols,
                                            cpx<double> *ldd, cpx<double> *ldb, cpx<T> *ldt, const cpx<T> *tw,
-                                           const cpx<T> *twq, const cpx<double> *r, double fh, double f1,
+                                           QuarterTw<T> twq, const cpx<double> *r, double fh, double f1,
                                            double *V, double *acc, void *U, double *fo, bool sine)
 {
     constexpr int TPL = L / 8;
@@ -312,7 +335,6 @@ __device__ __forceinline__ void rbc_lawson(const Grid &g, const RbcParams &rp, i
     if (stage >= 2) stage_in<L, CB>(ldb, (const cpx<double> *)acc, colw, ncw);
     __syncthreads();
     cpx<T> v[8];
-    cpx<double> pre[8];
 #pragma unroll
     for (int s = 0; s < 8; s++) {
         const int lx = j + TPL * s;
@@ -339,8 +361,11 @@ __device__ __forceinline__ void rbc_lawson(const Grid &g, const RbcParams &rp, i
             const double E1 = exp(f1 * ks);
             vn = E1 * v0 + ac;
         }
-        ldb[li] = stage == 4 ? vn : ac;
-        pre[s] = vn;
+        // component-wise select (selecting the objects keeps them in memory)
+        ldb[li] = mk<double>(stage == 4 ? vn.x : ac.x, stage == 4 ? vn.y : ac.y);
+        // the undealiased next-stage value (this field's share of the next
+        // force) replaces v0 in the thread's own slot of the state tile
+        if (fo) ldd[li] = vn;
         if (dealias_zero(g, lx, ly, lz)) vn = mk<double>(0, 0);
         v[s] = cvt<T>(vn);
     }
@@ -348,12 +373,8 @@ __device__ __forceinline__ void rbc_lawson(const Grid &g, const RbcParams &rp, i
     if (stage == 4) stage_out<L, CB>(ldb, (cpx<double> *)V, cols, ncs);
     else stage_out<L, CB>(ldb, (cpx<double> *)acc, colw, ncw);
     if (fo) {
-        // the undealiased next-stage value: this field's share of the next force
+        stage_out<L, CB>(ldd, (cpx<double> *)fo, cols, ncs);
         __syncthreads();
-#pragma unroll
-        for (int s = 0; s < 8; s++) ldb[c * LP + lpad(j + TPL * s)] = pre[s];
-        __syncthreads();
-        stage_out<L, CB>(ldb, (cpx<double> *)fo, cols, ncs);
     }
     sc_inverse<L>(v, ldt + c * LP, tw, twq, j, sine);
     __syncthreads();
@@ -374,7 +395,7 @@ __global__ void __launch_bounds__(CB *(L / 8), 2) k_x_rbc(XrbcArgs a)
     __shared__ __attribute__((aligned(16))) char rawA[CB * LP * sizeof(cpx<double>)];
     __shared__ __attribute__((aligned(16))) char rawB[CB * LP * sizeof(cpx<double>)];
     __shared__ int colw[CB], cols[CB];
-    __shared__ cpx<T> twl[L], twql[L];
+    __shared__ cpx<T> twl[L], twqlo[L / 8], twqhi[8];
     cpx<double> *ldd = (cpx<double> *)rawA;
     cpx<T> *ldt = (cpx<T> *)rawA;
     cpx<double> *ldb = (cpx<double> *)rawB;
@@ -384,10 +405,10 @@ __global__ void __launch_bounds__(CB *(L / 8), 2) k_x_rbc(XrbcArgs a)
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     if (tile >= x_tiles(g, CB, false)) return;
     tile_columns<CB>(colw, cols, g, false, tile);
-    for (int i = threadIdx.x; i < L; i += CB * TPL) {
-        twl[i] = ((const cpx<T> *)a.tw)[i];
-        twql[i] = ((const cpx<T> *)a.twq)[i];
-    }
+    for (int i = threadIdx.x; i < L; i += CB * TPL) twl[i] = ((const cpx<T> *)a.tw)[i];
+    if (threadIdx.x < TPL) twqlo[threadIdx.x] = ((const cpx<T> *)a.twq)[threadIdx.x];
+    if (threadIdx.x < 8) twqhi[threadIdx.x] = ((const cpx<T> *)a.twq)[TPL * threadIdx.x];
+    const QuarterTw<T> twql{twqlo, twqhi};
     __syncthreads();
     const int c = threadIdx.x / TPL, j = threadIdx.x % TPL;
     const int cw = colw[c];
@@ -430,11 +451,11 @@ __global__ void __launch_bounds__(CB *(L / 8), 2) k_x_rbc(XrbcArgs a)
             cpx<T> v[8];
             load_product(v, p, p < 8 ? p + 1 : -1, jj, p != 6);
             sc_forward_core<L>(v, ldt + c * LP, twl, twql, jj, p != 6);
-            const int sel = p == 6 ? 2 : (p == 7 ? 3 : 4);
+            const RbcTerm<T> tt(p == 6 ? 2 : (p == 7 ? 3 : 4), Kyt, Kzt);
 #pragma unroll
             for (int s = 0; s < 8; s++) {
                 const T Kx = (T)((jj + TPL * s) * g.kf[0]);
-                nt[s] = nt[s] + rbc_term<T>(sel, Kx, Kyt, Kzt, sc * v[s]);
+                nt[s] = nt[s] + tt(Kx, sc * v[s]);
             }
             if constexpr (!DMA) __syncthreads();
         }
@@ -479,14 +500,14 @@ __global__ void __launch_bounds__(CB *(L / 8), 2) k_x_rbc(XrbcArgs a)
         cpx<T> v[8];
         load_product(v, p, p < 5 ? p + 1 : -1, jj, p == 3 || p == 5);
         sc_forward_core<L>(v, ldt + c * LP, twl, twql, jj, p == 3 || p == 5);
-        const int s0 = rbc_sel(p, 0), s1 = rbc_sel(p, 1), s2 = rbc_sel(p, 2);
+        const RbcTerm<T> t0(rbc_sel(p, 0), Kyt, Kzt), t1(rbc_sel(p, 1), Kyt, Kzt), t2(rbc_sel(p, 2), Kyt, Kzt);
 #pragma unroll
         for (int s = 0; s < 8; s++) {
             const T Kx = (T)((jj + TPL * s) * g.kf[0]);
             const cpx<T> q = sc * v[s];
-            N[0][s] = N[0][s] + rbc_term<T>(s0, Kx, Kyt, Kzt, q);
-            N[1][s] = N[1][s] + rbc_term<T>(s1, Kx, Kyt, Kzt, q);
-            N[2][s] = N[2][s] + rbc_term<T>(s2, Kx, Kyt, Kzt, q);
+            N[0][s] = N[0][s] + t0(Kx, q);
+            N[1][s] = N[1][s] + t1(Kx, q);
+            N[2][s] = N[2][s] + t2(Kx, q);
         }
         if constexpr (!DMA) __syncthreads();
     }
@@ -543,7 +564,9 @@ __global__ void __launch_bounds__(CB *(L / 8), 2) k_x_rbc(XrbcArgs a)
             double *Vc = (double *)sel_ptr((void *const *)a.V, comp, 3);
             double *Ac = (double *)sel_ptr((void *const *)a.acc, comp, 3);
             void *Uc = sel_ptr(a.U, comp, 3);
-            rbc_lawson<T, L, CB>(g, rp, c, j, ly, lz, Ky, Kz, colw, cols, ldd, ldb, ldt, twl, twql, r, rp.fh_u,
+            int jj = j;  // keeps the per-thread LDS addresses inside the loop
+            asm volatile("" : "+v"(jj));
+            rbc_lawson<T, L, CB>(g, rp, c, jj, ly, lz, Ky, Kz, colw, cols, ldd, ldb, ldt, twl, twql, r, rp.fh_u,
                                  rp.f1_u, Vc, Ac, Uc, (comp == 0 && stage < 4) ? a.f_out[1] : nullptr, comp == 0);
         }
     }
