@@ -38,7 +38,7 @@ __global__ void __launch_bounds__(CB *(L / 8)) k_xinv_sym(XsymArgs a)
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     if (tile >= x_tiles(g, CB, false)) return;
     tile_columns<CB>(colw, cols, g, false, tile);
-    __syncthreads();
+    lds_barrier();
     const int c = threadIdx.x / TPL, j = threadIdx.x % TPL;
     const int cw = colw[c];
     const int ly = cw >= 0 ? cw / g.nzp : 0, lz = cw >= 0 ? cw % g.nzp : 0;
@@ -49,10 +49,10 @@ __global__ void __launch_bounds__(CB *(L / 8)) k_xinv_sym(XsymArgs a)
 #pragma unroll
     for (int comp = 0; comp < 3; comp++) {
         stage_in<L, CB>(ldd, (const cpx<double> *)a.G[comp], cols, ncs);
-        __syncthreads();
+        lds_barrier();
 #pragma unroll
         for (int s = 0; s < 8; s++) m[comp][s] = ldd[c * LP + lpad(j + TPL * s)];
-        __syncthreads();
+        lds_barrier();
     }
     // M = P G (Pressure.cc:56-72 form: G - K (K.G)/K^2, identity at K = 0)
 #pragma unroll
@@ -88,9 +88,9 @@ __global__ void __launch_bounds__(CB *(L / 8)) k_xinv_sym(XsymArgs a)
         fft_line<L, +1>(v, ldt + c * LP, tw, j);
 #pragma unroll
         for (int s = 0; s < 8; s++) ldt[c * LP + lpad(j + TPL * s)] = v[s];
-        __syncthreads();
+        lds_barrier();
         stage_out<L, CB>(ldt, (cpx<T> *)a.S[t], colw, ncw);
-        __syncthreads();
+        lds_barrier();
     }
 }
 
@@ -169,23 +169,23 @@ __global__ void __launch_bounds__(CB *(L / 8)) k_xfwd_adj(XadjArgs a)
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     if (tile >= x_tiles(g, CB, false)) return;
     tile_columns<CB>(colw, cols, g, false, tile);
-    __syncthreads();
+    lds_barrier();
     const int c = threadIdx.x / TPL, j = threadIdx.x % TPL;
     const cpx<T> *tw = (const cpx<T> *)a.tw;
     const int ncw = g.ny * g.nzp, ncs = g.ny * g.nzc;
     const double c_sixth = (1.0 / 6) * ap.dt, c_third = (1.0 / 3) * ap.dt, c_half = 0.5 * ap.dt;
     for (int comp = 0; comp < 3; comp++) {
         stage_in<L, CB>(ldt, (const cpx<T> *)a.Q[comp], colw, ncw);
-        __syncthreads();
+        lds_barrier();
         cpx<T> v[8];
 #pragma unroll
         for (int s = 0; s < 8; s++) v[s] = ldt[c * LP + lpad(j + TPL * s)];
-        __syncthreads();
+        lds_barrier();
         fft_line<L, -1>(v, ldt + c * LP, tw, j);
-        __syncthreads();
+        lds_barrier();
 #pragma unroll
         for (int s = 0; s < 8; s++) ldd[c * LP + lpad(j + TPL * s)] = ap.scale * cvt<double>(v[s]);
-        __syncthreads();
+        lds_barrier();
         cpx<double> *lam = (cpx<double> *)a.lam[comp], *acc = (cpx<double> *)a.acc[comp];
         cpx<double> *lg = (cpx<double> *)a.lg[comp];
         for (int e = threadIdx.x; e < L * CB; e += CB * TPL) {
@@ -221,7 +221,7 @@ __global__ void __launch_bounds__(CB *(L / 8)) k_xfwd_adj(XadjArgs a)
                 lam[i] = (E1 * l4 + acc[i]) + r;
             }
         }
-        __syncthreads();
+        lds_barrier();
     }
 }
 

@@ -127,6 +127,20 @@ __host__ __device__ constexpr int num_passes(int L)
     return n;
 }
 
+// Block barrier that orders LDS accesses only.  The pass kernels exchange
+// data between threads through LDS alone, so their barriers need not wait
+// for global memory: __syncthreads' fence waits for every outstanding global
+// load and store (stores count in vmcnt on this architecture, so each
+// barrier after a tile store would wait out the write acknowledgements) and
+// for direct-to-LDS loads in flight.  Direct-to-LDS tiles are waited for
+// explicitly (dma_wait) before the barrier that publishes them.
+__device__ __forceinline__ void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // Synchronisation of the threads that share one line.  A line of L points
 // is handled by TPL = L/8 consecutive threads; when TPL <= 64 the line lives
 // inside one wavefront, whose LDS operations execute in order, so only the
@@ -141,7 +155,7 @@ __device__ __forceinline__ void line_sync()
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
     } else {
-        __syncthreads();
+        lds_barrier();
     }
 }
 

@@ -50,16 +50,16 @@ __global__ void __launch_bounds__(CB *(L / 8)) k_colfft(ColArgs a)
         if (col < a.ncol) v = f[(size_t)r * a.row_stride + col];
         lds[c * LP + lpad(r)] = v;
     }
-    __syncthreads();
+    lds_barrier();
     const int c = threadIdx.x / TPL, j = threadIdx.x % TPL;
     cpx<T> v[8];
 #pragma unroll
     for (int s = 0; s < 8; s++) v[s] = lds[c * LP + lpad(j + TPL * s)];
-    __syncthreads();
+    lds_barrier();
     fft_line<L, DIR>(v, lds + c * LP, tw, j);
 #pragma unroll
     for (int s = 0; s < 8; s++) lds[c * LP + lpad(j + TPL * s)] = v[s];
-    __syncthreads();
+    lds_barrier();
     for (int e = threadIdx.x; e < L * CB; e += CB * TPL) {
         const int r = e / CB, cc = e % CB, col = c0 + cc;
         if (col < a.ncol) f[(size_t)r * a.row_stride + col] = lds[cc * LP + lpad(r)];
@@ -95,7 +95,7 @@ __global__ void __launch_bounds__(CB *(L / 8)) k_xinv_state(XinvArgs a)
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     if (tile >= x_tiles(g, CB, patch)) return;
     tile_columns<CB>(colw, cols, g, patch, tile);
-    __syncthreads();
+    lds_barrier();
     const int c = threadIdx.x / TPL, j = threadIdx.x % TPL;
     const int cw = colw[c];
     const int ly = cw >= 0 ? cw / g.nzp : 0, lz = cw >= 0 ? cw % g.nzp : 0;
@@ -104,7 +104,7 @@ __global__ void __launch_bounds__(CB *(L / 8)) k_xinv_state(XinvArgs a)
     const int ncw = g.ny * g.nzp, ncs = g.ny * g.nzc;
     for (int comp = 0; comp < a.ncomp; comp++) {
         stage_in<L, CB>(ldd, (const cpx<double> *)a.V[comp], cols, ncs);
-        __syncthreads();
+        lds_barrier();
         cpx<T> v[8];
 #pragma unroll
         for (int s = 0; s < 8; s++) {
@@ -125,13 +125,13 @@ __global__ void __launch_bounds__(CB *(L / 8)) k_xinv_state(XinvArgs a)
             }
             v[s] = cvt<T>(d);
         }
-        __syncthreads();
+        lds_barrier();
         fft_line<L, +1>(v, ldt + c * LP, tw, j);
 #pragma unroll
         for (int s = 0; s < 8; s++) ldt[c * LP + lpad(j + TPL * s)] = v[s];
-        __syncthreads();
+        lds_barrier();
         stage_out<L, CB>(ldt, (cpx<T> *)a.U[comp], colw, ncw);
-        __syncthreads();
+        lds_barrier();
     }
 }
 
@@ -160,24 +160,24 @@ __global__ void __launch_bounds__(CB *(L / 8)) k_xfwd_spec(XfwdArgs a)
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     if (tile >= x_tiles(a.g, CB, false)) return;
     tile_columns<CB>(colw, cols, a.g, false, tile);
-    __syncthreads();
+    lds_barrier();
     const int c = threadIdx.x / TPL, j = threadIdx.x % TPL;
     const cpx<T> *tw = (const cpx<T> *)a.tw;
     const int ncw = a.g.ny * a.g.nzp, ncs = a.g.ny * a.g.nzc;
     for (int comp = 0; comp < a.ncomp; comp++) {
         stage_in<L, CB>(ldt, (const cpx<T> *)a.W[comp], colw, ncw);
-        __syncthreads();
+        lds_barrier();
         cpx<T> v[8];
 #pragma unroll
         for (int s = 0; s < 8; s++) v[s] = ldt[c * LP + lpad(j + TPL * s)];
-        __syncthreads();
+        lds_barrier();
         fft_line<L, -1>(v, ldt + c * LP, tw, j);
-        __syncthreads();
+        lds_barrier();
 #pragma unroll
         for (int s = 0; s < 8; s++) ldd[c * LP + lpad(j + TPL * s)] = a.scale * cvt<double>(v[s]);
-        __syncthreads();
+        lds_barrier();
         stage_out<L, CB>(ldd, (cpx<double> *)a.out[comp], cols, ncs);
-        __syncthreads();
+        lds_barrier();
     }
 }
 
@@ -364,7 +364,7 @@ __global__ void __launch_bounds__(CB *(L / 8), 2) k_x_nlin(XnlArgs a)
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     if (tile >= x_tiles(g, CB, false)) return;
     tile_columns<CB>(colw, cols, g, false, tile);
-    __syncthreads();
+    lds_barrier();
     const int c = threadIdx.x / TPL, j = threadIdx.x % TPL;
     const int cw = colw[c];
     const int ly = cw >= 0 ? cw / g.nzp : 0, lz = cw >= 0 ? cw % g.nzp : 0;
@@ -417,19 +417,19 @@ __global__ void __launch_bounds__(CB *(L / 8), 2) k_x_nlin(XnlArgs a)
         if constexpr (DMA) {
             // product p has landed in B; read it, then start the next one
             dma_wait();
-            __syncthreads();
+            lds_barrier();
 #pragma unroll
             for (int s = 0; s < 8; s++) v[s] = dma_tile_read<T, L, CB>(rawB, jj + TPL * s, c);
-            __syncthreads();
+            lds_barrier();
             if (pi + 1 < NPROD)
                 dma_tile<T, L, CB>((const cpx<T> *)a.P[GIVER ? kGiverOrder[pi + 1] : kNsOrder[pi + 1]], cw0, ncw,
                                    rawB);
         } else {
             stage_in<L, CB>(ldt, (const cpx<T> *)a.P[p], colw, ncw);
-            __syncthreads();
+            lds_barrier();
 #pragma unroll
             for (int s = 0; s < 8; s++) v[s] = ldt[c * LP + lpad(j + TPL * s)];
-            __syncthreads();
+            lds_barrier();
         }
         fft_line<L, -1>(v, ldt + c * LP, tw, jj);
 #pragma unroll
@@ -443,10 +443,10 @@ __global__ void __launch_bounds__(CB *(L / 8), 2) k_x_nlin(XnlArgs a)
             N[1][s] = N[1][s] + mul_i(K1, q);
             N[2][s] = N[2][s] + mul_i(K2, q);
         }
-        if constexpr (!DMA) __syncthreads();
+        if constexpr (!DMA) lds_barrier();
     }
     // the FFT exchanges in A are done before A and B are reused below
-    if constexpr (DMA) __syncthreads();
+    if constexpr (DMA) lds_barrier();
 
     const int stage = a.rk.stage;
     const bool want_rhs = stage > 0 || a.rk.write_rhs;
@@ -509,9 +509,9 @@ __global__ void __launch_bounds__(CB *(L / 8), 2) k_x_nlin(XnlArgs a)
             TG_TAKE(comp, nc);
 #pragma unroll
             for (int s = 0; s < 8; s++) ldd[c * LP + lpad(j + TPL * s)] = cvt<double>(nc[s]);
-            __syncthreads();
+            lds_barrier();
             stage_out<L, CB>(ldd, (cpx<double> *)a.out[comp], cols, ncs);
-            __syncthreads();
+            lds_barrier();
         }
         return;
     }
@@ -527,7 +527,7 @@ __global__ void __launch_bounds__(CB *(L / 8), 2) k_x_nlin(XnlArgs a)
         TG_TAKE(comp, nc);
         stage_in<L, CB>(ldd, (const cpx<double> *)a.V[comp], cols, ncs);
         if (stage >= 2) stage_in<L, CB>(ldb, (const cpx<double> *)a.acc[comp], colw, ncw);
-        __syncthreads();
+        lds_barrier();
         cpx<T> v[8];
 #pragma unroll
         for (int s = 0; s < 8; s++) {
@@ -572,16 +572,16 @@ __global__ void __launch_bounds__(CB *(L / 8), 2) k_x_nlin(XnlArgs a)
             if (dealias_zero(g, lx, ly, lz)) vn = mk<double>(0, 0);
             v[s] = cvt<T>(vn);
         }
-        __syncthreads();
+        lds_barrier();
         if (stage == 4) stage_out<L, CB>(ldb, (cpx<double> *)a.V[comp], cols, ncs);
         else stage_out<L, CB>(ldb, (cpx<double> *)a.acc[comp], colw, ncw);
         // inverse x-FFT of the next stage input (exchanges in buffer A)
         fft_line<L, +1>(v, ldt + c * LP, tw, j);
 #pragma unroll
         for (int s = 0; s < 8; s++) ldt[c * LP + lpad(j + TPL * s)] = v[s];
-        __syncthreads();
+        lds_barrier();
         stage_out<L, CB>(ldt, (cpx<T> *)a.U[comp], colw, ncw);
-        __syncthreads();
+        lds_barrier();
     }
 }
 #undef TG_TAKE

@@ -149,7 +149,7 @@ __global__ void __launch_bounds__(CB *(L / 8)) k_xinv_sc(XscArgs a)
     if (tile >= x_tiles(g, CB, patch)) return;
     tile_columns<CB>(colw, cols, g, patch, tile);
     if (threadIdx.x < 8) qhi[threadIdx.x] = ((const cpx<T> *)a.twq)[(L / 8) * threadIdx.x];
-    __syncthreads();
+    lds_barrier();
     const int c = threadIdx.x / TPL, j = threadIdx.x % TPL;
     const int cw = colw[c];
     const int ly = cw >= 0 ? cw / g.nzp : 0, lz = cw >= 0 ? cw % g.nzp : 0;
@@ -158,7 +158,7 @@ __global__ void __launch_bounds__(CB *(L / 8)) k_xinv_sc(XscArgs a)
     const int ncw = g.ny * g.nzp, ncs = g.ny * g.nzc;
     for (int comp = 0; comp < a.ncomp; comp++) {
         stage_in<L, CB>(ldd, (const cpx<double> *)a.V[comp], cols, ncs);
-        __syncthreads();
+        lds_barrier();
         cpx<T> v[8];
 #pragma unroll
         for (int s = 0; s < 8; s++) {
@@ -167,11 +167,11 @@ __global__ void __launch_bounds__(CB *(L / 8)) k_xinv_sc(XscArgs a)
             if ((a.flags & 1) && dealias_zero(g, lx, ly, lz)) d = mk<double>(0, 0);
             v[s] = cvt<T>(d);
         }
-        __syncthreads();
+        lds_barrier();
         sc_inverse<L>(v, ldt + c * LP, tw, twq, j, ((a.parity >> comp) & 1) != 0);
-        __syncthreads();
+        lds_barrier();
         stage_out<L, CB>(ldt, (cpx<T> *)a.U[comp], colw, ncw);
-        __syncthreads();
+        lds_barrier();
     }
 }
 
@@ -190,22 +190,22 @@ __global__ void __launch_bounds__(CB *(L / 8)) k_xfwd_sc(XscArgs a)
     if (tile >= x_tiles(g, CB, false)) return;
     tile_columns<CB>(colw, cols, g, false, tile);
     if (threadIdx.x < 8) qhi[threadIdx.x] = ((const cpx<T> *)a.twq)[(L / 8) * threadIdx.x];
-    __syncthreads();
+    lds_barrier();
     const int c = threadIdx.x / TPL, j = threadIdx.x % TPL;
     const cpx<T> *tw = (const cpx<T> *)a.tw;
     const QuarterTw<T> twq{(const cpx<T> *)a.twq, qhi};
     const int ncw = g.ny * g.nzp, ncs = g.ny * g.nzc;
     for (int comp = 0; comp < a.ncomp; comp++) {
         stage_in<L, CB>(ldt, (const cpx<T> *)a.U[comp], colw, ncw);
-        __syncthreads();
+        lds_barrier();
         cpx<T> v[8];
         sc_forward<L>(v, ldt + c * LP, tw, twq, j, ((a.parity >> comp) & 1) != 0);
-        __syncthreads();
+        lds_barrier();
 #pragma unroll
         for (int s = 0; s < 8; s++) ldd[c * LP + lpad(j + TPL * s)] = a.scale * cvt<double>(v[s]);
-        __syncthreads();
+        lds_barrier();
         stage_out<L, CB>(ldd, (cpx<double> *)a.V[comp], cols, ncs);
-        __syncthreads();
+        lds_barrier();
     }
 }
 
@@ -333,7 +333,7 @@ __device__ __forceinline__ void rbc_lawson(const Grid &g, const RbcParams &rp, i
     const double c_sixth = (1.0 / 6) * dt, c_third = (1.0 / 3) * dt;
     stage_in<L, CB>(ldd, (const cpx<double> *)V, cols, ncs);
     if (stage >= 2) stage_in<L, CB>(ldb, (const cpx<double> *)acc, colw, ncw);
-    __syncthreads();
+    lds_barrier();
     cpx<T> v[8];
 #pragma unroll
     for (int s = 0; s < 8; s++) {
@@ -369,17 +369,17 @@ __device__ __forceinline__ void rbc_lawson(const Grid &g, const RbcParams &rp, i
         if (dealias_zero(g, lx, ly, lz)) vn = mk<double>(0, 0);
         v[s] = cvt<T>(vn);
     }
-    __syncthreads();
+    lds_barrier();
     if (stage == 4) stage_out<L, CB>(ldb, (cpx<double> *)V, cols, ncs);
     else stage_out<L, CB>(ldb, (cpx<double> *)acc, colw, ncw);
     if (fo) {
         stage_out<L, CB>(ldd, (cpx<double> *)fo, cols, ncs);
-        __syncthreads();
+        lds_barrier();
     }
     sc_inverse<L>(v, ldt + c * LP, tw, twq, j, sine);
-    __syncthreads();
+    lds_barrier();
     stage_out<L, CB>(ldt, (cpx<T> *)U, colw, ncw);
-    __syncthreads();
+    lds_barrier();
 }
 
 // PHASE 0: temperature (N_T, its right-hand side, Lawson for theta);
@@ -409,7 +409,7 @@ __global__ void __launch_bounds__(CB *(L / 8), 2) k_x_rbc(XrbcArgs a)
     if (threadIdx.x < TPL) twqlo[threadIdx.x] = ((const cpx<T> *)a.twq)[threadIdx.x];
     if (threadIdx.x < 8) twqhi[threadIdx.x] = ((const cpx<T> *)a.twq)[TPL * threadIdx.x];
     const QuarterTw<T> twql{twqlo, twqhi};
-    __syncthreads();
+    lds_barrier();
     const int c = threadIdx.x / TPL, j = threadIdx.x % TPL;
     const int cw = colw[c];
     const int ly = cw >= 0 ? cw / g.nzp : 0, lz = cw >= 0 ? cw % g.nzp : 0;
@@ -426,15 +426,15 @@ __global__ void __launch_bounds__(CB *(L / 8), 2) k_x_rbc(XrbcArgs a)
     auto load_product = [&](cpx<T> *v, int p, int p_next, int jj, bool sine) {
         if constexpr (DMA) {
             dma_wait();
-            __syncthreads();
+            lds_barrier();
             sc_gather<L, T>(v, jj, sine, [&](int r) { return dma_tile_read<T, L, CB>(rawB, r, c); });
-            __syncthreads();
+            lds_barrier();
             if (p_next >= 0) dma_tile<T, L, CB>((const cpx<T> *)sel_ptr(a.P, p_next, 9), cw0, ncw, rawB);
         } else {
             stage_in<L, CB>(ldt, (const cpx<T> *)sel_ptr(a.P, p, 9), colw, ncw);
-            __syncthreads();
+            lds_barrier();
             sc_gather<L, T>(v, jj, sine, [&](int r) { return ldt[c * LP + lpad(r)]; });
-            __syncthreads();
+            lds_barrier();
         }
     };
 
@@ -457,20 +457,20 @@ __global__ void __launch_bounds__(CB *(L / 8), 2) k_x_rbc(XrbcArgs a)
                 const T Kx = (T)((jj + TPL * s) * g.kf[0]);
                 nt[s] = nt[s] + tt(Kx, sc * v[s]);
             }
-            if constexpr (!DMA) __syncthreads();
+            if constexpr (!DMA) lds_barrier();
         }
-        if constexpr (DMA) __syncthreads();
+        if constexpr (DMA) lds_barrier();
         cpx<double> r[8];
         if (want_rhs) {
             // RHS_T = -(N_T - F_T), F_T = u_x of the stage input
             stage_in<L, CB>(ldb, (const cpx<double> *)a.f_in[1], cols, ncs);
-            __syncthreads();
+            lds_barrier();
 #pragma unroll
             for (int s = 0; s < 8; s++) {
                 const cpx<double> n = cvt<double>(nt[s]) - ldb[c * LP + lpad(j + TPL * s)];
                 r[s] = mk<double>(-n.x, -n.y);
             }
-            __syncthreads();
+            lds_barrier();
         } else {
 #pragma unroll
             for (int s = 0; s < 8; s++) r[s] = cvt<double>(nt[s]);
@@ -478,9 +478,9 @@ __global__ void __launch_bounds__(CB *(L / 8), 2) k_x_rbc(XrbcArgs a)
         if (stage == 0) {
 #pragma unroll
             for (int s = 0; s < 8; s++) ldd[c * LP + lpad(j + TPL * s)] = r[s];
-            __syncthreads();
+            lds_barrier();
             stage_out<L, CB>(ldd, (cpx<double> *)a.out[3], cols, ncs);
-            __syncthreads();
+            lds_barrier();
         } else {
             rbc_lawson<T, L, CB>(g, rp, c, j, ly, lz, Ky, Kz, colw, cols, ldd, ldb, ldt, twl, twql, r, rp.fh_t, rp.f1_t,
                                  a.V[3], a.acc[3], a.U[3], stage < 4 ? a.f_out[0] : nullptr, true);
@@ -509,15 +509,15 @@ __global__ void __launch_bounds__(CB *(L / 8), 2) k_x_rbc(XrbcArgs a)
             N[1][s] = N[1][s] + t1(Kx, q);
             N[2][s] = N[2][s] + t2(Kx, q);
         }
-        if constexpr (!DMA) __syncthreads();
+        if constexpr (!DMA) lds_barrier();
     }
-    if constexpr (DMA) __syncthreads();
+    if constexpr (DMA) lds_barrier();
     if (want_rhs) {
         // nlin -= F (F_x = theta of the stage input); p = div/K^2 with the SFF
         // divergence (+Kx on the 'S' x component); nlin += grad p ('C' parity:
         // -Kx p); negate (Compute_rhs_scalar)
         stage_in<L, CB>(ldd, (const cpx<double> *)a.f_in[0], cols, ncs);
-        __syncthreads();
+        lds_barrier();
 #pragma unroll
         for (int s = 0; s < 8; s++) {
             const int lx = j + TPL * s;
@@ -537,7 +537,7 @@ __global__ void __launch_bounds__(CB *(L / 8), 2) k_x_rbc(XrbcArgs a)
             N[1][s] = cvt<T>(mk<double>(-n1.x, -n1.y));
             N[2][s] = cvt<T>(mk<double>(-n2.x, -n2.y));
         }
-        __syncthreads();
+        lds_barrier();
     }
 #pragma unroll 1
     for (int comp = 0; comp < 3; comp++) {
@@ -557,9 +557,9 @@ __global__ void __launch_bounds__(CB *(L / 8), 2) k_x_rbc(XrbcArgs a)
         if (stage == 0) {
 #pragma unroll
             for (int s = 0; s < 8; s++) ldd[c * LP + lpad(j + TPL * s)] = r[s];
-            __syncthreads();
+            lds_barrier();
             stage_out<L, CB>(ldd, (cpx<double> *)sel_ptr((void *const *)a.out, comp, 3), cols, ncs);
-            __syncthreads();
+            lds_barrier();
         } else {
             double *Vc = (double *)sel_ptr((void *const *)a.V, comp, 3);
             double *Ac = (double *)sel_ptr((void *const *)a.acc, comp, 3);
