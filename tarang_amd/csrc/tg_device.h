@@ -145,15 +145,16 @@ __device__ __forceinline__ void lds_barrier()
 // is handled by TPL = L/8 consecutive threads; when TPL <= 64 the line lives
 // inside one wavefront, whose LDS operations execute in order, so only the
 // compiler has to be kept from moving LDS accesses across the exchange
-// (wavefront-scope fence).  Longer lines span waves and need the block
-// barrier.
+// (wavefront-scope fence on LDS only: global loads and stores may still be
+// moved across it, e.g. the next line's loads ahead of this line's FFT).
+// Longer lines span waves and need the block barrier.
 template <int TPL>
 __device__ __forceinline__ void line_sync()
 {
     if constexpr (TPL <= 64) {
-        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront", "local");
         __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront", "local");
     } else {
         lds_barrier();
     }

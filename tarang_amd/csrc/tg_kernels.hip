@@ -248,10 +248,14 @@ __global__ void __launch_bounds__(256) k_z_products(ZArgs a)
     const size_t lsafe = valid ? line : nlines - 1;
     const cpx<T> *twm = (const cpx<T> *)a.twm, *twn = (const cpx<T> *)a.twn;
     cpx<T> *L = lds + li * LP;
-    cpx<T> u[3][8];
+    // the three input lines are loaded before the first transform: their
+    // loads are in flight together
+    cpx<T> u[3][8], xm[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) zline_load<T, M>(u[c], xm[c], (const cpx<T> *)a.in[c] + lsafe * a.g.nzp, j);
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-        zline_c2r<T, M>(u[c], (const cpx<T> *)a.in[c] + lsafe * a.g.nzp, L, twm, twn, j);
+        zline_c2r_from<T, M>(u[c], xm[c], L, twm, twn, j);
         line_sync<M / 8>();
     }
     if (!GIVER && a.tape[0] && valid) {

@@ -211,15 +211,27 @@ __device__ __forceinline__ double kmag(const Grid &g, int kx, int ky, int kz)
 // r2c: Z = DFT_M(z); X[k] = ((Z[k] + conj Z[M-k]) - i W^k (Z[k] - conj Z[M-k]))/2.
 // W = exp(-2 pi i / N); twn holds W^k for k = 0..M.
 // ---------------------------------------------------------------------------
+// The M+1 points of a complex line: v[s] = X[j + TPL*s], xm = X[M] (read by
+// every thread of the line; member-wise copies keep v in registers).
 template <typename T, int M>
-__device__ __forceinline__ void zline_c2r(cpx<T> *v, const cpx<T> *__restrict__ X, cpx<T> *lds,
-                                          const cpx<T> *__restrict__ twm, const cpx<T> *__restrict__ twn, int j)
+__device__ __forceinline__ void zline_load(cpx<T> *v, cpx<T> &xm, const cpx<T> *__restrict__ X, int j)
+{
+    constexpr int TPL = M / 8;
+#pragma unroll
+    for (int s = 0; s < 8; s++) v[s] = mk<T>(X[j + TPL * s].x, X[j + TPL * s].y);
+    xm = mk<T>(X[M].x, X[M].y);
+}
+
+// c2r of a line already loaded by zline_load (v in, samples out in v).
+template <typename T, int M>
+__device__ __forceinline__ void zline_c2r_from(cpx<T> *v, cpx<T> xm, cpx<T> *lds, const cpx<T> *__restrict__ twm,
+                                               const cpx<T> *__restrict__ twn, int j)
 {
     constexpr int TPL = M / 8;
     // stage the line (M+1 points) in LDS
 #pragma unroll
-    for (int s = 0; s < 8; s++) lds[lpad(j + TPL * s)] = X[j + TPL * s];
-    if (j == 0) lds[lpad(M)] = X[M];
+    for (int s = 0; s < 8; s++) lds[lpad(j + TPL * s)] = mk<T>(v[s].x, v[s].y);
+    if (j == 0) lds[lpad(M)] = xm;
     line_sync<M / 8>();
 #pragma unroll
     for (int s = 0; s < 8; s++) {
@@ -235,6 +247,15 @@ __device__ __forceinline__ void zline_c2r(cpx<T> *v, const cpx<T> *__restrict__ 
     }
     line_sync<M / 8>();
     fft_line<M, +1>(v, lds, twm, j);
+}
+
+template <typename T, int M>
+__device__ __forceinline__ void zline_c2r(cpx<T> *v, const cpx<T> *__restrict__ X, cpx<T> *lds,
+                                          const cpx<T> *__restrict__ twm, const cpx<T> *__restrict__ twn, int j)
+{
+    cpx<T> xm;
+    zline_load<T, M>(v, xm, X, j);
+    zline_c2r_from<T, M>(v, xm, lds, twm, twn, j);
 }
 
 template <typename T, int M>
