@@ -908,7 +908,8 @@ int tg_plan_create(const tg_config *cfg, tg_plan **plan)
         if (hipMemset(p->U[i], 0, p->ncw * p->elem) != hipSuccess) rc = fail(TG_ERR_HIP, "hipMemset failed");
     for (int i = 0; i < 9 && !rc; i++)
         if (hipMemset(p->P[i], 0, p->ncw * p->elem) != hipSuccess) rc = fail(TG_ERR_HIP, "hipMemset failed");
-    // accumulator: FP64, padded work layout (also holds giver outputs in the boundary layout)
+    // accumulator: FP64-sized, padded work layout (the NS stage keeps it at the
+    // transform precision; also holds giver outputs in the boundary layout)
     for (int i = 0; i < 3 && !rc; i++) rc = dmalloc(p, (void **)&p->acc[i], p->ncw * 16);
     p->red_blocks = (int)std::min<size_t>(2048, (p->nc + 1023) / 1024);
     if (p->red_blocks < 1) p->red_blocks = 1;

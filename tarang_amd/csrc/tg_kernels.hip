@@ -304,7 +304,7 @@ __global__ void __launch_bounds__(256) k_z_products(ZArgs a)
 struct XnlArgs {
     void *P[9];
     double *V[3];
-    double *acc[3];
+    void *acc[3];  // stage accumulator at the transform precision T
     void *U[3];
     double *out[3];
     const double *slots_in;
@@ -521,7 +521,10 @@ __global__ void __launch_bounds__(CB *(L / 8), 2) k_x_nlin(XnlArgs a)
     }
 
     // --- Lawson RK4 stage algebra (RK.cc:91-170, Tadvance_functions.cc:298-324,566-570) ---
-    // V: FP64 state, boundary layout; acc: FP64, padded work layout.
+    // V: FP64 state, boundary layout; acc: padded work layout at the
+    // transform precision (it sums dt-weighted right-hand sides, which carry
+    // that precision; the algebra runs in FP64 and only the state persists
+    // between steps).
     const double dt = a.rk.dt;
     const double c_half = 0.5 * dt, c_one = 1.0 * dt;
     const double c_sixth = (1.0 / 6) * dt, c_third = (1.0 / 3) * dt;
@@ -530,7 +533,7 @@ __global__ void __launch_bounds__(CB *(L / 8), 2) k_x_nlin(XnlArgs a)
         cpx<T> nc[8];
         TG_TAKE(comp, nc);
         stage_in<L, CB>(ldd, (const cpx<double> *)a.V[comp], cols, ncs);
-        if (stage >= 2) stage_in<L, CB>(ldb, (const cpx<double> *)a.acc[comp], colw, ncw);
+        if (stage >= 2) stage_in<L, CB>(ldb, (const cpx<T> *)a.acc[comp], colw, ncw);
         lds_barrier();
         cpx<T> v[8];
 #pragma unroll
@@ -578,7 +581,7 @@ __global__ void __launch_bounds__(CB *(L / 8), 2) k_x_nlin(XnlArgs a)
         }
         lds_barrier();
         if (stage == 4) stage_out<L, CB>(ldb, (cpx<double> *)a.V[comp], cols, ncs);
-        else stage_out<L, CB>(ldb, (cpx<double> *)a.acc[comp], colw, ncw);
+        else stage_out<L, CB>(ldb, (cpx<T> *)a.acc[comp], colw, ncw);
         // inverse x-FFT of the next stage input (exchanges in buffer A)
         fft_line<L, +1>(v, ldt + c * LP, tw, j);
 #pragma unroll
