@@ -15,6 +15,8 @@
 //   y pass  tiles of all Ny rows x CB contiguous lz columns of one lx.
 //   z pass  whole z-lines (contiguous), TPL = Nz/16 threads per line; a
 //           real line of Nz points is packed as Nz/2 complex points.
+#include <algorithm>
+
 #include "tg_device.h"
 #include "tg_kernels.h"
 #include "tg_pass.h"
@@ -665,7 +667,9 @@ hipError_t launch_colfft(const Grid &g, int axis, int dir, void *const *fields, 
         a.ncol = g.ny * g.nzp;
     }
     TG_DISPATCH_L(L, {
-        constexpr int CB = col_block<T>(LL, sizeof(cpx<T>));
+        // at most 512 threads a block (FP32 at 512 points would otherwise
+        // take 1024-thread blocks, measured slower)
+        constexpr int CB = std::min(col_block<T>(LL, sizeof(cpx<T>)), std::max(1, 512 / (LL / 8)));
         dim3 grid(a.nouter * ((a.ncol + CB - 1) / CB), nfields);
         if (dir < 0) hipLaunchKernelGGL((k_colfft<T, LL, -1, CB>), grid, dim3(CB * (LL / 8)), 0, st, a);
         else hipLaunchKernelGGL((k_colfft<T, LL, +1, CB>), grid, dim3(CB * (LL / 8)), 0, st, a);
