@@ -188,37 +188,51 @@ __global__ void __launch_bounds__(CB *(L / 8)) k_xfwd_adj(XadjArgs a)
         lds_barrier();
         cpx<double> *lam = (cpx<double> *)a.lam[comp], *acc = (cpx<double> *)a.acc[comp];
         cpx<double> *lg = (cpx<double> *)a.lg[comp];
-        for (int e = threadIdx.x; e < L * CB; e += CB * TPL) {
-            const int lx = e / CB, cc = e % CB, col = cols[cc];
-            if (col < 0) continue;
+        // thread t: column t % CB, rows t / CB + TPL i; the eight elements'
+        // inputs are loaded before the first result is stored
+        const int cc = threadIdx.x % CB, r0 = threadIdx.x / CB, col = cols[cc];
+        if (col >= 0) {
             const int ly = col / g.nzc, lz = col % g.nzc;
-            const size_t i = (size_t)lx * ncs + col;
-            cpx<double> r = ldd[cc * LP + lpad(lx)];
             const double cw = (lz == 0 || lz == g.nz / 2) ? -1.0 : -2.0;
-            r = dealias_zero(g, lx, ly, lz) ? mk<double>(0, 0) : cw * r;
-            if (ap.mode == ADJ_RHS) {
-                lg[i] = r;
-                continue;
+            const double Ky = kwave(ly, g.ny) * g.kf[1], Kz = lz * g.kf[2];
+            const bool need_l = ap.mode != ADJ_RHS;
+            const bool need_a = ap.mode == ADJ_S2 || ap.mode == ADJ_S1 || ap.mode == ADJ_S0;
+            cpx<double> l4[8], a4[8];
+#pragma unroll
+            for (int ii = 0; ii < 8; ii++) {
+                const size_t i = (size_t)(r0 + TPL * ii) * ncs + col;
+                l4[ii] = need_l ? mk<double>(lam[i].x, lam[i].y) : mk<double>(0, 0);
+                a4[ii] = need_a ? mk<double>(acc[i].x, acc[i].y) : mk<double>(0, 0);
             }
-            const double Kx = kwave(lx, g.nx) * g.kf[0], Ky = kwave(ly, g.ny) * g.kf[1], Kz = lz * g.kf[2];
-            const double ks = (Kx * Kx + Ky * Ky) + Kz * Kz;
-            const cpx<double> l4 = lam[i];
-            if (ap.mode == ADJ_S3) {
-                const double Eh = exp(ap.fh * ks);
-                const cpx<double> lv = c_sixth * r;
-                acc[i] = (Eh * Eh) * lv;
-                lg[i] = c_third * (Eh * l4) + ap.dt * (Eh * lv);
-            } else if (ap.mode == ADJ_S2) {
-                const double Eh = exp(ap.fh * ks);
-                acc[i] = acc[i] + Eh * r;
-                lg[i] = c_third * (Eh * l4) + c_half * r;
-            } else if (ap.mode == ADJ_S1) {
-                const double Eh = exp(ap.fh * ks), E1 = exp(ap.f1 * ks);
-                acc[i] = acc[i] + Eh * r;
-                lg[i] = c_sixth * (E1 * l4) + c_half * (Eh * r);
-            } else {
-                const double E1 = exp(ap.f1 * ks);
-                lam[i] = (E1 * l4 + acc[i]) + r;
+#pragma unroll
+            for (int ii = 0; ii < 8; ii++) {
+                const int lx = r0 + TPL * ii;
+                const size_t i = (size_t)lx * ncs + col;
+                cpx<double> r = ldd[cc * LP + lpad(lx)];
+                r = dealias_zero(g, lx, ly, lz) ? mk<double>(0, 0) : cw * r;
+                if (ap.mode == ADJ_RHS) {
+                    lg[i] = r;
+                    continue;
+                }
+                const double Kx = kwave(lx, g.nx) * g.kf[0];
+                const double ks = (Kx * Kx + Ky * Ky) + Kz * Kz;
+                if (ap.mode == ADJ_S3) {
+                    const double Eh = exp(ap.fh * ks);
+                    const cpx<double> lv = c_sixth * r;
+                    acc[i] = (Eh * Eh) * lv;
+                    lg[i] = c_third * (Eh * l4[ii]) + ap.dt * (Eh * lv);
+                } else if (ap.mode == ADJ_S2) {
+                    const double Eh = exp(ap.fh * ks);
+                    acc[i] = a4[ii] + Eh * r;
+                    lg[i] = c_third * (Eh * l4[ii]) + c_half * r;
+                } else if (ap.mode == ADJ_S1) {
+                    const double Eh = exp(ap.fh * ks), E1 = exp(ap.f1 * ks);
+                    acc[i] = a4[ii] + Eh * r;
+                    lg[i] = c_sixth * (E1 * l4[ii]) + c_half * (Eh * r);
+                } else {
+                    const double E1 = exp(ap.f1 * ks);
+                    lam[i] = (E1 * l4[ii] + a4[ii]) + r;
+                }
             }
         }
         lds_barrier();

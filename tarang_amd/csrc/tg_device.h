@@ -141,6 +141,22 @@ __device__ __forceinline__ void lds_barrier()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// lds_barrier for a point where a direct-to-LDS load into a tile that no
+// thread touches until its own wait is in flight: the workgroup-scope release
+// fence would wait for that load too (it counts as a pending LDS write).  The
+// wave's own LDS instructions are retired explicitly (s_waitcnt lgkmcnt(0)),
+// the wavefront-scope fences keep the compiler from moving LDS accesses
+// across the barrier, and the LDS executes the instructions of the waves in
+// issue order, so accesses issued before the barrier by one wave complete
+// before those issued after it by another.
+__device__ __forceinline__ void lds_barrier_async()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+    __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0); vmcnt and expcnt at their maxima
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+}
+
 // Synchronisation of the threads that share one line.  A line of L points
 // is handled by TPL = L/8 consecutive threads; when TPL <= 64 the line lives
 // inside one wavefront, whose LDS operations execute in order, so only the

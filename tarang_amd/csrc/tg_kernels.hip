@@ -16,6 +16,7 @@
 //   z pass  whole z-lines (contiguous), TPL = Nz/16 threads per line; a
 //           real line of Nz points is packed as Nz/2 complex points.
 #include <algorithm>
+#include <type_traits>
 
 #include "tg_device.h"
 #include "tg_kernels.h"
@@ -46,11 +47,19 @@ __global__ void __launch_bounds__(CB *(L / 8)) k_colfft(ColArgs a)
     const int c0 = (blockIdx.x % nblk) * CB;
     cpx<T> *f = (cpx<T> *)a.f[blockIdx.y] + (size_t)outer * a.outer_stride;
     const cpx<T> *tw = (const cpx<T> *)a.tw;
-    for (int e = threadIdx.x; e < L * CB; e += CB * TPL) {
-        const int r = e / CB, c = e % CB, col = c0 + c;
-        cpx<T> v = mk<T>(0, 0);
-        if (col < a.ncol) v = f[(size_t)r * a.row_stride + col];
-        lds[c * LP + lpad(r)] = v;
+    // thread t moves column t % CB, rows t / CB + TPL i: the eight loads are
+    // issued before the first LDS write (a column past the last one loads
+    // the last one, stages zeros and is not stored)
+    const int sc = threadIdx.x % CB, sr = threadIdx.x / CB, scol = c0 + sc;
+    const bool live = scol < a.ncol;
+    {
+        const cpx<T> *p = f + (size_t)sr * a.row_stride + min(scol, a.ncol - 1);
+        const size_t step = (size_t)TPL * a.row_stride;
+        cpx<T> x[8];
+#pragma unroll
+        for (int i = 0; i < 8; i++) x[i] = mk<T>(p[i * step].x, p[i * step].y);
+#pragma unroll
+        for (int i = 0; i < 8; i++) lds[sc * LP + lpad(sr + TPL * i)] = live ? x[i] : mk<T>(0, 0);
     }
     lds_barrier();
     const int c = threadIdx.x / TPL, j = threadIdx.x % TPL;
@@ -62,9 +71,14 @@ __global__ void __launch_bounds__(CB *(L / 8)) k_colfft(ColArgs a)
 #pragma unroll
     for (int s = 0; s < 8; s++) lds[c * LP + lpad(j + TPL * s)] = v[s];
     lds_barrier();
-    for (int e = threadIdx.x; e < L * CB; e += CB * TPL) {
-        const int r = e / CB, cc = e % CB, col = c0 + cc;
-        if (col < a.ncol) f[(size_t)r * a.row_stride + col] = lds[cc * LP + lpad(r)];
+    if (live) {
+        cpx<T> *p = f + (size_t)sr * a.row_stride + scol;
+        const size_t step = (size_t)TPL * a.row_stride;
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const cpx<T> v = lds[sc * LP + lpad(sr + TPL * i)];
+            p[i * step] = mk<T>(v.x, v.y);
+        }
     }
 }
 
@@ -430,6 +444,9 @@ __global__ void __launch_bounds__(CB *(L / 8), 2) k_x_nlin(XnlArgs a)
             if (pi + 1 < NPROD)
                 dma_tile<T, L, CB>((const cpx<T> *)a.P[GIVER ? kGiverOrder[pi + 1] : kNsOrder[pi + 1]], cw0, ncw,
                                    rawB);
+            else if (a.rk.stage > 0)
+                // the state of the first component, for the RK update below
+                dma_state_tile<L, CB>((const cpx<double> *)a.V[0], cols, ncs, rawB);
         } else {
             stage_in<L, CB>(ldt, (const cpx<T> *)a.P[p], colw, ncw);
             lds_barrier();
@@ -451,8 +468,9 @@ __global__ void __launch_bounds__(CB *(L / 8), 2) k_x_nlin(XnlArgs a)
         }
         if constexpr (!DMA) lds_barrier();
     }
-    // the FFT exchanges in A are done before A and B are reused below
-    if constexpr (DMA) lds_barrier();
+    // the FFT exchanges in A are done before A and B are reused below (B may
+    // be receiving the first state component)
+    if constexpr (DMA) lds_barrier_async();
 
     const int stage = a.rk.stage;
     const bool want_rhs = stage > 0 || a.rk.write_rhs;
@@ -530,6 +548,78 @@ __global__ void __launch_bounds__(CB *(L / 8), 2) k_x_nlin(XnlArgs a)
     const double dt = a.rk.dt;
     const double c_half = 0.5 * dt, c_one = 1.0 * dt;
     const double c_sixth = (1.0 / 6) * dt, c_third = (1.0 / 3) * dt;
+    if constexpr (DMA) {
+        // Component K: its state V[K] sits in tile X (direct-to-LDS load,
+        // column-major chunks), the accumulator is staged into tile Y, which
+        // then takes the updated accumulator (or state); the next stage input
+        // is x-inverted with X as the exchange buffer, and V[K+1] is loaded
+        // into Y while that input is stored.  The components are unrolled so
+        // that X and Y are known LDS objects (the waits for the direct-to-LDS
+        // loads then cover only the tile they write).
+        auto rk_comp = [&](auto kc, char *X, char *Y) {
+            constexpr int K = decltype(kc)::value;
+            cpx<double> *Yd = (cpx<double> *)Y;
+            cpx<T> *Xt = (cpx<T> *)X;
+            if (stage >= 2) stage_in<L, CB>(Yd, (const cpx<T> *)a.acc[K], colw, ncw);
+            dma_wait();
+            lds_barrier();
+            cpx<T> v[8];
+#pragma unroll
+            for (int s = 0; s < 8; s++) {
+                const int lx = j + TPL * s;
+                const int li = c * LP + lpad(lx);
+                const double Kx = kwave(lx, g.nx) * g.kf[0];
+                const double kxs = Kx * Kx;
+                const double ks = (kxs + Ky * Ky) + Kz * Kz;
+                const double Eh = exp(a.rk.fh * ks);
+                const cpx<double> r = cvt<double>(N[K][s]);
+                const cpx<double> v0 = dma_tile_read<double, L, CB>(X, lx, c);
+                cpx<double> vn, ac;
+                if (stage == 1) {
+                    cpx<double> d0 = v0;
+                    if (dealias_zero(g, lx, ly, lz)) d0 = mk<double>(0, 0);
+                    const double E1 = exp(a.rk.f1 * ks);
+                    ac = c_sixth * (E1 * r);
+                    vn = Eh * (d0 + c_half * r);
+                } else if (stage == 2) {
+                    ac = Yd[li] + c_third * (Eh * r);
+                    vn = Eh * v0 + c_half * r;
+                } else if (stage == 3) {
+                    ac = Yd[li] + c_third * (Eh * r);
+                    vn = Eh * (Eh * v0 + c_one * r);
+                } else {
+                    ac = Yd[li] + c_sixth * r;
+                    const double E1 = exp(a.rk.f1 * ks);
+                    vn = E1 * v0 + ac;
+                }
+                Yd[li] = mk<double>(stage == 4 ? vn.x : ac.x, stage == 4 ? vn.y : ac.y);
+                if (a.b.kind == 1 && cw >= 0) {
+                    const int kx = kwave(lx, g.nx);
+                    const int sl = band_slot(a.b, kx, ky, lz);
+                    if (sl >= 0) {
+                        const double Kb = kmag(g, kx, ky, lz);
+                        if (Kb > a.b.r_in && Kb <= a.b.r_out) ((cpx<double> *)a.slots_out)[sl * 3 + K] = vn;
+                    }
+                }
+                if (dealias_zero(g, lx, ly, lz)) vn = mk<double>(0, 0);
+                v[s] = cvt<T>(vn);
+            }
+            lds_barrier();
+            if (stage == 4) stage_out<L, CB>(Yd, (cpx<double> *)a.V[K], cols, ncs);
+            else stage_out<L, CB>(Yd, (cpx<T> *)a.acc[K], colw, ncw);
+            fft_line<L, +1>(v, Xt + c * LP, tw, j);
+#pragma unroll
+            for (int s = 0; s < 8; s++) Xt[c * LP + lpad(j + TPL * s)] = v[s];
+            lds_barrier();
+            if constexpr (K < 2) dma_state_tile<L, CB>((const cpx<double> *)a.V[K + 1], cols, ncs, Y);
+            stage_out<L, CB>(Xt, (cpx<T> *)a.U[K], colw, ncw);
+            lds_barrier_async();
+        };
+        rk_comp(std::integral_constant<int, 0>{}, rawB, rawA);
+        rk_comp(std::integral_constant<int, 1>{}, rawA, rawB);
+        rk_comp(std::integral_constant<int, 2>{}, rawB, rawA);
+        return;
+    }
 #pragma unroll 1
     for (int comp = 0; comp < 3; comp++) {
         cpx<T> nc[8];

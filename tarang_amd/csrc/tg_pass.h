@@ -60,17 +60,23 @@ __device__ __forceinline__ cpx<A> cvt(cpx<B> v) { return mk<A>((A)v.x, (A)v.y); 
 // ---------------------------------------------------------------------------
 // x pass helpers: staging of an (Nx x CB) column tile through LDS.
 // ---------------------------------------------------------------------------
+// Thread t moves column t % CB, rows t / CB + TPL i (i < 8): consecutive
+// lanes cover the CB columns of a row.  The eight loads are issued before the
+// first LDS write (a runtime-bounded loop would wait for each load in turn);
+// a column outside the field (cols[c] < 0) loads column 0 and stages zeros.
 template <int L, int CB, typename S, typename G>
 __device__ __forceinline__ void stage_in(cpx<S> *lds, const cpx<G> *__restrict__ f, const int *cols, int ncol_total)
 {
     constexpr int TPL = L / 8;
     constexpr int LP = lpad_len(L);
-    for (int e = threadIdx.x; e < L * CB; e += CB * TPL) {
-        const int r = e / CB, c = e % CB, col = cols[c];
-        cpx<S> v = mk<S>(0, 0);
-        if (col >= 0) v = cvt<S>(f[(size_t)r * ncol_total + col]);
-        lds[c * LP + lpad(r)] = v;
-    }
+    const int c = threadIdx.x % CB, r0 = threadIdx.x / CB, col = cols[c];
+    const cpx<G> *p = f + (size_t)r0 * ncol_total + max(col, 0);
+    const size_t step = (size_t)TPL * ncol_total;
+    cpx<G> x[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) x[i] = mk<G>(p[i * step].x, p[i * step].y);
+#pragma unroll
+    for (int i = 0; i < 8; i++) lds[c * LP + lpad(r0 + TPL * i)] = col >= 0 ? cvt<S>(x[i]) : mk<S>(0, 0);
 }
 
 template <int L, int CB, typename S, typename G>
@@ -78,9 +84,14 @@ __device__ __forceinline__ void stage_out(const cpx<S> *lds, cpx<G> *__restrict_
 {
     constexpr int TPL = L / 8;
     constexpr int LP = lpad_len(L);
-    for (int e = threadIdx.x; e < L * CB; e += CB * TPL) {
-        const int r = e / CB, c = e % CB, col = cols[c];
-        if (col >= 0) f[(size_t)r * ncol_total + col] = cvt<G>(lds[c * LP + lpad(r)]);
+    const int c = threadIdx.x % CB, r0 = threadIdx.x / CB, col = cols[c];
+    if (col < 0) return;
+    cpx<G> *p = f + (size_t)r0 * ncol_total + col;
+    const size_t step = (size_t)TPL * ncol_total;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        const cpx<S> v = lds[c * LP + lpad(r0 + TPL * i)];
+        p[i * step] = mk<G>((G)v.x, (G)v.y);
     }
 }
 
@@ -167,6 +178,26 @@ __device__ __forceinline__ void dma_tile(const cpx<T> *__restrict__ f, int cw0, 
             constexpr int RPC2 = 128 / CB;
             src = f + (size_t)(k * RPC2 + lane % RPC2) * ncol_total + cw0 + 2 * (lane / RPC2);
         }
+        __builtin_amdgcn_global_load_lds((const void *)src, (__attribute__((address_space(3))) void *)(lds + k * 1024),
+                                         16, 0, 0);
+    }
+}
+
+// The same chunk layout for an FP64 tile of the state (boundary layout):
+// column c of the tile is state column cols[c]; a column outside the state
+// (cols[c] < 0, the padding of the work layout) loads state column 0 instead,
+// whose values the caller ignores.
+template <int L, int CB>
+__device__ __forceinline__ void dma_state_tile(const cpx<double> *__restrict__ f, const int *cols, int ncol_total,
+                                               char *lds)
+{
+    constexpr int NWAVE = CB * (L / 8) / 64;
+    constexpr int NCHUNK = L * CB * 16 / 1024;
+    constexpr int RPC = 64 / CB;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int col = max(cols[lane / RPC], 0);
+    for (int k = wave; k < NCHUNK; k += NWAVE) {
+        const cpx<double> *src = f + (size_t)(k * RPC + lane % RPC) * ncol_total + col;
         __builtin_amdgcn_global_load_lds((const void *)src, (__attribute__((address_space(3))) void *)(lds + k * 1024),
                                          16, 0, 0);
     }
