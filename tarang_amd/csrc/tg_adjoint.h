@@ -34,8 +34,12 @@ hipError_t launch_z_adj(const Grid &g, void *const *S, void *const *Q, void *con
                         const void *twn, hipStream_t st);
 
 // x-forward of q with r = -c D scale F(q) and the stage epilogue above.
+// With S_next (modes S3..S0), the next stage's cotangent (lg, or lam after
+// S0) is not stored as lg but taken, in registers, through the next stage's
+// launch_xinv_sym work into S_next.
 template <typename T>
 hipError_t launch_xfwd_adj(const Grid &g, const AdjParams &ap, void *const *Q, double *const *lam,
-                           double *const *acc, double *const *lg, const void *twx, hipStream_t st);
+                           double *const *acc, double *const *lg, const void *twx, hipStream_t st,
+                           void *const *S_next = nullptr);
 
 }  // namespace tg

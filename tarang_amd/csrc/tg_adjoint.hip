@@ -25,30 +25,26 @@ struct XsymArgs {
     Grid g;
 };
 
-template <typename T, int L, int CB>
-__global__ void __launch_bounds__(CB *(L / 8)) k_xinv_sym(XsymArgs a)
+// From the cotangent G of a tile (put(k) stages component k of G into the
+// LDS tile in the stage_in layout): M = P G, the six symmetrised fields
+// (-i K_m M_j - i K_j M_m)/c, their x-inverse, stored into S.
+template <typename T, int L, int CB, typename Put>
+__device__ __forceinline__ void xsym_tile(const Grid &g, char *raw, const int *colw, const cpx<T> *tw,
+                                          void *const *S, Put put)
 {
     constexpr int TPL = L / 8;
     constexpr int LP = lpad_len(L);
-    __shared__ __attribute__((aligned(16))) char raw[CB * LP * sizeof(cpx<double>)];
-    __shared__ int colw[CB], cols[CB];
     cpx<double> *ldd = (cpx<double> *)raw;
     cpx<T> *ldt = (cpx<T> *)raw;
-    const Grid &g = a.g;
-    const int tile = xcd_tile(blockIdx.x, gridDim.x);
-    if (tile >= x_tiles(g, CB, false)) return;
-    tile_columns<CB>(colw, cols, g, false, tile);
-    lds_barrier();
     const int c = threadIdx.x / TPL, j = threadIdx.x % TPL;
     const int cw = colw[c];
     const int ly = cw >= 0 ? cw / g.nzp : 0, lz = cw >= 0 ? cw % g.nzp : 0;
     const double Ky = kwave(ly, g.ny) * g.kf[1], Kz = lz * g.kf[2];
-    const cpx<T> *tw = (const cpx<T> *)a.tw;
-    const int ncw = g.ny * g.nzp, ncs = g.ny * g.nzc;
+    const int ncw = g.ny * g.nzp;
     cpx<double> m[3][8];
 #pragma unroll
     for (int comp = 0; comp < 3; comp++) {
-        stage_in<L, CB>(ldd, (const cpx<double> *)a.G[comp], cols, ncs);
+        put(comp);
         lds_barrier();
 #pragma unroll
         for (int s = 0; s < 8; s++) m[comp][s] = ldd[c * LP + lpad(j + TPL * s)];
@@ -89,9 +85,29 @@ __global__ void __launch_bounds__(CB *(L / 8)) k_xinv_sym(XsymArgs a)
 #pragma unroll
         for (int s = 0; s < 8; s++) ldt[c * LP + lpad(j + TPL * s)] = v[s];
         lds_barrier();
-        stage_out<L, CB>(ldt, (cpx<T> *)a.S[t], colw, ncw);
+        stage_out<L, CB>(ldt, (cpx<T> *)S[t], colw, ncw);
         lds_barrier();
     }
+}
+
+template <typename T, int L, int CB>
+__global__ void __launch_bounds__(CB *(L / 8)) k_xinv_sym(XsymArgs a)
+{
+    constexpr int LP = lpad_len(L);
+    __shared__ __attribute__((aligned(16))) char raw[CB * LP * sizeof(cpx<double>)];
+    __shared__ int colw[CB], cols[CB];
+    const Grid &g = a.g;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    if (tile >= x_tiles(g, CB, false)) return;
+    tile_columns<CB>(colw, cols, g, false, tile);
+    lds_barrier();
+    const int ncs = g.ny * g.nzc;
+    // the three cotangent tiles' loads are in flight together
+    cpx<double> gr[3][8];
+#pragma unroll
+    for (int comp = 0; comp < 3; comp++) stage_load<L, CB>(gr[comp], (const cpx<double> *)a.G[comp], cols, ncs);
+    xsym_tile<T, L, CB>(g, raw, colw, (const cpx<T> *)a.tw, a.S,
+                        [&](int k) { stage_put<L, CB>((cpx<double> *)raw, gr[k], cols); });
 }
 
 // q += u * v on the packed real points of this thread (u from the tape line).
@@ -137,10 +153,24 @@ __global__ void __launch_bounds__(256) k_z_adj(ZArgs a)
             const int t = m == 0 ? sig[0][jj] : (m == 1 ? sig[1][jj] : sig[2][jj]);
             const cpx<T> *X = (const cpx<T> *)(t == 0 ? a.in[0] : t == 1 ? a.in[1] : t == 2 ? a.in[2] :
                                                 t == 3 ? a.in[3] : t == 4 ? a.in[4] : a.in[5]);
-            zline_c2r<T, M>(v, X + off, L, twm, twn, j);
-            line_sync<M / 8>();
             const cpx<T> *u = (const cpx<T> *)(jj == 0 ? a.tape[0] : jj == 1 ? a.tape[1] : a.tape[2]);
-            tape_mac<T, M>(q, v, u + lsafe * M, j);
+            if constexpr (sizeof(T) == 4 || M <= 128) {
+                // the sigma line and the tape line are loaded together, before
+                // the transform (longer FP64 lines have no registers to spare
+                // for the tape line across the transform)
+                cpx<T> xm, ut[8];
+                zline_load<T, M>(v, xm, X + off, j);
+#pragma unroll
+                for (int s = 0; s < 8; s++) ut[s] = mk<T>(u[lsafe * M + j + TPL * s].x, u[lsafe * M + j + TPL * s].y);
+                zline_c2r_from<T, M>(v, xm, L, twm, twn, j);
+                line_sync<M / 8>();
+#pragma unroll
+                for (int s = 0; s < 8; s++) q[s] = q[s] + mk<T>(ut[s].x * v[s].x, ut[s].y * v[s].y);
+            } else {
+                zline_c2r<T, M>(v, X + off, L, twm, twn, j);
+                line_sync<M / 8>();
+                tape_mac<T, M>(q, v, u + lsafe * M, j);
+            }
         }
         cpx<T> *Q = (cpx<T> *)(m == 0 ? a.out[0] : m == 1 ? a.out[1] : a.out[2]);
         zline_r2c_post<T, M>(q, Q + off, L, twm, twn, j, valid);
@@ -150,12 +180,16 @@ __global__ void __launch_bounds__(256) k_z_adj(ZArgs a)
 struct XadjArgs {
     void *Q[3];
     double *lam[3], *acc[3], *lg[3];
+    void *S[6];  // FUSE: the next stage's six symmetrised fields
     const void *tw;
     Grid g;
     AdjParams ap;
 };
 
-template <typename T, int L, int CB>
+// FUSE: the next stage's cotangent (lg, or lam after the step's last stage)
+// stays in registers instead of going through HBM, and the next stage's
+// xsym_tile runs on it here (lg is not stored).
+template <typename T, int L, int CB, bool FUSE>
 __global__ void __launch_bounds__(CB *(L / 8)) k_xfwd_adj(XadjArgs a)
 {
     constexpr int TPL = L / 8;
@@ -174,7 +208,11 @@ __global__ void __launch_bounds__(CB *(L / 8)) k_xfwd_adj(XadjArgs a)
     const cpx<T> *tw = (const cpx<T> *)a.tw;
     const int ncw = g.ny * g.nzp, ncs = g.ny * g.nzc;
     const double c_sixth = (1.0 / 6) * ap.dt, c_third = (1.0 / 3) * ap.dt, c_half = 0.5 * ap.dt;
+    cpx<double> gn[3][8];  // FUSE: next cotangent, stage_in layout
+#pragma unroll
     for (int comp = 0; comp < 3; comp++) {
+#pragma unroll
+        for (int ii = 0; ii < 8; ii++) gn[comp][ii] = mk<double>(0, 0);
         stage_in<L, CB>(ldt, (const cpx<T> *)a.Q[comp], colw, ncw);
         lds_barrier();
         cpx<T> v[8];
@@ -216,27 +254,33 @@ __global__ void __launch_bounds__(CB *(L / 8)) k_xfwd_adj(XadjArgs a)
                 }
                 const double Kx = kwave(lx, g.nx) * g.kf[0];
                 const double ks = (Kx * Kx + Ky * Ky) + Kz * Kz;
+                cpx<double> nx;
                 if (ap.mode == ADJ_S3) {
                     const double Eh = exp(ap.fh * ks);
                     const cpx<double> lv = c_sixth * r;
                     acc[i] = (Eh * Eh) * lv;
-                    lg[i] = c_third * (Eh * l4[ii]) + ap.dt * (Eh * lv);
+                    nx = c_third * (Eh * l4[ii]) + ap.dt * (Eh * lv);
                 } else if (ap.mode == ADJ_S2) {
                     const double Eh = exp(ap.fh * ks);
                     acc[i] = a4[ii] + Eh * r;
-                    lg[i] = c_third * (Eh * l4[ii]) + c_half * r;
+                    nx = c_third * (Eh * l4[ii]) + c_half * r;
                 } else if (ap.mode == ADJ_S1) {
                     const double Eh = exp(ap.fh * ks), E1 = exp(ap.f1 * ks);
                     acc[i] = a4[ii] + Eh * r;
-                    lg[i] = c_sixth * (E1 * l4[ii]) + c_half * (Eh * r);
+                    nx = c_sixth * (E1 * l4[ii]) + c_half * (Eh * r);
                 } else {
                     const double E1 = exp(ap.f1 * ks);
-                    lam[i] = (E1 * l4[ii] + a4[ii]) + r;
+                    nx = (E1 * l4[ii] + a4[ii]) + r;
+                    lam[i] = nx;
                 }
+                if (ap.mode != ADJ_S0 && !FUSE) lg[i] = nx;
+                if constexpr (FUSE) gn[comp][ii] = nx;
             }
         }
         lds_barrier();
     }
+    if constexpr (FUSE)
+        xsym_tile<T, L, CB>(g, raw, colw, tw, a.S, [&](int k) { stage_put<L, CB>(ldd, gn[k], cols); });
 }
 
 template <typename T>
@@ -279,9 +323,12 @@ hipError_t launch_z_adj(const Grid &g, void *const *S, void *const *Q, void *con
 
 template <typename T>
 hipError_t launch_xfwd_adj(const Grid &g, const AdjParams &ap, void *const *Q, double *const *lam,
-                           double *const *acc, double *const *lg, const void *twx, hipStream_t st)
+                           double *const *acc, double *const *lg, const void *twx, hipStream_t st,
+                           void *const *S_next)
 {
     XadjArgs a{};
+    if (S_next && ap.mode == ADJ_RHS) return hipErrorInvalidValue;
+    for (int i = 0; i < 6; i++) a.S[i] = S_next ? S_next[i] : nullptr;
     for (int i = 0; i < 3; i++) {
         a.Q[i] = Q[i];
         a.lam[i] = lam ? lam[i] : nullptr;
@@ -293,8 +340,12 @@ hipError_t launch_xfwd_adj(const Grid &g, const AdjParams &ap, void *const *Q, d
     a.ap = ap;
     TG_DISPATCH_L(g.nx, {
         constexpr int CB = col_block<T>(LL, sizeof(cpx<double>));
-        hipLaunchKernelGGL((k_xfwd_adj<T, LL, CB>), dim3(x_grid(x_tiles(g, CB, false))), dim3(CB * (LL / 8)), 0, st,
-                           a);
+        if (S_next)
+            hipLaunchKernelGGL((k_xfwd_adj<T, LL, CB, true>), dim3(x_grid(x_tiles(g, CB, false))),
+                               dim3(CB * (LL / 8)), 0, st, a);
+        else
+            hipLaunchKernelGGL((k_xfwd_adj<T, LL, CB, false>), dim3(x_grid(x_tiles(g, CB, false))),
+                               dim3(CB * (LL / 8)), 0, st, a);
     });
     return hipGetLastError();
 }
@@ -304,7 +355,8 @@ hipError_t launch_xfwd_adj(const Grid &g, const AdjParams &ap, void *const *Q, d
     template hipError_t launch_z_adj<T>(const Grid &, void *const *, void *const *, void *const *, const void *,  \
                                         const void *, hipStream_t);                                              \
     template hipError_t launch_xfwd_adj<T>(const Grid &, const AdjParams &, void *const *, double *const *,       \
-                                           double *const *, double *const *, const void *, hipStream_t);
+                                           double *const *, double *const *, const void *, hipStream_t,         \
+                                           void *const *);
 
 TG_ADJ_INST(double)
 TG_ADJ_INST(float)

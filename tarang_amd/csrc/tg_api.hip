@@ -571,17 +571,21 @@ int record_steps(tg_plan *p, double *const V[3], const tg_fluid *f, int nsteps)
 // One transposed right-hand side application with the stage epilogue:
 // cotangent G, taped velocity tp, outputs per AdjParams.
 template <typename T>
+// One backward stage.  sym_done: the symmetrised fields of this stage are
+// already in S (made by the previous stage's fused x pass); fuse_next: this
+// stage's x pass also makes the next stage's symmetrised fields (the next
+// stage then skips launch_xinv_sym and its cotangent never goes to HBM).
 int adj_stage(tg_plan *p, double *const G[3], void *const tp[3], const AdjParams &ap, double *const lam[3],
-              double *const acc[3], double *const lg[3])
+              double *const acc[3], double *const lg[3], bool sym_done = false, bool fuse_next = false)
 {
     void *S[6] = {p->P[0], p->P[1], p->P[2], p->P[3], p->P[4], p->P[5]};
     void *Q[3] = {p->P[6], p->P[7], p->P[8]};
-    { Prof pr(p, TG_KERNEL_X_INV_SYM); HIPCHK(launch_xinv_sym<T>(p->g, G, S, p->twx, p->stream)); }
+    if (!sym_done) { Prof pr(p, TG_KERNEL_X_INV_SYM); HIPCHK(launch_xinv_sym<T>(p->g, G, S, p->twx, p->stream)); }
     { Prof pr(p, TG_KERNEL_Y_FFT); HIPCHK(launch_colfft<T>(p->g, 1, +1, S, 6, p->twy, p->stream)); }
     { Prof pr(p, TG_KERNEL_Z_ADJ); HIPCHK(launch_z_adj<T>(p->g, S, Q, tp, p->twm, p->twn, p->stream)); }
     { Prof pr(p, TG_KERNEL_Y_FFT); HIPCHK(launch_colfft<T>(p->g, 1, -1, Q, 3, p->twy, p->stream)); }
     Prof pr(p, TG_KERNEL_X_FWD_ADJ);
-    HIPCHK(launch_xfwd_adj<T>(p->g, ap, Q, lam, acc, lg, p->twx, p->stream));
+    HIPCHK(launch_xfwd_adj<T>(p->g, ap, Q, lam, acc, lg, p->twx, p->stream, fuse_next ? S : nullptr));
     return TG_OK;
 }
 
@@ -598,7 +602,10 @@ AdjParams adj_params(const tg_plan *p, const tg_fluid *f, int mode)
 
 // lam <- (dV_{n+1}/dV_n)^T lam for the taped step slot `step`.
 template <typename T>
-int adj_step(tg_plan *p, int step, double *const lam[3])
+// first: this step's first stage makes its own symmetrised fields; last: no
+// later stage follows within the segment (the recomputation of the previous
+// segment reuses the S buffers), so the last stage does not fuse.
+int adj_step(tg_plan *p, int step, double *const lam[3], bool first, bool last)
 {
     const size_t nc = p->nc;
     double *lg[3] = {p->adj.lg, p->adj.lg + 2 * nc, p->adj.lg + 4 * nc};
@@ -606,7 +613,9 @@ int adj_step(tg_plan *p, int step, double *const lam[3])
     const tg_fluid *f = &p->adj.fluid;
     for (int s = 3; s >= 0; s--) {
         void *tp[3] = {tape_ptr(p, step, s, 0), tape_ptr(p, step, s, 1), tape_ptr(p, step, s, 2)};
-        int rc = adj_stage<T>(p, s == 3 ? lam : lg, tp, adj_params(p, f, s), lam, acc, lg);
+        const bool sym_done = !(s == 3 && first);
+        const bool fuse_next = !(s == 0 && last);
+        int rc = adj_stage<T>(p, s == 3 ? lam : lg, tp, adj_params(p, f, s), lam, acc, lg, sym_done, fuse_next);
         if (rc) return rc;
     }
     return TG_OK;
@@ -664,7 +673,7 @@ int run_adjoint_backward(tg_plan *p, double *const lam[3])
             a.cached_seg = seg;
         }
         for (int st = n - 1; st >= 0; st--) {
-            int rc = adj_step<T>(p, st, lam);
+            int rc = adj_step<T>(p, st, lam, st == n - 1, st == 0);
             if (rc) return rc;
         }
     }

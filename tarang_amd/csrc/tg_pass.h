@@ -79,6 +79,31 @@ __device__ __forceinline__ void stage_in(cpx<S> *lds, const cpx<G> *__restrict__
     for (int i = 0; i < 8; i++) lds[c * LP + lpad(r0 + TPL * i)] = col >= 0 ? cvt<S>(x[i]) : mk<S>(0, 0);
 }
 
+// stage_in in two halves, so that several tiles' loads can be in flight
+// together: stage_load fills x[8] (this thread's share), stage_put writes it
+// to LDS (zeros for a column outside the field).
+template <int L, int CB, typename G>
+__device__ __forceinline__ void stage_load(cpx<G> *x, const cpx<G> *__restrict__ f, const int *cols, int ncol_total)
+{
+    constexpr int TPL = L / 8;
+    const int c = threadIdx.x % CB, r0 = threadIdx.x / CB;
+    const cpx<G> *p = f + (size_t)r0 * ncol_total + max(cols[c], 0);
+    const size_t step = (size_t)TPL * ncol_total;
+#pragma unroll
+    for (int i = 0; i < 8; i++) x[i] = mk<G>(p[i * step].x, p[i * step].y);
+}
+
+template <int L, int CB, typename S, typename G>
+__device__ __forceinline__ void stage_put(cpx<S> *lds, const cpx<G> *x, const int *cols)
+{
+    constexpr int TPL = L / 8;
+    constexpr int LP = lpad_len(L);
+    const int c = threadIdx.x % CB, r0 = threadIdx.x / CB;
+    const bool in = cols[c] >= 0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) lds[c * LP + lpad(r0 + TPL * i)] = in ? cvt<S>(x[i]) : mk<S>(0, 0);
+}
+
 template <int L, int CB, typename S, typename G>
 __device__ __forceinline__ void stage_out(const cpx<S> *lds, cpx<G> *__restrict__ f, const int *cols, int ncol_total)
 {
