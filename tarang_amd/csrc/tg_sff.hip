@@ -19,6 +19,8 @@
 // every pass forms W^k = W^j W^(TPL s) from those TPL + 8 table entries
 // (QuarterTw), so all sin/cos passes round alike and the fused kernels keep
 // only 8 + TPL entries in LDS.  Along y and z the FFF passes are reused.
+#include <type_traits>
+
 #include "tg_device.h"
 #include "tg_pass.h"
 #include "tg_sff.h"
@@ -539,37 +541,30 @@ __global__ void __launch_bounds__(CB *(L / 8), 2) k_x_rbc(XrbcArgs a)
         }
         lds_barrier();
     }
-#pragma unroll 1
-    for (int comp = 0; comp < 3; comp++) {
-        // explicit branches: a select on comp would be folded into a
-        // dynamically indexed N[comp][s], which moves N to scratch memory
+    // the three components unrolled: N[K] and the field pointers are static
+    // (a runtime component index put N in scratch memory)
+    auto vel_comp = [&](auto kc) {
+        constexpr int K = decltype(kc)::value;
         cpx<double> r[8];
-        if (comp == 0) {
 #pragma unroll
-            for (int s = 0; s < 8; s++) r[s] = cvt<double>(N[0][s]);
-        } else if (comp == 1) {
-#pragma unroll
-            for (int s = 0; s < 8; s++) r[s] = cvt<double>(N[1][s]);
-        } else {
-#pragma unroll
-            for (int s = 0; s < 8; s++) r[s] = cvt<double>(N[2][s]);
-        }
+        for (int s = 0; s < 8; s++) r[s] = cvt<double>(N[K][s]);
         if (stage == 0) {
 #pragma unroll
             for (int s = 0; s < 8; s++) ldd[c * LP + lpad(j + TPL * s)] = r[s];
             lds_barrier();
-            stage_out<L, CB>(ldd, (cpx<double> *)sel_ptr((void *const *)a.out, comp, 3), cols, ncs);
+            stage_out<L, CB>(ldd, (cpx<double> *)a.out[K], cols, ncs);
             lds_barrier();
         } else {
-            double *Vc = (double *)sel_ptr((void *const *)a.V, comp, 3);
-            double *Ac = (double *)sel_ptr((void *const *)a.acc, comp, 3);
-            void *Uc = sel_ptr(a.U, comp, 3);
-            int jj = j;  // keeps the per-thread LDS addresses inside the loop
+            int jj = j;  // keeps the per-thread LDS addresses inside each component
             asm volatile("" : "+v"(jj));
             rbc_lawson<T, L, CB>(g, rp, c, jj, ly, lz, Ky, Kz, colw, cols, ldd, ldb, ldt, twl, twql, r, rp.fh_u,
-                                 rp.f1_u, Vc, Ac, Uc, (comp == 0 && stage < 4) ? a.f_out[1] : nullptr, comp == 0);
+                                 rp.f1_u, a.V[K], a.acc[K], a.U[K], (K == 0 && stage < 4) ? a.f_out[1] : nullptr,
+                                 K == 0);
         }
-    }
+    };
+    vel_comp(std::integral_constant<int, 0>{});
+    vel_comp(std::integral_constant<int, 1>{});
+    vel_comp(std::integral_constant<int, 2>{});
 }
 
 // ---------------------------------------------------------------------------
