@@ -144,11 +144,11 @@ __device__ __forceinline__ void lds_barrier()
 // lds_barrier for a point where a direct-to-LDS load into a tile that no
 // thread touches until its own wait is in flight: the workgroup-scope release
 // fence would wait for that load too (it counts as a pending LDS write).  The
-// wave's own LDS instructions are retired explicitly (s_waitcnt lgkmcnt(0)),
-// the wavefront-scope fences keep the compiler from moving LDS accesses
-// across the barrier, and the LDS executes the instructions of the waves in
-// issue order, so accesses issued before the barrier by one wave complete
-// before those issued after it by another.
+// wave's own LDS instructions are complete before it reaches the barrier
+// (s_waitcnt lgkmcnt(0)), as with the workgroup fence, and the
+// wavefront-scope fences keep the compiler from moving LDS accesses across
+// the barrier; the direct-to-LDS tile is published later by its own wait
+// (dma_wait) and an lds_barrier.
 __device__ __forceinline__ void lds_barrier_async()
 {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
