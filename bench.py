@@ -365,9 +365,10 @@ def adjoint_cost(api, torch, np, n=256, nsteps=200, every=20, nu=1e-3, dt=1e-3, 
     m = ((K >= r_lo) & (K < r_hi)).to(torch.float64)
     m[:, :, 0] *= 0.5
     adj = api.Adjoint_incompress(U, dt, checkpoint_every=every)
-    # allocation of checkpoints and tape outside the timed region
+    # allocation of checkpoints and tape outside the timed region: a sweep of
+    # the full length (the buffers grow with the number of segments)
     U.set_field(V0)
-    adj.Forward(every)
+    adj.Forward(nsteps)
     U.set_field(V0)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
