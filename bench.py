@@ -57,6 +57,8 @@ def parse():
     ap.add_argument("--adjoint-only", action="store_true", help="run only the adjoint cost measurement")
     ap.add_argument("--no-rbc", action="store_true", help="skip the Rayleigh-Benard measurement")
     ap.add_argument("--rbc-only", action="store_true", help="run only the Rayleigh-Benard measurement")
+    ap.add_argument("--no-mhd", action="store_true", help="skip the MHD precision measurement")
+    ap.add_argument("--mhd-only", action="store_true", help="run only the MHD precision measurement")
     ap.add_argument("--copy-calib", action="store_true", help="run a plain 1 GiB device copy (counter calibration)")
     return ap.parse_args()
 
@@ -117,6 +119,9 @@ def main():
         return
     if args.rbc_only:
         print(json.dumps({"rbc": rbc_rate(api, torch, np)}))
+        return
+    if args.mhd_only:
+        print(json.dumps({"mhd": mhd_precision(api, torch, np)}))
         return
 
     N = (args.n, args.n, args.n)
@@ -250,6 +255,9 @@ def main():
         if not args.no_rbc:
             extra["rbc"] = rbc_rate(api, torch, np)
             torch.cuda.empty_cache()
+        if not args.no_mhd:
+            extra["mhd"] = mhd_precision(api, torch, np)
+            torch.cuda.empty_cache()
 
     cpu = None
     if not args.no_cpu:
@@ -280,10 +288,11 @@ def main():
     print(json.dumps(out))
 
 
-def rbc_rate(api, torch, np, N=(512, 1024, 1024), steps=3, warmup=1, Ra=1e8, Pr=1.0, dt=1e-3, seed=6):
+def rbc_run(api, torch, np, prec, N, steps, warmup, Ra, Pr, dt, seed, profile):
+    """warmup + steps (timed) + 1 (profiled) RK4 steps of the RBC workload in one precision mode."""
     import ctypes
     from tarang_amd._lib import KERNELS, check
-    b = api.SFF_Basis(N, precision=api.PREC_FP64, dealias=api.DEALIAS_REF_CORNER, reality=2)
+    b = api.SFF_Basis(N, precision=prec, dealias=api.DEALIAS_REF_CORNER, reality=2)
     U, T = api.FluidVF(b), api.FluidSF(b)
     nx, ny, nzc = b.shape_complex_array
     gen = torch.Generator(device="cuda")
@@ -303,30 +312,103 @@ def rbc_rate(api, torch, np, N=(512, 1024, 1024), steps=3, warmup=1, Ra=1e8, Pr=
     torch.cuda.synchronize()
     ms = (time.perf_counter() - t0) * 1e3 / steps
     L = b.L
-    check("tg_profile_reset", L.tg_profile_reset(b.plan))
-    check("tg_profile_enable", L.tg_profile_enable(b.plan, 1))
-    ta.Time_advance_step(1)
-    check("tg_profile_enable", L.tg_profile_enable(b.plan, 0))
     kern = {}
-    for k, name in enumerate(KERNELS):
-        kms, cnt = ctypes.c_double(), ctypes.c_int64()
-        check("tg_profile_read", L.tg_profile_read(b.plan, k, ctypes.byref(kms), ctypes.byref(cnt)))
-        if cnt.value:
-            kern[name] = {"ms_per_step": round(kms.value, 3), "launches_per_step": cnt.value}
-    nu = api.Correlation.Get_Nusselt_no(U, T, Ra, Pr)
-    S = nx * ny * nzc * 16
-    model = 8 * (4 + 9) * S + 11 * 4 * S
-    print(f"[bench] rbc {N}: {ms:.1f} ms/step, Nu {nu:.12g}", file=sys.stderr, flush=True)
-    out = {"workload": f"SFF Rayleigh-Benard {N[0]}x{N[1]}x{N[2]} FP64, Ra={Ra:g}, Pr={Pr:g}, corner dealias, "
-                       "weak reality, theta noise 1e-3 in the first 32 modes",
-           "steps_per_s": round(1e3 / ms, 4), "ms_per_step": round(ms, 2), "steps": steps,
-           "step_roofline": {"bytes_model": model, "achieved": round(model / (ms * 1e-3) / 1e9, 1),
-                             "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                             "frac": round(model / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
-           "kernels": kern, "nusselt": nu, "energy_theta": T.Compute_total_energy(),
-           "plan_GB": round(b.device_bytes() / 1e9, 1)}
+    if profile:
+        check("tg_profile_reset", L.tg_profile_reset(b.plan))
+        check("tg_profile_enable", L.tg_profile_enable(b.plan, 1))
+    ta.Time_advance_step(1)
+    if profile:
+        check("tg_profile_enable", L.tg_profile_enable(b.plan, 0))
+        for k, name in enumerate(KERNELS):
+            kms, cnt = ctypes.c_double(), ctypes.c_int64()
+            check("tg_profile_read", L.tg_profile_read(b.plan, k, ctypes.byref(kms), ctypes.byref(cnt)))
+            if cnt.value:
+                kern[name] = {"ms_per_step": round(kms.value, 3), "launches_per_step": cnt.value}
+    res = {"ms": ms, "kernels": kern, "nusselt": api.Correlation.Get_Nusselt_no(U, T, Ra, Pr),
+           "energy_u": U.Compute_total_energy(), "energy_theta": T.Compute_total_energy(),
+           "plan_GB": round(b.device_bytes() / 1e9, 1), "S": nx * ny * nzc * 16}
     del ta, U, T, b
-    return out
+    torch.cuda.empty_cache()
+    return res
+
+
+def rbc_rate(api, torch, np, N=(512, 1024, 1024), steps=3, warmup=1, Ra=1e8, Pr=1.0, dt=1e-3, seed=6):
+    r = rbc_run(api, torch, np, api.PREC_FP64, N, steps, warmup, Ra, Pr, dt, seed, True)
+    ms, S = r["ms"], r["S"]
+    model = 8 * (4 + 9) * S + 11 * 4 * S
+    print(f"[bench] rbc {N}: {ms:.1f} ms/step, Nu {r['nusselt']:.12g}", file=sys.stderr, flush=True)
+    # FP32 transforms from the same fields, the same steps: Nusselt number and energies against FP64
+    r32 = rbc_run(api, torch, np, api.PREC_FP32, N, steps, warmup, Ra, Pr, dt, seed, False)
+    rel = lambda a, b: abs(a - b) / abs(b)
+    print(f"[bench] rbc FP32: {r32['ms']:.1f} ms/step, Nu {r32['nusselt']:.12g}", file=sys.stderr, flush=True)
+    return {"workload": f"SFF Rayleigh-Benard {N[0]}x{N[1]}x{N[2]} FP64, Ra={Ra:g}, Pr={Pr:g}, corner dealias, "
+                        "weak reality, theta noise 1e-3 in the first 32 modes",
+            "steps_per_s": round(1e3 / ms, 4), "ms_per_step": round(ms, 2), "steps": steps,
+            "step_roofline": {"bytes_model": model, "achieved": round(model / (ms * 1e-3) / 1e9, 1),
+                              "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                              "frac": round(model / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+            "kernels": r["kernels"], "nusselt": r["nusselt"], "energy_u": r["energy_u"],
+            "energy_theta": r["energy_theta"], "plan_GB": r["plan_GB"],
+            "fp32": {"ms_per_step": round(r32["ms"], 2), "speedup_vs_fp64": round(ms / r32["ms"], 3),
+                     "steps_compared": warmup + steps + 1,
+                     "nusselt_rel_err_vs_fp64": rel(r32["nusselt"], r["nusselt"]),
+                     "energy_u_rel_err_vs_fp64": rel(r32["energy_u"], r["energy_u"]),
+                     "energy_theta_rel_err_vs_fp64": rel(r32["energy_theta"], r["energy_theta"]),
+                     "plan_GB": r32["plan_GB"]}}
+
+
+def mhd_run(api, torch, np, prec, N, steps, nu, eta, dt):
+    b = api.FFF_Basis(N, precision=prec)
+    U, B = api.FluidVF(b, nu), api.FluidVF(b, eta)
+    U.Init_cond_spectrum(kind=1, kmin=2.0, kmax=170.0, energy=0.5, seed=11)
+    B.Init_cond_spectrum(kind=1, kmin=2.0, kmax=170.0, energy=0.25, seed=12)
+    ta = api.Time_advance_incompress_MHD(U, B, dt)
+    ta.Time_advance_step(1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ta.Time_advance_step(steps - 1)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / (steps - 1)
+    radii = b.default_radii(20)
+    u2b, b2b = api.EnergyTr(b, radii, radii).Compute_flux_MHD(U, B)
+    res = {"ms": ms, "E_u": U.Compute_total_energy(), "E_b": B.Compute_total_energy(),
+           "Hc": api.Correlation.Get_cross_helicity(U, B),
+           "Ek_u": sum(b.Compute_shell_spectrum(a) for a in U.cvf),
+           "Ek_b": sum(b.Compute_shell_spectrum(a) for a in B.cvf),
+           "u2b": u2b, "b2b": b2b, "fin": np.isfinite(radii[1:])}
+    del ta, U, B, b
+    torch.cuda.empty_cache()
+    return res
+
+
+def mhd_precision(api, torch, np, n=512, steps=100, nu=1e-3, eta=1e-3, dt=1e-3):
+    """FFF MHD (Elsasser form) at n^3, nu = eta = 1e-3, seeded solenoidal u and b with E(k) ~ k^-5/3 on
+    2 <= k <= 170 and E_b = 0.5 E_u: the same steps with FP64 and with FP32 transforms and products (FP64
+    state, FP64 reductions); relative errors of the FP32 run against the FP64 run."""
+    N = (n, n, n)
+    r64 = mhd_run(api, torch, np, api.PREC_FP64, N, steps, nu, eta, dt)
+    r32 = mhd_run(api, torch, np, api.PREC_FP32, N, steps, nu, eta, dt)
+    rel = lambda a, b: abs(a - b) / abs(b)
+
+    def rel_arr(a, b, mask=None):
+        a, b = np.asarray(a), np.asarray(b)
+        if mask is not None:
+            a, b = a[1:][mask], b[1:][mask]
+        return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+    print(f"[bench] mhd {n}^3 x {steps}: FP64 {r64['ms']:.1f} ms/step, FP32 {r32['ms']:.1f} ms/step",
+          file=sys.stderr, flush=True)
+    return {"workload": f"FFF MHD {n}^3, nu = eta = {nu:g}, dt = {dt:g}, u and b seeded with E(k) ~ k^-5/3 on "
+                        f"2 <= k <= 170, E_b = 0.5 E_u; {steps} steps",
+            "fp64_ms_per_step": round(r64["ms"], 2), "fp32_ms_per_step": round(r32["ms"], 2),
+            "fp32_speedup": round(r64["ms"] / r32["ms"], 3),
+            "fp32_rel_err_vs_fp64": {
+                "energy_u": rel(r32["E_u"], r64["E_u"]), "energy_b": rel(r32["E_b"], r64["E_b"]),
+                "cross_helicity": rel(r32["Hc"], r64["Hc"]),
+                "spectrum_u": rel_arr(r32["Ek_u"], r64["Ek_u"]), "spectrum_b": rel_arr(r32["Ek_b"], r64["Ek_b"]),
+                "flux_u_to_b": rel_arr(r32["u2b"], r64["u2b"], r64["fin"]),
+                "flux_b_to_b": rel_arr(r32["b2b"], r64["b2b"], r64["fin"])},
+            "fp64": {"energy_u": r64["E_u"], "energy_b": r64["E_b"], "cross_helicity": r64["Hc"]}}
 
 
 def measured_traffic(kernel_class, N, precision):
