@@ -427,7 +427,8 @@ def measured_traffic(kernel_class, N, precision):
     return None
 
 
-def adjoint_cost(api, torch, np, n=256, nsteps=200, every=20, nu=1e-3, dt=1e-3, r_lo=8.0, r_hi=16.0):
+def adjoint_cost(api, torch, np, n=256, nsteps=200, every=20, nu=1e-3, dt=1e-3, r_lo=8.0, r_hi=16.0,
+                 hbm_sized=True):
     N = (n, n, n)
     b = api.FFF_Basis(N, precision=api.PREC_FP64, dealias=api.DEALIAS_REF_CORNER, strong_reality=False)
     U = api.FluidVF(b, nu)
@@ -479,15 +480,46 @@ def adjoint_cost(api, torch, np, n=256, nsteps=200, every=20, nu=1e-3, dt=1e-3, 
         if cnt.value:
             kern[name] = {"ms_per_step": round(ms.value / every, 3), "launches_per_step": cnt.value // every}
     nc = n * n * (n // 2 + 1)
-    tape_bytes = (nsteps + every - 1) // every * 3 * nc * 16 + every * 12 * n * n * (n // 2) * 16
+
+    def tape_bytes(e):
+        # checkpoints (FP64 spectral) + four stage tapes of three real fields per step of one segment
+        return (nsteps + e - 1) // e * 3 * nc * 16 + e * 12 * n * n * (n // 2) * 16
+
     adj.Free()
     print(f"[bench] adjoint {n}^3 x {nsteps}: forward {t_fwd:.3f} s, forward+backward {t_adj:.3f} s",
           file=sys.stderr, flush=True)
-    return {"workload": f"{n}^3 FP64, {nsteps} unforced RK4 steps, checkpoints every {every}, "
-                        f"loss = E in {r_lo:g} <= |k| < {r_hi:g}",
-            "forward_s": round(t_fwd, 4), "forward_and_backward_s": round(t_adj, 4),
-            "cost_ratio": round(t_adj / t_fwd, 3), "target_ratio": 3.0, "loss": loss, "grad_norm": gnorm,
-            "checkpoint_and_tape_GB": round(tape_bytes / 1e9, 2), "segment_kernels": kern}
+    out = {"workload": f"{n}^3 FP64, {nsteps} unforced RK4 steps, checkpoints every {every}, "
+                       f"loss = E in {r_lo:g} <= |k| < {r_hi:g}",
+           "forward_s": round(t_fwd, 4), "forward_and_backward_s": round(t_adj, 4),
+           "cost_ratio": round(t_adj / t_fwd, 3), "target_ratio": 3.0, "loss": loss, "grad_norm": gnorm,
+           "checkpoint_and_tape_GB": round(tape_bytes(every) / 1e9, 2), "segment_kernels": kern}
+    if hbm_sized:
+        # the same sweep with the checkpoint interval sized to the device memory: the largest divisor of
+        # nsteps whose checkpoints and tape fit in 60% of it (fewer recomputed steps, the same gradient)
+        hbm = torch.cuda.get_device_properties(0).total_memory
+        fits = [e for e in range(every, nsteps + 1) if nsteps % e == 0 and tape_bytes(e) <= 0.6 * hbm]
+        e2 = max(fits) if fits else every
+        if e2 != every:
+            adj2 = api.Adjoint_incompress(U, dt, checkpoint_every=e2)
+            U.set_field(V0)
+            adj2.Forward(nsteps)  # allocation outside the timed region
+            U.set_field(V0)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            adj2.Forward(nsteps)
+            lam2 = [2.0 * m * v for v in U.cvf]
+            adj2.Backward(lam2)
+            torch.cuda.synchronize()
+            t_adj2 = time.perf_counter() - t0
+            gnorm2 = float(sum((x.real ** 2 + x.imag ** 2).sum().item() for x in lam2)) ** 0.5
+            adj2.Free()
+            print(f"[bench] adjoint, checkpoints every {e2}: forward+backward {t_adj2:.3f} s", file=sys.stderr,
+                  flush=True)
+            out["hbm_sized_checkpoints"] = {
+                "checkpoint_every": e2, "forward_and_backward_s": round(t_adj2, 4),
+                "cost_ratio": round(t_adj2 / t_fwd, 3), "checkpoint_and_tape_GB": round(tape_bytes(e2) / 1e9, 2),
+                "grad_norm_rel_diff_vs_every_%d" % every: abs(gnorm2 - gnorm) / gnorm}
+    return out
 
 
 def cpu_baseline(N, nu, dt, force, V_dev):
