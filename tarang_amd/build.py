@@ -15,6 +15,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libtarang_amd.so")
 DRIVER = os.path.join(HERE, "tarang_ifluid")
+DRIVER_RBC = os.path.join(HERE, "tarang_rbc")
 SOURCES = ["tg_kernels.hip", "tg_diag.hip", "tg_mhd.hip", "tg_adjoint.hip", "tg_sff.hip", "tg_api.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("TG_OFFLOAD_ARCH", "gfx950")
@@ -45,16 +46,16 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
-    # C++ host driver over the C ABI (tarang_amd.hpp)
-    cmd = [HIPCC, f"--offload-arch={ARCH}", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"),
-           os.path.join(HERE, "host", "ifluid_main.cpp"), "-o", DRIVER + ".tmp", "-L", HERE, "-ltarang_amd",
-           "-Wl,-rpath,$ORIGIN"]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError(f"driver build failed:\n{r.stdout}\n{r.stderr}")
-    os.replace(DRIVER + ".tmp", DRIVER)
+    # C++ host drivers over the C ABI (tarang_amd.hpp)
+    for src, out in (("ifluid_main.cpp", DRIVER), ("rbc_main.cpp", DRIVER_RBC)):
+        cmd = [HIPCC, f"--offload-arch={ARCH}", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"),
+               os.path.join(HERE, "host", src), "-o", out + ".tmp", "-L", HERE, "-ltarang_amd", "-Wl,-rpath,$ORIGIN"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"driver build failed for {src}:\n{r.stdout}\n{r.stderr}")
+        os.replace(out + ".tmp", out)
     if verbose:
-        print(f"built {LIB} and {DRIVER}")
+        print(f"built {LIB}, {DRIVER} and {DRIVER_RBC}")
     return LIB
 
 
