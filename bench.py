@@ -244,7 +244,10 @@ def main():
         tdiag = time.perf_counter() - t0
         extra["diag"] = {"flux_and_shell_to_shell_ms": round(tdiag * 1e3, 1), "spheres": 20, "shells": 20,
                          "shell_to_shell_sum": float(S.sum()), "shell_to_shell_abs_sum": float(np.abs(S).sum()),
-                         "flux_max": float(np.max(flux))}
+                         "flux_max": float(np.max(flux)),
+                         # one RK4 step followed by both diagnostics, as in a run that outputs them every step
+                         "step_with_diagnostics_ms": round(ms_step + tdiag * 1e3, 1),
+                         "steps_per_s_with_diagnostics": round(1e3 / (ms_step + tdiag * 1e3), 3)}
         print(f"[bench] diagnostics {tdiag * 1e3:.1f} ms", file=sys.stderr, flush=True)
         extra["energy"] = E64
         if not args.no_adjoint:

@@ -342,17 +342,25 @@ int run_nlin(tg_plan *p, double *const V[3], double *const out[3], const tg_flui
 // N_i = FT[d_j(x_j w_i)] with x the advecting field X and w = Y (filter:
 // Y restricted to r_lo < K <= r_hi, Fill_array_shell, universal_ET.cc:880-892).
 template <typename T>
+// receiver_ready: U[0..2] still hold the x- and y-inverse of X from the
+// previous call (the z pass only reads them), so only the giver is
+// transformed (consecutive evaluations with one receiver: flux spheres,
+// shell-to-shell givers).
 int run_giver_nlin(tg_plan *p, double *const X[3], double *const Y[3], bool filter, double r_lo, double r_hi,
-                   double *const out[3])
+                   double *const out[3], bool receiver_ready)
 {
     Band nb{};
     {
         Prof pr(p, TG_KERNEL_X_INV);
-        HIPCHK(launch_xinv_state<T>(p->g, nb, X, 3, p->U, nullptr, 0, p->twx, p->stream));
+        if (!receiver_ready) HIPCHK(launch_xinv_state<T>(p->g, nb, X, 3, p->U, nullptr, 0, p->twx, p->stream));
         HIPCHK(launch_xinv_state<T>(p->g, nb, Y, 3, p->U + 3, nullptr, filter ? 8 : 0, p->twx, p->stream, r_lo,
                                     r_hi));
     }
-    { Prof pr(p, TG_KERNEL_Y_FFT); HIPCHK(launch_colfft<T>(p->g, 1, +1, p->U, 6, p->twy, p->stream)); }
+    {
+        Prof pr(p, TG_KERNEL_Y_FFT);
+        if (receiver_ready) HIPCHK(launch_colfft<T>(p->g, 1, +1, p->U + 3, 3, p->twy, p->stream));
+        else HIPCHK(launch_colfft<T>(p->g, 1, +1, p->U, 6, p->twy, p->stream));
+    }
     { Prof pr(p, TG_KERNEL_Z_PRODUCTS); HIPCHK(launch_z_products<T>(p->g, p->U, p->P, 1, p->twm, p->twn, p->stream)); }
     { Prof pr(p, TG_KERNEL_Y_FFT); HIPCHK(launch_colfft<T>(p->g, 1, -1, p->P, 9, p->twy, p->stream)); }
     RkParams rk = rk_params(p, nullptr, -1);
@@ -1131,7 +1139,7 @@ int tg_compute_nlin_giver(tg_plan *plan, double *const V_dev[3], double *const W
         HIPCHK(launch_dealias(plan->g, V_dev, 3, plan->stream));
         HIPCHK(launch_dealias(plan->g, W_dev, 3, plan->stream));
     }
-    return DISPATCH_PREC(plan, run_giver_nlin)(plan, V_dev, W_dev, false, 0.0, 0.0, nlin_dev);
+    return DISPATCH_PREC(plan, run_giver_nlin)(plan, V_dev, W_dev, false, 0.0, 0.0, nlin_dev, false);
 }
 
 int tg_rk4_step(tg_plan *plan, double *const V_dev[3], const tg_fluid *fluid, int nsteps, int flags)
@@ -1197,7 +1205,7 @@ int tg_flux(tg_plan *plan, double *const V_dev[3], const double *radii, int nsph
     if (plan->g.dealias) HIPCHK(launch_dealias(plan->g, V_dev, 3, plan->stream));
     flux[0] = 0.0;
     for (int s = 1; s <= nspheres; s++) {
-        rc = DISPATCH_PREC(plan, run_giver_nlin)(plan, V_dev, V_dev, true, 0.0, radii[s], plan->acc);
+        rc = DISPATCH_PREC(plan, run_giver_nlin)(plan, V_dev, V_dev, true, 0.0, radii[s], plan->acc, s > 1);
         if (rc) return rc;
         double v = 0.0;
         const double *A[3] = {plan->acc[0], plan->acc[1], plan->acc[2]};
@@ -1223,7 +1231,8 @@ int tg_shell_to_shell(tg_plan *plan, double *const V_dev[3], const double *radii
     std::vector<double> row(nr);
     const int kin = max_radius_inside(plan->g);
     for (int m = 1; m < nshells; m++) {
-        rc = DISPATCH_PREC(plan, run_giver_nlin)(plan, V_dev, V_dev, true, radii[m - 1], radii[m], plan->acc);
+        rc = DISPATCH_PREC(plan, run_giver_nlin)(plan, V_dev, V_dev, true, radii[m - 1], radii[m], plan->acc,
+                                                 m > 1);
         if (rc) return rc;
         const double *A[3] = {plan->acc[0], plan->acc[1], plan->acc[2]};
         const double *B[3] = {V_dev[0], V_dev[1], V_dev[2]};
@@ -1302,14 +1311,14 @@ int tg_flux_mhd(tg_plan *plan, double *const V_dev[3], double *const W_dev[3], c
     u2b[0] = b2b[0] = 0.0;
     for (int s = 1; s <= nspheres; s++) {
         // W.nlin = FT[d_j(b_j u<_i)]; u< -> b> = sum_{K > r} 2m Re(N.b*)
-        rc = DISPATCH_PREC(plan, run_giver_nlin)(plan, W_dev, V_dev, true, 0.0, radii[s], plan->acc);
+        rc = DISPATCH_PREC(plan, run_giver_nlin)(plan, W_dev, V_dev, true, 0.0, radii[s], plan->acc, false);
         if (rc) return rc;
         rc = reduce_to_host(plan, RED_SHELL_SINGLE, A, B, 3, 0, nullptr, 1, 1, 0, radii[s], INFINITY, &u2b[s]);
         if (rc) return rc;
     }
     for (int s = 1; s <= nspheres; s++) {
         // U.nlin = FT[d_j(u_j b<_i)]; b< -> b> = -sum_{K > r} 2m Re(N.b*)
-        rc = DISPATCH_PREC(plan, run_giver_nlin)(plan, V_dev, W_dev, true, 0.0, radii[s], plan->acc);
+        rc = DISPATCH_PREC(plan, run_giver_nlin)(plan, V_dev, W_dev, true, 0.0, radii[s], plan->acc, false);
         if (rc) return rc;
         double v = 0.0;
         rc = reduce_to_host(plan, RED_SHELL_SINGLE, A, B, 3, 0, nullptr, 1, 1, 0, radii[s], INFINITY, &v);
