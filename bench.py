@@ -233,7 +233,8 @@ def main():
                          "kernels": kern32}
         del ta32, U32, b32
         torch.cuda.empty_cache()
-        # energy-transfer diagnostics on the FP64 state (20 spheres / 20 shells)
+        # energy-transfer diagnostics on the FP64 state (20 spheres / 20 shells, the same radii): the
+        # two reference calls, then both from one giver evaluation per shell
         radii = b64.default_radii(20)
         et = api.EnergyTr(b64, radii, radii)
         torch.cuda.synchronize()
@@ -241,14 +242,22 @@ def main():
         flux = et.Compute_flux(U64)
         S = et.Compute_shell_tr(U64)
         torch.cuda.synchronize()
+        tsep = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        flux_c, S_c = et.Compute_flux_and_shell_tr(U64)
+        torch.cuda.synchronize()
         tdiag = time.perf_counter() - t0
         extra["diag"] = {"flux_and_shell_to_shell_ms": round(tdiag * 1e3, 1), "spheres": 20, "shells": 20,
+                         "separate_calls_ms": round(tsep * 1e3, 1),
+                         "combined_vs_separate_flux_rel": float(np.max(np.abs(flux_c - flux)) / np.max(np.abs(flux))),
+                         "combined_vs_separate_S_rel": float(np.max(np.abs(S_c - S)) / np.max(np.abs(S))),
                          "shell_to_shell_sum": float(S.sum()), "shell_to_shell_abs_sum": float(np.abs(S).sum()),
                          "flux_max": float(np.max(flux)),
                          # one RK4 step followed by both diagnostics, as in a run that outputs them every step
                          "step_with_diagnostics_ms": round(ms_step + tdiag * 1e3, 1),
                          "steps_per_s_with_diagnostics": round(1e3 / (ms_step + tdiag * 1e3), 3)}
-        print(f"[bench] diagnostics {tdiag * 1e3:.1f} ms", file=sys.stderr, flush=True)
+        print(f"[bench] diagnostics {tdiag * 1e3:.1f} ms (separate calls {tsep * 1e3:.1f} ms)", file=sys.stderr,
+              flush=True)
         extra["energy"] = E64
         if not args.no_adjoint:
             del et

@@ -421,6 +421,17 @@ class EnergyTr {
         return S;
     }
 
+    // both, for flux spheres equal to the shell radii (tg_energy_transfer)
+    void Compute_flux_and_shell_tr(FluidVF &U, std::vector<double> &flux, std::vector<double> &S)
+    {
+        if (flux_radii != shell_radii) throw std::invalid_argument("flux_radii must equal shell_radii");
+        const int n = (int)shell_radii.size() - 1;
+        flux.assign(n + 1, 0.0);
+        S.assign((size_t)(n + 1) * (n + 1), 0.0);
+        auto v = U.V();
+        check("tg_energy_transfer", tg_energy_transfer(basis.plan(), v.data(), shell_radii.data(), n, flux.data(),
+                                                       S.data()));
+    }
     // u< -> b> and b< -> b> fluxes (compute_flux.cc:158-211)
     void Compute_flux(FluidVF &U, FluidVF &W, std::vector<double> &u2b, std::vector<double> &b2b)
     {

@@ -189,6 +189,14 @@ int tg_flux(tg_plan *plan, double *const V_dev[3], const double *radii, int nsph
 /* EnergyTr::Compute_shell_tr(FluidVF&) (compute_shell_tr.cc:57-75):
  * S[(nshells+1)*(nshells+1)], row = giver shell.  Dealiases V in place. */
 int tg_shell_to_shell(tg_plan *plan, double *const V_dev[3], const double *radii, int nshells, double *S);
+/* Both at once when the flux spheres are the shell radii (no single
+ * counterpart: EnergyTr::Compute_flux and Compute_shell_tr called with the
+ * same radii, compute_flux.cc:51-65 and compute_shell_tr.cc:57-75): one giver
+ * evaluation per shell gives the shell-to-shell row S (as tg_shell_to_shell)
+ * and, summed over the givers inside each sphere and the receivers outside
+ * it, the flux (as tg_flux, to round-off; the nonlinear term is linear in the
+ * giver).  radii[0..n], flux[0..n], S[(n+1)*(n+1)].  Dealiases V in place. */
+int tg_energy_transfer(tg_plan *plan, double *const V_dev[3], const double *radii, int n, double *flux, double *S);
 
 /* Per-kernel timing with HIP events on the plan's stream (no counterpart
  * in the reference, whose only timing is the total wall time printed by

@@ -405,6 +405,22 @@ class EnergyTr:
                                              u2b.ctypes.data_as(dp), b2b.ctypes.data_as(dp)))
         return u2b, b2b
 
+    def Compute_flux_and_shell_tr(self, U):
+        """Compute_flux and Compute_shell_tr together, for flux spheres equal to
+        the shell radii (one giver evaluation per shell; tg_energy_transfer)."""
+        if not np.array_equal(np.asarray(self.flux_radii), np.asarray(self.shell_radii)):
+            raise ValueError("Compute_flux_and_shell_tr needs flux_radii == shell_radii")
+        r = np.ascontiguousarray(self.shell_radii, dtype=np.float64)
+        n = len(r) - 1
+        flux = np.zeros(n + 1)
+        S = np.zeros((n + 1, n + 1))
+        b = self.basis
+        dp = ctypes.POINTER(ctypes.c_double)
+        check("tg_energy_transfer", b.L.tg_energy_transfer(b.plan, U.ptrs(), r.ctypes.data_as(dp), n,
+                                                           flux.ctypes.data_as(dp), S.ctypes.data_as(dp)))
+        self.flux_self, self.shelltoshell_self = flux, S
+        return flux, S
+
     def Compute_shell_tr(self, U):
         r = np.ascontiguousarray(self.shell_radii, dtype=np.float64)
         n = len(r) - 1

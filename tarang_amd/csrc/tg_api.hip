@@ -1243,6 +1243,46 @@ int tg_shell_to_shell(tg_plan *plan, double *const V_dev[3], const double *radii
     return TG_OK;
 }
 
+int tg_energy_transfer(tg_plan *plan, double *const V_dev[3], const double *radii, int n, double *flux, double *S)
+{
+    if (!plan || !radii || !S || !flux || n < 2) return fail(TG_ERR_ARG, "bad argument");
+    if (int rb_ = need_basis(plan, 0, "tg_energy_transfer")) return rb_;
+    int rc = check_ptrs((const void *const *)V_dev, 3, "V");
+    if (!rc) rc = set_device(plan);
+    if (rc) return rc;
+    const int nr = n + 1;
+    if (nr > 1000) return fail(TG_ERR_ARG, "too many shells");
+    if (plan->g.dealias) HIPCHK(launch_dealias(plan->g, V_dev, 3, plan->stream));
+    for (int i = 0; i < nr * nr; i++) S[i] = 0.0;
+    // T[m][k]: transfer from giver shell m into receiver shell k, all modes
+    // (the flux's receiver includes the modes beyond the largest inscribed
+    // sphere, which the shell-to-shell matrix leaves out)
+    std::vector<double> row(nr), T((size_t)nr * nr, 0.0);
+    const int kin = max_radius_inside(plan->g);
+    for (int m = 1; m < n; m++) {
+        rc = DISPATCH_PREC(plan, run_giver_nlin)(plan, V_dev, V_dev, true, radii[m - 1], radii[m], plan->acc,
+                                                 m > 1);
+        if (rc) return rc;
+        const double *A[3] = {plan->acc[0], plan->acc[1], plan->acc[2]};
+        const double *B[3] = {V_dev[0], V_dev[1], V_dev[2]};
+        rc = reduce_to_host(plan, RED_SHELL_ALL, A, B, 3, 0, radii, nr, 1, kin, 0, 0, row.data());
+        if (rc) return rc;
+        for (int k = 0; k < nr; k++) S[(size_t)m * nr + k] = -row[k];
+        rc = reduce_to_host(plan, RED_SHELL_ALL_NOCUT, A, B, 3, 0, radii, nr, 1, kin, 0, 0, row.data());
+        if (rc) return rc;
+        for (int k = 0; k < nr; k++) T[(size_t)m * nr + k] = -row[k];
+    }
+    // flux through sphere s: givers inside (shells 1..s), receivers outside
+    flux[0] = 0.0;
+    for (int s = 1; s <= n; s++) {
+        double f = 0.0;
+        for (int m = 1; m <= std::min(s, n - 1); m++)
+            for (int k = s + 1; k < nr; k++) f += T[(size_t)m * nr + k];
+        flux[s] = f;
+    }
+    return TG_OK;
+}
+
 int tg_compute_nlin_mhd(tg_plan *plan, double *const V_dev[3], double *const W_dev[3], double *const nlinV_dev[3],
                         double *const nlinW_dev[3])
 {

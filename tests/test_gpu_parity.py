@@ -191,6 +191,30 @@ def test_flux_and_shell_to_shell():
     assert rel_l2(S, S_ref) < 1e-11
 
 
+@pytest.mark.parametrize("N,radii", [
+    ((32, 32, 32), np.array([0.0, 2.0, 4.0, 6.0, 8.0, 10.0, 12.0, 16.0, np.inf])),
+    ((64, 32, 32), None),
+])
+def test_energy_transfer_combined(N, radii):
+    """tg_energy_transfer: the shell-to-shell matrix and the flux derived from
+    its (uncut) rows against the oracle's direct evaluations.  The last shell
+    (largest inscribed radius, inf) holds the corner modes the flux's receiver
+    includes."""
+    o = to.Oracle(N)
+    V = _init(o, kind=1, seed=12)
+    b = api.FFF_Basis(N)
+    if radii is None:
+        radii = b.default_radii(8)
+    U = api.FluidVF(b)
+    U.set_field([dev(v) for v in V])
+    flux_ref = o.flux([v.copy() for v in V], radii)
+    S_ref = o.shell_to_shell([v.copy() for v in V], radii)
+    flux, S = api.EnergyTr(b, radii, radii).Compute_flux_and_shell_tr(U)
+    log(f"combined N={N}: flux rel {rel_l2(flux, flux_ref):.3e}  S rel {rel_l2(S, S_ref):.3e}")
+    assert rel_l2(flux, flux_ref) < 1e-11
+    assert rel_l2(S, S_ref) < 1e-11
+
+
 def test_init_field():
     N = (32, 32, 32)
     o = to.Oracle(N)
