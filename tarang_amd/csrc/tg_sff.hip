@@ -213,8 +213,10 @@ __global__ void __launch_bounds__(CB *(L / 8)) k_xfwd_sc(XscArgs a)
 
 // ---------------------------------------------------------------------------
 // z pass of the RBC nonlinear terms (Compute_nlin_VxT, compute_diag_offdiag.cc
-// :116-136, and the diag/offdiag products :47-112): one line held in
-// registers, the others streamed, nine c2r and nine r2c per line.
+// :116-136, and the diag/offdiag products :47-112): the three velocity lines
+// are loaded together, transformed once and held in registers for the six
+// velocity products, then theta for the three u theta products; four c2r
+// and nine r2c per line.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int rbc_pair(int ia, int ib)
 {
@@ -233,8 +235,10 @@ __device__ __forceinline__ void *sel_ptr(void *const *p, int i, int n)
     return r;
 }
 
+// Two waves per SIMD also for FP64 512-point lines (a few spilled registers
+// there; 19.3 ms per stage at 512x1024x1024 against 21.0 ms at one wave).
 template <typename T, int M>
-__global__ void __launch_bounds__(256) k_z_rbc(ZArgs a)
+__global__ void __launch_bounds__(256, 2) k_z_rbc(ZArgs a)
 {
     constexpr int TPL = M / 8;
     constexpr int LPB = z_lines_per_block<M>();
@@ -248,24 +252,29 @@ __global__ void __launch_bounds__(256) k_z_rbc(ZArgs a)
     const cpx<T> *twm = (const cpx<T> *)a.twm, *twn = (const cpx<T> *)a.twn;
     cpx<T> *L = lds + li * LP;
     const size_t off = lsafe * a.g.nzp;
-#pragma unroll 1
-    for (int ia = 0; ia < 3; ia++) {
-        cpx<T> h[8], p[8];
-        zline_c2r<T, M>(h, (const cpx<T> *)sel_ptr(a.in, ia, 3) + off, L, twm, twn, j);
+    cpx<T> u[3][8], xm[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) zline_load<T, M>(u[c], xm[c], (const cpx<T> *)a.in[c] + off, j);
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        zline_c2r_from<T, M>(u[c], xm[c], L, twm, twn, j);
         line_sync<M / 8>();
-#pragma unroll
-        for (int s = 0; s < 8; s++) p[s] = mk<T>(h[s].x * h[s].x, h[s].y * h[s].y);
-        zline_r2c_post<T, M>(p, (cpx<T> *)sel_ptr(a.out, ia, 3) + off, L, twm, twn, j, valid);
-#pragma unroll 1
-        for (int ib = ia + 1; ib < 4; ib++) {
-            cpx<T> b[8];
-            zline_c2r<T, M>(b, (const cpx<T> *)sel_ptr(a.in, ib, 4) + off, L, twm, twn, j);
-            line_sync<M / 8>();
-#pragma unroll
-            for (int s = 0; s < 8; s++) p[s] = mk<T>(h[s].x * b[s].x, h[s].y * b[s].y);
-            zline_r2c_post<T, M>(p, (cpx<T> *)sel_ptr(a.out, rbc_pair(ia, ib), 9) + off, L, twm, twn, j, valid);
-        }
     }
+    auto emit = [&](const cpx<T> *x, const cpx<T> *y, int o) {
+        cpx<T> p[8];
+#pragma unroll
+        for (int s = 0; s < 8; s++) p[s] = mk<T>(x[s].x * y[s].x, x[s].y * y[s].y);
+        zline_r2c_post<T, M>(p, (cpx<T> *)a.out[o] + off, L, twm, twn, j, valid);
+    };
+#pragma unroll
+    for (int ia = 0; ia < 3; ia++)
+#pragma unroll
+        for (int ib = ia; ib < 3; ib++) emit(u[ia], u[ib], rbc_pair(ia, ib));
+    cpx<T> t[8];
+    zline_c2r<T, M>(t, (const cpx<T> *)a.in[3] + off, L, twm, twn, j);
+    line_sync<M / 8>();
+#pragma unroll
+    for (int ia = 0; ia < 3; ia++) emit(u[ia], t, rbc_pair(ia, 3));
 }
 
 // ---------------------------------------------------------------------------
