@@ -42,9 +42,14 @@ __global__ void __launch_bounds__(CB *(L / 8)) k_colfft(ColArgs a)
     constexpr int TPL = L / 8;
     constexpr int LP = lpad_len(L);
     __shared__ cpx<T> lds[CB * LP];
+    // xcd_tile: the column blocks of one outer index run in turn on one XCD,
+    // so the rows' 128-byte lines that two blocks share (CB columns of fewer
+    // than 128 bytes) are fetched into one L2
     const int nblk = (a.ncol + CB - 1) / CB;
-    const int outer = blockIdx.x / nblk;
-    const int c0 = (blockIdx.x % nblk) * CB;
+    const int b = xcd_tile(blockIdx.x, gridDim.x);
+    if (b >= a.nouter * nblk) return;
+    const int outer = b / nblk;
+    const int c0 = (b % nblk) * CB;
     cpx<T> *f = (cpx<T> *)a.f[blockIdx.y] + (size_t)outer * a.outer_stride;
     const cpx<T> *tw = (const cpx<T> *)a.tw;
     // thread t moves column t % CB, rows t / CB + TPL i: the eight loads are
@@ -763,7 +768,7 @@ hipError_t launch_colfft(const Grid &g, int axis, int dir, void *const *fields, 
         // at most 512 threads a block (FP32 at 512 points would otherwise
         // take 1024-thread blocks, measured slower)
         constexpr int CB = std::min(col_block<T>(LL, sizeof(cpx<T>)), std::max(1, 512 / (LL / 8)));
-        dim3 grid(a.nouter * ((a.ncol + CB - 1) / CB), nfields);
+        dim3 grid(x_grid(a.nouter * ((a.ncol + CB - 1) / CB)), nfields);
         if (dir < 0) hipLaunchKernelGGL((k_colfft<T, LL, -1, CB>), grid, dim3(CB * (LL / 8)), 0, st, a);
         else hipLaunchKernelGGL((k_colfft<T, LL, +1, CB>), grid, dim3(CB * (LL / 8)), 0, st, a);
     });
