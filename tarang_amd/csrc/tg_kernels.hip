@@ -766,8 +766,11 @@ hipError_t launch_colfft(const Grid &g, int axis, int dir, void *const *fields, 
     }
     TG_DISPATCH_L(L, {
         // at most 512 threads a block (FP32 at 512 points would otherwise
-        // take 1024-thread blocks, measured slower)
-        constexpr int CB = std::min(col_block<T>(LL, sizeof(cpx<T>)), std::max(1, 512 / (LL / 8)));
+        // take 1024-thread blocks, measured slower), except FP32 lines of
+        // 1024 points and more: 1024-thread blocks of 8 columns (64-byte row
+        // segments) run 9% faster at 1024 points than 4 columns
+        constexpr int CB = std::min(col_block<T>(LL, sizeof(cpx<T>)),
+                                    std::max(1, (sizeof(T) == 4 && LL >= 1024 ? 1024 : 512) / (LL / 8)));
         dim3 grid(x_grid(a.nouter * ((a.ncol + CB - 1) / CB)), nfields);
         if (dir < 0) hipLaunchKernelGGL((k_colfft<T, LL, -1, CB>), grid, dim3(CB * (LL / 8)), 0, st, a);
         else hipLaunchKernelGGL((k_colfft<T, LL, +1, CB>), grid, dim3(CB * (LL / 8)), 0, st, a);
