@@ -765,12 +765,11 @@ hipError_t launch_colfft(const Grid &g, int axis, int dir, void *const *fields, 
         a.ncol = g.ny * g.nzp;
     }
     TG_DISPATCH_L(L, {
-        // at most 512 threads a block (FP32 at 512 points would otherwise
-        // take 1024-thread blocks, measured slower), except FP32 lines of
-        // 1024 points and more: 1024-thread blocks of 8 columns (64-byte row
-        // segments) run 9% faster at 1024 points than 4 columns
+        // at most 512 threads a block, except FP32 lines of 512 points and
+        // more: 1024-thread blocks (16 columns at 512 points, 8 at 1024) run
+        // 2% and 9% faster than the 512-thread tiles of half the width
         constexpr int CB = std::min(col_block<T>(LL, sizeof(cpx<T>)),
-                                    std::max(1, (sizeof(T) == 4 && LL >= 1024 ? 1024 : 512) / (LL / 8)));
+                                    std::max(1, (sizeof(T) == 4 && LL >= 512 ? 1024 : 512) / (LL / 8)));
         dim3 grid(x_grid(a.nouter * ((a.ncol + CB - 1) / CB)), nfields);
         if (dir < 0) hipLaunchKernelGGL((k_colfft<T, LL, -1, CB>), grid, dim3(CB * (LL / 8)), 0, st, a);
         else hipLaunchKernelGGL((k_colfft<T, LL, +1, CB>), grid, dim3(CB * (LL / 8)), 0, st, a);

@@ -25,9 +25,11 @@ def _rng_real(shape, seed):
     return r
 
 
-@pytest.mark.parametrize("N", [(16, 16, 16), (32, 16, 64), (64, 64, 64), (8, 32, 16)])
+@pytest.mark.parametrize("N", [(16, 16, 16), (32, 16, 64), (64, 64, 64), (8, 32, 16), (8, 512, 16), (8, 1024, 16)])
 @pytest.mark.parametrize("prec", [api.PREC_FP64, api.PREC_FP32])
 def test_transforms(N, prec):
+    """(8, 512, 16) and (8, 1024, 16): y lines of 512 and 1024 points (the
+    1024-thread FP32 y tiles of 16 and 8 columns)."""
     log(f"test_transforms N={N} prec={prec}")
     o = to.Oracle(N, dealias=to.DEALIAS_NONE)
     b = api.FFF_Basis(N, precision=prec, dealias=api.DEALIAS_NONE)
