@@ -73,8 +73,8 @@ def algorithmic_bytes(kernel, N, prec_bytes, nstate=3):
     if kernel == "z_products":    # 4 stages x (read 3, write 6)
         return 4 * 9 * Sp
     if kernel == "x_nlin_rk":     # 4 stages x (read 6 products, write 3) + per component: V read 4x and
-        # written once (FP64), the stage accumulator written 3x and read 3x (transform precision)
-        return 4 * 9 * Sp + nstate * (5 * S64 + 6 * Sp)
+        # written once, the stage accumulator written 3x and read 3x (both FP64)
+        return 4 * 9 * Sp + nstate * 11 * S64
     if kernel == "reality":       # two planes of V (read+write) and the patched columns
         return 2 * 2 * nstate * N[0] * N[1] * 16 + nstate * N[0] * N[1] * 2 * (16 + 2 * prec_bytes)
     if kernel == "x_inv":
@@ -83,11 +83,11 @@ def algorithmic_bytes(kernel, N, prec_bytes, nstate=3):
 
 
 def step_bytes_model(N, prec_bytes, nstate=3, n_tr=9):
-    """B_step = 8 (n_inv + n_fwd) S_p + n_state (5 S64 + 6 S_p) (two-sweep model: the
-    FP64 state read at every stage and written once, the stage accumulator written
-    and read three times each at the transform precision)."""
+    """B_step = 8 (n_inv + n_fwd) S_p + 11 n_state S64 (two-sweep model: the FP64
+    state read at every stage and written once, the FP64 stage accumulator written
+    and read three times each)."""
     nc = N[0] * N[1] * (N[2] // 2 + 1)
-    return 8 * n_tr * nc * 2 * prec_bytes + nstate * nc * (5 * 16 + 6 * 2 * prec_bytes)
+    return 8 * n_tr * nc * 2 * prec_bytes + nstate * nc * 11 * 16
 
 
 def main():

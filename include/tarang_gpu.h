@@ -35,8 +35,8 @@ extern "C" {
 #define TG_ERR_SHAPE (-2)
 #define TG_ERR_OOM (-3)
 #define TG_ERR_HIP (-4)
-#define TG_ERR_NAN (-5)
-#define TG_ERR_STATE (-6)
+#define TG_ERR_NAN (-5)    /* a time advance wrote a non-finite value into the state */
+#define TG_ERR_STATE (-6)  /* call out of sequence (adjoint backward before a forward sweep) */
 
 /* Bases: FFF (Fourier x, y, z) and SFF (sin/cos along x, Fourier y, z;
  * SFF_pencil.cc).  For SFF, kfactor[0] = pi / L_x and strong_reality selects
@@ -134,8 +134,10 @@ int tg_compute_nlin_giver(tg_plan *plan, double *const V_dev[3], double *const W
 /* Time_advance_incompress::Time_advance_step with integration_scheme RK4
  * (Time_advance.cc:43-65, RK.cc:91-170): nsteps Lawson RK4 steps of V in
  * place.  flags & TG_STEP_CONTINUE reuses the stage-1 transform prepared by
- * the previous call on the same V (valid unless V was modified outside
- * this library since). */
+ * the previous call on the same V and the same forcing band (valid unless V
+ * was modified outside this library since).  Returns TG_ERR_NAN when the
+ * state holds a non-finite value after the call (the reference stops on a
+ * NaN energy, Ifluid_main.cc:127-128); the call waits for its work to end. */
 int tg_rk4_step(tg_plan *plan, double *const V_dev[3], const tg_fluid *fluid, int nsteps, int flags);
 
 /* MHD, Elsasser form.  Nlin_incompress::Compute_nlin(FluidVF& U, FluidVF& W)
@@ -145,7 +147,8 @@ int tg_rk4_step(tg_plan *plan, double *const V_dev[3], const tg_fluid *fluid, in
 int tg_compute_nlin_mhd(tg_plan *plan, double *const V_dev[3], double *const W_dev[3], double *const nlinV_dev[3],
                         double *const nlinW_dev[3]);
 /* Time_advance_incompress::RK4(U, W, P, F) (RK_VF_MHD.cc:126-190), nu =
- * fluid->nu, eta for the magnetic field; no forcing (fluid->force_kind 0). */
+ * fluid->nu, eta for the magnetic field; no forcing (fluid->force_kind 0).
+ * Returns TG_ERR_NAN when u or b holds a non-finite value after the call. */
 int tg_rk4_step_mhd(tg_plan *plan, double *const V_dev[3], double *const W_dev[3], const tg_fluid *fluid, double eta,
                     int nsteps);
 /* Correlation::Get_cross_helicity (Correlation.cc:225-230). */
@@ -168,7 +171,8 @@ int tg_compute_nlin_rbc(tg_plan *plan, double *const V_dev[3], double *T_dev, do
 int tg_compute_rhs_rbc(tg_plan *plan, double *const V_dev[3], double *T_dev, const tg_rbc *rbc,
                        double *const rhs_dev[3], double *rhsT_dev);
 /* Time_advance_incompress::RK4(U, T, P, F) (RK_SF.cc:97-215) followed by the
- * plan's reality condition: nsteps Lawson RK4 steps of (V, T) in place. */
+ * plan's reality condition: nsteps Lawson RK4 steps of (V, T) in place.
+ * Returns TG_ERR_NAN when the state holds a non-finite value after the call. */
 int tg_rk4_step_rbc(tg_plan *plan, double *const V_dev[3], double *T_dev, const tg_rbc *rbc, int nsteps);
 /* Correlation::Get_Nusselt_no (Correlation.cc:144-160), PRLARGE / ULARGE:
  * Nu = 1 + 2 sqrt(Ra Pr) E(V1, T) with the SFF pair energy. */

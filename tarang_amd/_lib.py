@@ -12,6 +12,7 @@ LIB_PATH = os.path.join(HERE, "libtarang_amd.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "tarang_gpu.h")
 
 TG_OK = 0
+ERR_ARG, ERR_SHAPE, ERR_OOM, ERR_HIP, ERR_NAN, ERR_STATE = -1, -2, -3, -4, -5, -6
 PREC_FP64, PREC_FP32 = 0, 1
 DEALIAS_NONE, DEALIAS_REF_CORNER, DEALIAS_TWOTHIRDS = 0, 1, 2
 STEP_CONTINUE = 1

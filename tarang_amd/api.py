@@ -283,6 +283,13 @@ class Time_advance_incompress:
         self.fluid = TgFluid(U.dissipation_coefficient, float(dt), int(force_kind), float(force_eps),
                              float(force_r_in), float(force_r_out))
         self._continue = False
+        self._versions = None
+
+    def _state_versions(self):
+        # torch bumps a tensor's version on every in-place write (set_field,
+        # copy_, zero_, ...); the library's own kernels write through raw
+        # pointers and leave it unchanged
+        return tuple((id(t), t._version) for t in self.U.cvf)
 
     def Compute_force_TO_rhs(self):
         b = self.U.basis
@@ -291,10 +298,17 @@ class Time_advance_incompress:
         self._continue = False
 
     def Time_advance_step(self, nsteps=1, chain=True):
+        """nsteps RK4 steps.  chain: reuse the stage-1 transform left by the
+        previous call when the state tensors were not written in place since
+        (raises TarangGpuError with code TG_ERR_NAN = -5 when the state turns
+        non-finite)."""
         b = self.U.basis
-        flags = STEP_CONTINUE if (chain and self._continue) else 0
+        cont = chain and self._continue and self._versions == self._state_versions()
+        flags = STEP_CONTINUE if cont else 0
+        self._continue = False
         check("tg_rk4_step", b.L.tg_rk4_step(b.plan, self.U.ptrs(), ctypes.byref(self.fluid), int(nsteps), flags))
         self._continue = True
+        self._versions = self._state_versions()
 
     def invalidate(self):
         self._continue = False

@@ -67,9 +67,14 @@ struct tg_plan {
     size_t partial_bytes = 0;
     double *small = nullptr;  // device scratch for reduction outputs / radii
     int red_blocks = 0;
-    // chaining of RK4 steps
+    // chaining of RK4 steps: valid for the same state pointers and band
     bool unext_valid = false;
     double *unext_V[3] = {};
+    Band unext_band{};
+    // non-finite state detection: device flag set by the stage-4 kernels,
+    // read back into pinned host memory at the end of every time-advance call
+    int *nonfinite_dev = nullptr;
+    int *nonfinite_host = nullptr;
     // host-side band config for the current step
     Band band{};
     int x_tile_columns = 0;  // fused x-pass tile width (0 = LDS-budget width)
@@ -202,15 +207,34 @@ void free_plan(tg_plan *p)
         if (q) (void)hipFree(q);
     for (void *q : p->mhd)
         if (q) (void)hipFree(q);
-    void *aptrs[] = {p->adj.ckpt, p->adj.tape, p->adj.vw, p->adj.lg, p->adj.acc};
+    void *aptrs[] = {p->adj.ckpt, p->adj.tape, p->adj.vw, p->adj.lg, p->adj.acc, p->nonfinite_dev};
     for (void *q : aptrs)
         if (q) (void)hipFree(q);
+    if (p->nonfinite_host) (void)hipHostFree(p->nonfinite_host);
     for (auto &r : p->prof_recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
     for (auto e : p->prof_pool) (void)hipEventDestroy(e);
     delete p;
 }
 
 int set_device(tg_plan *p) { HIPCHK(hipSetDevice(p->cfg.device)); return TG_OK; }
+
+// Non-finite state detection around a time-advance call: the reference stops
+// the run when the total energy is NaN (Ifluid_main.cc:127-128); here the
+// stage-4 kernels flag any non-finite value they write into the state.
+int nonfinite_begin(tg_plan *p)
+{
+    HIPCHK(launch_fill_int(p->nonfinite_dev, 1, 0, p->stream));
+    return TG_OK;
+}
+
+int nonfinite_end(tg_plan *p, const char *what)
+{
+    HIPCHK(hipMemcpyAsync(p->nonfinite_host, p->nonfinite_dev, sizeof(int), hipMemcpyDeviceToHost, p->stream));
+    HIPCHK(hipStreamSynchronize(p->stream));
+    if (*p->nonfinite_host)
+        return fail(TG_ERR_NAN, "%s: the state holds non-finite values after the step (NaN or Inf)", what);
+    return TG_OK;
+}
 
 int check_ptrs(const void *const *v, int n, const char *what)
 {
@@ -241,6 +265,8 @@ int ensure_slots(tg_plan *p, const tg_fluid *f)
     p->band = b;
     if (b.kind == 0) return TG_OK;
     if (p->slot_kb != b.kb) {
+        // the stage-1 transform prepared for a continuation has no slots here
+        p->unext_valid = false;
         for (int i = 0; i < 2; i++)
             if (p->slots[i]) { (void)hipFree(p->slots[i]); p->slots[i] = nullptr; }
         p->nslots = (size_t)band_nslots(b.kb);
@@ -376,8 +402,13 @@ int run_rk4(tg_plan *p, double *const V[3], const tg_fluid *f, int nsteps, int f
 {
     int rc = ensure_slots(p, f);
     if (rc) return rc;
-    const bool cont = (flags & TG_STEP_CONTINUE) && p->unext_valid && p->unext_V[0] == V[0] &&
+    const Band &ob = p->unext_band, &nb = p->band;
+    const bool same_band = ob.kind == nb.kind && ob.kb == nb.kb && ob.r_in == nb.r_in && ob.r_out == nb.r_out &&
+                           ob.eps == nb.eps;
+    const bool cont = (flags & TG_STEP_CONTINUE) && p->unext_valid && same_band && p->unext_V[0] == V[0] &&
                       p->unext_V[1] == V[1] && p->unext_V[2] == V[2];
+    rc = nonfinite_begin(p);
+    if (rc) return rc;
     if (!cont) {
         Prof pr(p, TG_KERNEL_X_INV);
         HIPCHK(launch_xinv_state<T>(p->g, p->band, V, 3, p->U, p->slots[0], 1 | (p->band.kind ? 2 : 0), p->twx,
@@ -388,6 +419,7 @@ int run_rk4(tg_plan *p, double *const V[3], const tg_fluid *f, int nsteps, int f
             rc = ns_middle<T>(p);
             if (rc) return rc;
             RkParams rk = rk_params(p, f, s);
+            if (s == 4) rk.nonfinite = p->nonfinite_dev;
             Prof pr(p, TG_KERNEL_X_NLIN_RK);
             HIPCHK(launch_x_nlin<T>(p->g, p->band, rk, p->P, V, p->acc, p->U, p->slots[(s - 1) & 1],
                                     p->slots[s & 1], nullptr, p->twx, p->stream, p->x_tile_columns));
@@ -400,8 +432,9 @@ int run_rk4(tg_plan *p, double *const V[3], const tg_fluid *f, int nsteps, int f
         }
     }
     p->unext_valid = true;
+    p->unext_band = p->band;
     for (int c = 0; c < 3; c++) p->unext_V[c] = V[c];
-    return TG_OK;
+    return nonfinite_end(p, "tg_rk4_step");
 }
 
 int ensure_mhd(tg_plan *p)
@@ -452,6 +485,8 @@ int run_rk4_mhd(tg_plan *p, double *const V[3], double *const W[3], const tg_flu
     double *Zp[3] = {p->mhd[9], p->mhd[10], p->mhd[11]};
     double *Zm[3] = {p->mhd[12], p->mhd[13], p->mhd[14]};
     p->unext_valid = false;
+    rc = nonfinite_begin(p);
+    if (rc) return rc;
     for (int step = 0; step < nsteps; step++) {
         HIPCHK(launch_mhd_elsasser(p->g, V, W, Zp, Zm, p->stream));
         rc = mhd_xinv<T>(p, Zp, Zm);
@@ -461,7 +496,8 @@ int run_rk4_mhd(tg_plan *p, double *const V[3], double *const W[3], const tg_flu
             if (rc) return rc;
             {
                 Prof pr(p, TG_KERNEL_X_NLIN_RK);
-                HIPCHK(launch_mhd_rk(p->g, s, f->dt, f->nu, eta, V, W, p->acc, accW, Np, Nm, Zp, Zm, p->stream));
+                HIPCHK(launch_mhd_rk(p->g, s, f->dt, f->nu, eta, V, W, p->acc, accW, Np, Nm, Zp, Zm, p->stream,
+                                     s == 4 ? p->nonfinite_dev : nullptr));
             }
             if (s < 4) {
                 rc = mhd_xinv<T>(p, Zp, Zm);
@@ -474,7 +510,7 @@ int run_rk4_mhd(tg_plan *p, double *const V[3], double *const W[3], const tg_flu
             HIPCHK(launch_reality(p->g, W, 3, p->stream));
         }
     }
-    return TG_OK;
+    return nonfinite_end(p, "tg_rk4_step_mhd");
 }
 
 template <typename T>
@@ -793,6 +829,8 @@ int run_rk4_rbc(tg_plan *p, double *const V[3], double *T_, const tg_rbc *r, int
     double *acc[4] = {p->acc[0], p->acc[1], p->acc[2], p->rbc[0]};
     const int reality = p->cfg.strong_reality;
     p->unext_valid = false;
+    rc = nonfinite_begin(p);
+    if (rc) return rc;
     {
         Prof pr(p, TG_KERNEL_X_SC);
         HIPCHK(launch_xinv_sc<T>(p->g, S, 4, kRbcStateParity, p->U, 1, p->twx, p->twq, p->stream));
@@ -807,6 +845,7 @@ int run_rk4_rbc(tg_plan *p, double *const V[3], double *T_, const tg_rbc *r, int
             const double *f_in[2] = {s == 1 ? T_ : p->rbc[1 + 2 * rd], s == 1 ? V[0] : p->rbc[2 + 2 * rd]};
             double *f_out[2] = {p->rbc[1 + 2 * wr], p->rbc[2 + 2 * wr]};
             RbcParams rp = rbc_params(p, r, s);
+            if (s == 4) rp.nonfinite = p->nonfinite_dev;
             Prof pr(p, TG_KERNEL_X_RBC);
             HIPCHK(launch_x_rbc<T>(p->g, rp, p->P, S, acc, p->U, f_in, f_out, nullptr, p->twx, p->twq, p->stream));
         }
@@ -817,7 +856,7 @@ int run_rk4_rbc(tg_plan *p, double *const V[3], double *T_, const tg_rbc *r, int
             HIPCHK(launch_xinv_sc<T>(p->g, S, 4, kRbcStateParity, p->U, 1 | 4, p->twx, p->twq, p->stream));
         }
     }
-    return TG_OK;
+    return nonfinite_end(p, "tg_rk4_step_rbc");
 }
 
 int reduce_to_host(tg_plan *p, int mode, const double *const *A, const double *const *B, int ncomp, int n,
@@ -925,14 +964,19 @@ int tg_plan_create(const tg_config *cfg, tg_plan **plan)
         if (hipMemset(p->U[i], 0, p->ncw * p->elem) != hipSuccess) rc = fail(TG_ERR_HIP, "hipMemset failed");
     for (int i = 0; i < 9 && !rc; i++)
         if (hipMemset(p->P[i], 0, p->ncw * p->elem) != hipSuccess) rc = fail(TG_ERR_HIP, "hipMemset failed");
-    // accumulator: FP64-sized, padded work layout (the NS stage keeps it at the
-    // transform precision; also holds giver outputs in the boundary layout)
+    // accumulator: FP64, padded work layout (the RK state accumulation is FP64
+    // in both precisions; also holds giver outputs in the boundary layout)
     for (int i = 0; i < 3 && !rc; i++) rc = dmalloc(p, (void **)&p->acc[i], p->ncw * 16);
     p->red_blocks = (int)std::min<size_t>(2048, (p->nc + 1023) / 1024);
     if (p->red_blocks < 1) p->red_blocks = 1;
     p->partial_bytes = (size_t)p->red_blocks * 2 * 1100 * sizeof(double);
     if (!rc) rc = dmalloc(p, (void **)&p->partial, p->partial_bytes);
     if (!rc) rc = dmalloc(p, (void **)&p->small, 65536);
+    if (!rc) rc = dmalloc(p, (void **)&p->nonfinite_dev, 256);
+    if (!rc && hipHostMalloc((void **)&p->nonfinite_host, sizeof(int), hipHostMallocDefault) != hipSuccess) {
+        p->nonfinite_host = nullptr;
+        rc = fail(TG_ERR_OOM, "hipHostMalloc failed");
+    }
     if (rc) { free_plan(p); return rc; }
     *plan = p;
     return TG_OK;
@@ -1397,7 +1441,7 @@ int tg_adjoint_forward(tg_plan *plan, double *const V_dev[3], const tg_fluid *fl
 int tg_adjoint_backward(tg_plan *plan, double *const lam_dev[3])
 {
     if (!plan) return fail(TG_ERR_ARG, "null plan");
-    if (!plan->adj.recorded) return fail(TG_ERR_ARG, "no tg_adjoint_forward sweep recorded on this plan");
+    if (!plan->adj.recorded) return fail(TG_ERR_STATE, "no tg_adjoint_forward sweep recorded on this plan");
     int rc = check_ptrs((const void *const *)lam_dev, 3, "lam");
     if (!rc) rc = set_device(plan);
     if (rc) return rc;

@@ -1,11 +1,12 @@
 // tg_diag.hip -- per-mode reductions (energy, Sn, shell spectrum, shell
 // products), the seeded initial field, and the energy-transfer giver pass.
 //
-// Reductions are two-stage and deterministic in their block order: each
-// block owns a fixed slice of modes, reduces it into per-wave LDS bins, sums
-// the wave bins in a fixed order and writes one partial per bin; a second
-// kernel sums the partials of each bin in block order with Neumaier
-// compensation.
+// Reductions are two-stage, compensated and deterministic: each block owns a
+// fixed slice of modes and reduces it into per-wavefront (sum, compensation)
+// bins in LDS (no atomics: lanes that share a bin are summed in a fixed-order
+// tree and added with Neumaier compensation), merges the wavefront bins in a
+// fixed order and writes one partial per bin; a second kernel sums the
+// partials of each bin in block order with Neumaier compensation.
 #include "tg_device.h"
 #include "tg_kernels.h"
 #include "tg_diag.h"
@@ -63,43 +64,87 @@ struct RedArgs {
 constexpr int kRedThreads = 256;
 constexpr int kRedMaxBins = 1100;
 
+// Sum over the 64 lanes of a wavefront (xor butterfly: every lane ends with
+// the same, order-fixed result).
+__device__ __forceinline__ double wave_sum(double v)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Adds each active lane's values val[0..nq) into this wavefront's bins
+// (sum, compensation) pairs at bin `bin`.  The lanes that share a bin are
+// summed by a fixed-order tree, then one lane adds that sum to the bin with
+// Neumaier compensation; the loop runs once per distinct bin among the lanes
+// (consecutive modes fall in few shells).  Deterministic, no atomics.
+template <int NQ>
+__device__ __forceinline__ void wave_bin_add(double *bs, double *bc, int nb, int bin, const double (&val)[NQ],
+                                             bool active)
+{
+    const int lane = threadIdx.x & 63;
+    unsigned long long pending = __ballot(active);
+    while (pending) {
+        const int leader = __ffsll((long long)pending) - 1;
+        const int b = __shfl(bin, leader, 64);
+        const bool mine = active && bin == b;
+#pragma unroll
+        for (int q = 0; q < NQ; q++) {
+            const double t = wave_sum(mine ? val[q] : 0.0);
+            if (lane == leader) neumaier(bs[q * nb + b], bc[q * nb + b], t);
+        }
+        pending &= ~__ballot(mine);
+    }
+}
+
 // Modes of block b: a fixed contiguous slice of the flattened mode index.
+// Scalar modes: per-lane Neumaier sums, then a fixed-order block tree.
+// Binned modes: per-wavefront compensated bins (LDS: [NW][nq][nbins] sums,
+// then as many compensations), merged over the wavefronts in order.
 template <int MODE>
 __global__ void __launch_bounds__(kRedThreads) k_reduce(RedArgs a)
 {
-    constexpr int NW = kRedThreads / 64;
-    extern __shared__ double dyn[];  // [NW][nq][nbins]
+    const int NW = blockDim.x / 64;
+    extern __shared__ double dyn[];
     const Grid &g = a.g;
     const int wave = threadIdx.x / 64;
     const int nb = a.nbins, nq = a.nq;
-    for (int i = threadIdx.x; i < NW * nq * nb; i += kRedThreads) dyn[i] = 0.0;
-    __syncthreads();
+    constexpr bool BINNED = MODE == RED_SPECTRUM || MODE == RED_SHELL_ALL || MODE == RED_SHELL_ALL_NOCUT;
+    double *bsum = dyn + (size_t)wave * nq * nb;
+    double *bcmp = dyn + (size_t)(NW + wave) * nq * nb;
+    if (BINNED) {
+        for (int i = threadIdx.x; i < 2 * NW * nq * nb; i += blockDim.x) dyn[i] = 0.0;
+        __syncthreads();
+    }
     const size_t total = (size_t)g.nx * g.ny * g.nzc;
     const size_t chunk = (total + gridDim.x - 1) / gridDim.x;
     const size_t beg = (size_t)blockIdx.x * chunk;
     const size_t end = beg + chunk < total ? beg + chunk : total;
     double s0 = 0.0, c0 = 0.0;  // single-bin compensated accumulators
-    for (size_t i = beg + threadIdx.x; i < end; i += kRedThreads) {
-        const int lz = i % g.nzc;
-        const int ly = (i / g.nzc) % g.ny;
-        const int lx = i / ((size_t)g.nzc * g.ny);
+    // every lane runs the same number of iterations (the binned modes vote
+    // across the wavefront)
+    for (size_t i0 = beg; i0 < end; i0 += blockDim.x) {
+        const size_t i = i0 + threadIdx.x;
+        const bool valid = i < end;
+        const size_t ii = valid ? i : beg;
+        const int lz = ii % g.nzc;
+        const int ly = (ii / g.nzc) % g.ny;
+        const int lx = ii / ((size_t)g.nzc * g.ny);
         // FFF: w = 1/2 on kz = 0 (FFF_pencil_energy.cc:59-68); SFF: 2 sum - kz=0
         // plane - kx=0 plane + half the kx=0,kz=0 line (SFF_pencil_energy.cc:
         // 88-104), i.e. w = (lx ? 2 : 1)(lz ? 2 : 1)/2
         const double w = g.basis ? 0.5 * (lx ? 2.0 : 1.0) * (lz ? 2.0 : 1.0) : (lz == 0 ? 0.5 : 1.0);
         if (MODE == RED_ENERGY) {
             const double *A = a.A[0];
-            const double re = A[2 * i], im = A[2 * i + 1];
-            neumaier(s0, c0, w * (re * re + im * im));
+            const double re = A[2 * ii], im = A[2 * ii + 1];
+            if (valid) neumaier(s0, c0, w * (re * re + im * im));
         } else if (MODE == RED_CROSS) {
             const double *A = a.A[0], *B = a.B[0];
-            neumaier(s0, c0, w * (A[2 * i] * B[2 * i] + A[2 * i + 1] * B[2 * i + 1]));
+            if (valid) neumaier(s0, c0, w * (A[2 * ii] * B[2 * ii] + A[2 * ii + 1] * B[2 * ii + 1]));
         } else if (MODE == RED_SN) {
-            if (i == 0 && a.n == 0) continue;
             const double K = kmag_l(g, lx, ly, lz);
-            if (K <= a.kcut) {
+            if (valid && !(ii == 0 && a.n == 0) && K <= a.kcut) {
                 const double *A = a.A[0];
-                const double re = A[2 * i], im = A[2 * i + 1];
+                const double re = A[2 * ii], im = A[2 * ii + 1];
                 double kn = 1.0;
                 for (int t = 0; t < a.n; t++) kn *= K;
                 neumaier(s0, c0, multiplicity(g, lx, ly, lz) * kn * (re * re + im * im));
@@ -107,39 +152,38 @@ __global__ void __launch_bounds__(kRedThreads) k_reduce(RedArgs a)
         } else if (MODE == RED_SPECTRUM) {
             const double K = kmag_l(g, lx, ly, lz);
             const int idx = (int)ceil(K);
-            if (idx <= a.kcut) {
-                const double *A = a.A[0];
-                const double re = A[2 * i], im = A[2 * i + 1];
-                const double m = multiplicity(g, lx, ly, lz);
-                double kn = 1.0;
-                for (int t = 0; t < a.n; t++) kn *= K;
-                atomicAdd(&dyn[(wave * nq + 0) * nb + idx], m * kn * (re * re + im * im) * (K * K));
-                atomicAdd(&dyn[(wave * nq + 1) * nb + idx], 2 * m);
-            }
+            const bool act = valid && idx <= a.kcut;
+            const double *A = a.A[0];
+            const double re = A[2 * ii], im = A[2 * ii + 1];
+            const double m = multiplicity(g, lx, ly, lz);
+            double kn = 1.0;
+            for (int t = 0; t < a.n; t++) kn *= K;
+            const double val[2] = {m * kn * (re * re + im * im) * (K * K), 2 * m};
+            wave_bin_add<2>(bsum, bcmp, nb, act ? idx : 0, val, act);
         } else if (MODE == RED_SHELL_ALL || MODE == RED_SHELL_ALL_NOCUT) {
             const double K = kmag_l(g, lx, ly, lz);
-            if (MODE == RED_SHELL_ALL_NOCUT || K <= a.kcut) {
-                const int sidx = shell_of(K, a.radii, nb);
-                double d = 0.0;
-                for (int c = 0; c < a.ncomp; c++) {
-                    const double *A = a.A[c], *B = a.B[c];
-                    d += A[2 * i] * B[2 * i] + A[2 * i + 1] * B[2 * i + 1];
-                }
-                atomicAdd(&dyn[wave * nb + sidx], 2 * multiplicity(g, lx, ly, lz) * d);
+            const bool act = valid && (MODE == RED_SHELL_ALL_NOCUT || K <= a.kcut);
+            const int sidx = shell_of(K, a.radii, nb);
+            double d = 0.0;
+            for (int c = 0; c < a.ncomp; c++) {
+                const double *A = a.A[c], *B = a.B[c];
+                d += A[2 * ii] * B[2 * ii] + A[2 * ii + 1] * B[2 * ii + 1];
             }
+            const double val[1] = {2 * multiplicity(g, lx, ly, lz) * d};
+            wave_bin_add<1>(bsum, bcmp, nb, sidx, val, act);
         } else if (MODE == RED_SHELL_SINGLE) {
             const double K = kmag_l(g, lx, ly, lz);
-            if (K > a.r_lo && K <= a.r_hi) {
+            if (valid && K > a.r_lo && K <= a.r_hi) {
                 double d = 0.0;
                 for (int c = 0; c < a.ncomp; c++) {
                     const double *A = a.A[c], *B = a.B[c];
-                    d += A[2 * i] * B[2 * i] + A[2 * i + 1] * B[2 * i + 1];
+                    d += A[2 * ii] * B[2 * ii] + A[2 * ii + 1] * B[2 * ii + 1];
                 }
                 neumaier(s0, c0, 2 * multiplicity(g, lx, ly, lz) * d);
             }
         }
     }
-    if (MODE == RED_ENERGY || MODE == RED_SN || MODE == RED_SHELL_SINGLE || MODE == RED_CROSS) {
+    if (!BINNED) {
         // block tree reduction of (s0 + c0)
         double v = s0 + c0;
         for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
@@ -153,10 +197,13 @@ __global__ void __launch_bounds__(kRedThreads) k_reduce(RedArgs a)
         return;
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < nq * nb; i += kRedThreads) {
-        double t = 0.0;
-        for (int w = 0; w < NW; w++) t += dyn[w * nq * nb + i];
-        a.partial[(size_t)blockIdx.x * nq * nb + i] = t;
+    for (int i = threadIdx.x; i < nq * nb; i += blockDim.x) {
+        double t = dyn[i], c = dyn[(size_t)NW * nq * nb + i];
+        for (int w = 1; w < NW; w++) {
+            neumaier(t, c, dyn[(size_t)w * nq * nb + i]);
+            c += dyn[(size_t)(NW + w) * nq * nb + i];
+        }
+        a.partial[(size_t)blockIdx.x * nq * nb + i] = t + c;
     }
 }
 
@@ -193,19 +240,26 @@ hipError_t launch_reduce(const Grid &g, int mode, const double *const *A, const 
     const bool single = mode == RED_ENERGY || mode == RED_SN || mode == RED_SHELL_SINGLE || mode == RED_CROSS;
     const int width = single ? 1 : nq * nbins;
     if (!single && nbins > kRedMaxBins) return hipErrorInvalidValue;
-    const size_t shm = single ? 64 : (size_t)(kRedThreads / 64) * nq * nbins * sizeof(double);
+    // binned: (sum, compensation) per bin and wavefront, within 64 KB of LDS
+    int threads = kRedThreads;
+    if (!single) {
+        const size_t per_wave = (size_t)2 * nq * nbins * sizeof(double);
+        int nw = (int)std::min<size_t>(kRedThreads / 64, 65536 / per_wave);
+        threads = 64 * (nw < 1 ? 1 : nw);
+    }
+    const size_t shm = single ? 64 : (size_t)(threads / 64) * 2 * nq * nbins * sizeof(double);
     switch (mode) {
-        case RED_ENERGY: hipLaunchKernelGGL(k_reduce<RED_ENERGY>, dim3(nblocks), dim3(kRedThreads), shm, st, a); break;
-        case RED_SN: hipLaunchKernelGGL(k_reduce<RED_SN>, dim3(nblocks), dim3(kRedThreads), shm, st, a); break;
-        case RED_SPECTRUM: hipLaunchKernelGGL(k_reduce<RED_SPECTRUM>, dim3(nblocks), dim3(kRedThreads), shm, st, a); break;
-        case RED_SHELL_ALL: hipLaunchKernelGGL(k_reduce<RED_SHELL_ALL>, dim3(nblocks), dim3(kRedThreads), shm, st, a); break;
+        case RED_ENERGY: hipLaunchKernelGGL(k_reduce<RED_ENERGY>, dim3(nblocks), dim3(threads), shm, st, a); break;
+        case RED_SN: hipLaunchKernelGGL(k_reduce<RED_SN>, dim3(nblocks), dim3(threads), shm, st, a); break;
+        case RED_SPECTRUM: hipLaunchKernelGGL(k_reduce<RED_SPECTRUM>, dim3(nblocks), dim3(threads), shm, st, a); break;
+        case RED_SHELL_ALL: hipLaunchKernelGGL(k_reduce<RED_SHELL_ALL>, dim3(nblocks), dim3(threads), shm, st, a); break;
         case RED_SHELL_ALL_NOCUT:
-            hipLaunchKernelGGL(k_reduce<RED_SHELL_ALL_NOCUT>, dim3(nblocks), dim3(kRedThreads), shm, st, a);
+            hipLaunchKernelGGL(k_reduce<RED_SHELL_ALL_NOCUT>, dim3(nblocks), dim3(threads), shm, st, a);
             break;
         case RED_SHELL_SINGLE:
-            hipLaunchKernelGGL(k_reduce<RED_SHELL_SINGLE>, dim3(nblocks), dim3(kRedThreads), shm, st, a);
+            hipLaunchKernelGGL(k_reduce<RED_SHELL_SINGLE>, dim3(nblocks), dim3(threads), shm, st, a);
             break;
-        case RED_CROSS: hipLaunchKernelGGL(k_reduce<RED_CROSS>, dim3(nblocks), dim3(kRedThreads), shm, st, a); break;
+        case RED_CROSS: hipLaunchKernelGGL(k_reduce<RED_CROSS>, dim3(nblocks), dim3(threads), shm, st, a); break;
         default: return hipErrorInvalidValue;
     }
     hipError_t e = hipGetLastError();

@@ -69,6 +69,8 @@ struct RkParams {
     int write_rhs;        // stage 0: apply force/projection/negation (Compute_rhs)
     double dt, fh, f1;    // fh = -nu*0.5*dt, f1 = -nu*1.0*dt
     double scale;         // 1/(Nx Ny Nz)
+    int *nonfinite;       // stage 4: set to 1 if the new state holds a
+                          // non-finite value (null: not checked)
 };
 
 // Fused x pass: forward x-FFT of the 6 (giver: 9) products, nonlinear term, RHS and the
@@ -81,6 +83,8 @@ hipError_t launch_x_nlin(const Grid &g, const Band &b, const RkParams &rk, void 
                          double *const *out, const void *twx, hipStream_t st, int x_tile_columns = 0);
 
 hipError_t launch_reality(const Grid &g, double *const *V, int ncomp, hipStream_t st);
+// n ints at p set to v by a kernel (no memset node, so it may be captured)
+hipError_t launch_fill_int(int *p, int n, int v, hipStream_t st);
 hipError_t launch_dealias(const Grid &g, double *const *V, int ncomp, hipStream_t st);
 
 }  // namespace tg

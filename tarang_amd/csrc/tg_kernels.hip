@@ -328,7 +328,7 @@ __global__ void __launch_bounds__(256, (sizeof(T) == 4 && !GIVER && M == 256) ? 
 struct XnlArgs {
     void *P[9];
     double *V[3];
-    void *acc[3];  // stage accumulator at the transform precision T
+    double *acc[3];  // stage accumulator, FP64 (padded work layout)
     void *U[3];
     double *out[3];
     const double *slots_in;
@@ -549,10 +549,8 @@ __global__ void __launch_bounds__(CB *(L / 8), 2) k_x_nlin(XnlArgs a)
     }
 
     // --- Lawson RK4 stage algebra (RK.cc:91-170, Tadvance_functions.cc:298-324,566-570) ---
-    // V: FP64 state, boundary layout; acc: padded work layout at the
-    // transform precision (it sums dt-weighted right-hand sides, which carry
-    // that precision; the algebra runs in FP64 and only the state persists
-    // between steps).
+    // V: FP64 state, boundary layout; acc: FP64, padded work layout (the RK
+    // state accumulation stays FP64 whatever the transform precision).
     const double dt = a.rk.dt;
     const double c_half = 0.5 * dt, c_one = 1.0 * dt;
     const double c_sixth = (1.0 / 6) * dt, c_third = (1.0 / 3) * dt;
@@ -568,7 +566,7 @@ __global__ void __launch_bounds__(CB *(L / 8), 2) k_x_nlin(XnlArgs a)
             constexpr int K = decltype(kc)::value;
             cpx<double> *Yd = (cpx<double> *)Y;
             cpx<T> *Xt = (cpx<T> *)X;
-            if (stage >= 2) stage_in<L, CB>(Yd, (const cpx<T> *)a.acc[K], colw, ncw);
+            if (stage >= 2) stage_in<L, CB>(Yd, (const cpx<double> *)a.acc[K], colw, ncw);
             dma_wait();
             lds_barrier();
             cpx<T> v[8];
@@ -601,6 +599,7 @@ __global__ void __launch_bounds__(CB *(L / 8), 2) k_x_nlin(XnlArgs a)
                     vn = E1 * v0 + ac;
                 }
                 Yd[li] = mk<double>(stage == 4 ? vn.x : ac.x, stage == 4 ? vn.y : ac.y);
+                if (stage == 4 && !(isfinite(vn.x) && isfinite(vn.y)) && a.rk.nonfinite) *a.rk.nonfinite = 1;
                 if (a.b.kind == 1 && cw >= 0) {
                     const int kx = kwave(lx, g.nx);
                     const int sl = band_slot(a.b, kx, ky, lz);
@@ -614,7 +613,7 @@ __global__ void __launch_bounds__(CB *(L / 8), 2) k_x_nlin(XnlArgs a)
             }
             lds_barrier();
             if (stage == 4) stage_out<L, CB>(Yd, (cpx<double> *)a.V[K], cols, ncs);
-            else stage_out<L, CB>(Yd, (cpx<T> *)a.acc[K], colw, ncw);
+            else stage_out<L, CB>(Yd, (cpx<double> *)a.acc[K], colw, ncw);
             fft_line<L, +1>(v, Xt + c * LP, tw, j);
 #pragma unroll
             for (int s = 0; s < 8; s++) Xt[c * LP + lpad(j + TPL * s)] = v[s];
@@ -633,7 +632,7 @@ __global__ void __launch_bounds__(CB *(L / 8), 2) k_x_nlin(XnlArgs a)
         cpx<T> nc[8];
         TG_TAKE(comp, nc);
         stage_in<L, CB>(ldd, (const cpx<double> *)a.V[comp], cols, ncs);
-        if (stage >= 2) stage_in<L, CB>(ldb, (const cpx<T> *)a.acc[comp], colw, ncw);
+        if (stage >= 2) stage_in<L, CB>(ldb, (const cpx<double> *)a.acc[comp], colw, ncw);
         lds_barrier();
         cpx<T> v[8];
 #pragma unroll
@@ -667,6 +666,7 @@ __global__ void __launch_bounds__(CB *(L / 8), 2) k_x_nlin(XnlArgs a)
             // accumulator (stages 1-3) or the new state (stage 4) back to B
             // (component-wise select: selecting the objects keeps them in memory)
             ldb[li] = mk<double>(stage == 4 ? vn.x : ac.x, stage == 4 ? vn.y : ac.y);
+            if (stage == 4 && !(isfinite(vn.x) && isfinite(vn.y)) && a.rk.nonfinite) *a.rk.nonfinite = 1;
             // next stage input: band slots, dealias
             if (a.b.kind == 1 && cw >= 0) {
                 const int kx = kwave(lx, g.nx);
@@ -681,7 +681,7 @@ __global__ void __launch_bounds__(CB *(L / 8), 2) k_x_nlin(XnlArgs a)
         }
         lds_barrier();
         if (stage == 4) stage_out<L, CB>(ldb, (cpx<double> *)a.V[comp], cols, ncs);
-        else stage_out<L, CB>(ldb, (cpx<T> *)a.acc[comp], colw, ncw);
+        else stage_out<L, CB>(ldb, (cpx<double> *)a.acc[comp], colw, ncw);
         // inverse x-FFT of the next stage input (exchanges in buffer A)
         fft_line<L, +1>(v, ldt + c * LP, tw, j);
 #pragma unroll
@@ -923,6 +923,18 @@ hipError_t launch_x_nlin(const Grid &g, const Band &b, const RkParams &rk, void 
             else TG_XNL(CB, 0);
         }
     });
+    return hipGetLastError();
+}
+
+__global__ void k_fill_int(int *p, int n, int v)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+hipError_t launch_fill_int(int *p, int n, int v, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_fill_int, dim3((n + 63) / 64), dim3(64), 0, st, p, n, v);
     return hipGetLastError();
 }
 

@@ -11,6 +11,7 @@ hipError_t launch_mhd_elsasser(const Grid &g, double *const *V, double *const *W
 hipError_t launch_mhd_combine(const Grid &g, double *const *NU, double *const *NW, hipStream_t st);
 hipError_t launch_mhd_rk(const Grid &g, int stage, double dt, double nu, double eta, double *const *V,
                          double *const *W, double *const *accV, double *const *accW, const double *const *Np,
-                         const double *const *Nm, double *const *Zp, double *const *Zm, hipStream_t st);
+                         const double *const *Nm, double *const *Zp, double *const *Zm, hipStream_t st,
+                         int *nonfinite = nullptr);
 
 }  // namespace tg

@@ -42,6 +42,7 @@ struct MhdArgs {
     Grid g;
     int stage;
     double dt, fh_u, f1_u, fh_b, f1_b;
+    int *nonfinite;  // stage 4: set to 1 if the new state holds a non-finite value
 };
 
 __global__ void k_mhd_elsasser(MhdArgs a)
@@ -125,6 +126,8 @@ __global__ void k_mhd_rk(MhdArgs a)
             if (a.stage == 4) {
                 ((cpx<double> *)a.V[c])[i] = un;
                 ((cpx<double> *)a.W[c])[i] = bn;
+                if (!(isfinite(un.x) && isfinite(un.y) && isfinite(bn.x) && isfinite(bn.y)) && a.nonfinite)
+                    *a.nonfinite = 1;
             } else {
                 ((cpx<double> *)a.accV[c])[i] = au;
                 ((cpx<double> *)a.accW[c])[i] = ab;
@@ -181,7 +184,8 @@ hipError_t launch_mhd_combine(const Grid &g, double *const *NU, double *const *N
 
 hipError_t launch_mhd_rk(const Grid &g, int stage, double dt, double nu, double eta, double *const *V,
                          double *const *W, double *const *accV, double *const *accW, const double *const *Np,
-                         const double *const *Nm, double *const *Zp, double *const *Zm, hipStream_t st)
+                         const double *const *Nm, double *const *Zp, double *const *Zm, hipStream_t st,
+                         int *nonfinite)
 {
     MhdArgs a = mhd_args(g, V, W);
     for (int c = 0; c < 3; c++) {
@@ -198,6 +202,7 @@ hipError_t launch_mhd_rk(const Grid &g, int stage, double dt, double nu, double 
     a.f1_u = -nu * 1.0 * dt;
     a.fh_b = -eta * 0.5 * dt;
     a.f1_b = -eta * 1.0 * dt;
+    a.nonfinite = nonfinite;
     hipLaunchKernelGGL(k_mhd_rk, dim3(8192), dim3(256), 0, st, a);
     return hipGetLastError();
 }

@@ -33,6 +33,8 @@ struct RbcParams {
     double fh_u, f1_u;    // -nu dt/2, -nu dt
     double fh_t, f1_t;    // -kappa dt/2, -kappa dt
     double scale;         // 1/(2 Nx Ny Nz)
+    int *nonfinite;       // stage 4: set to 1 if the new state holds a
+                          // non-finite value (null: not checked)
 };
 
 // Fused x pass of an RBC stage: sin/cos x-forward of the 9 products, N and

@@ -374,6 +374,7 @@ __device__ __forceinline__ void rbc_lawson(const Grid &g, const RbcParams &rp, i
         }
         // component-wise select (selecting the objects keeps them in memory)
         ldb[li] = mk<double>(stage == 4 ? vn.x : ac.x, stage == 4 ? vn.y : ac.y);
+        if (stage == 4 && !(isfinite(vn.x) && isfinite(vn.y)) && rp.nonfinite) *rp.nonfinite = 1;
         // the undealiased next-stage value (this field's share of the next
         // force) replaces v0 in the thread's own slot of the state tile
         if (fo) ldd[li] = vn;
