@@ -63,6 +63,25 @@ def parse():
     return ap.parse_args()
 
 
+def copy_bandwidth(torch, gib=4, reps=5):
+    """Attainable HBM bandwidth: a device-to-device copy of `gib` GiB (read +
+    write bytes over the event-timed average)."""
+    a = torch.empty(gib << 27, dtype=torch.float64, device="cuda").fill_(1.0)
+    b = torch.empty_like(a)
+    b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    t = e0.elapsed_time(e1) * 1e-3 / reps
+    gbs = 2 * a.numel() * 8 / t / 1e9
+    del a, b
+    torch.cuda.empty_cache()
+    return gbs
+
+
 def algorithmic_bytes(kernel, N, prec_bytes, nstate=3):
     """Algorithmic HBM bytes of one launch class per RK4 step (all launches)."""
     nc = N[0] * N[1] * (N[2] // 2 + 1)
@@ -124,6 +143,8 @@ def main():
         print(json.dumps({"mhd": mhd_precision(api, torch, np)}))
         return
 
+    hbm_copy = copy_bandwidth(torch)
+    print(f"[bench] device copy: {hbm_copy:.0f} GB/s", file=sys.stderr, flush=True)
     N = (args.n, args.n, args.n)
     nu, dt = 1e-3, 1e-3
     force = dict(force_kind=1, force_eps=0.1, force_r_in=2.0, force_r_out=3.0)
@@ -176,6 +197,8 @@ def main():
         "frac": round(dom_achieved / HBM_PEAK_GBS, 4),
         "bytes_per_launch": int(dom_bytes_step / dom_launch), "avg_launch_ms": round(dom_ms_step / dom_launch, 4),
         "traffic": measured_traffic(dom, N, "fp64"),
+        "attainable_copy_GBps": round(hbm_copy, 1),
+        "frac_of_copy": round(dom_achieved / hbm_copy, 4),
     }
     per_kernel = {k: {"ms_per_step": round(v[0], 3), "launches_per_step": v[1],
                       "GBps": round(algorithmic_bytes(k, N, 8) / (v[0] * 1e-3) / 1e9, 1)
@@ -258,10 +281,28 @@ def main():
                          "steps_per_s_with_diagnostics": round(1e3 / (ms_step + tdiag * 1e3), 3)}
         print(f"[bench] diagnostics {tdiag * 1e3:.1f} ms (separate calls {tsep * 1e3:.1f} ms)", file=sys.stderr,
               flush=True)
+        # the transfer identity at full size: 2/3 rule, the initial field (excited for K <= 170, inside the
+        # largest inscribed sphere, so no receiver is cut): sum S = 0, S antisymmetric, to round-off
+        del et
+        torch.cuda.empty_cache()
+        b23 = api.FFF_Basis(N, precision=api.PREC_FP64, dealias=api.DEALIAS_TWOTHIRDS, strong_reality=True)
+        U23 = api.FluidVF(b23, nu)
+        U23.set_field(V_init)
+        f23, S23 = api.EnergyTr(b23, radii, radii).Compute_flux_and_shell_tr(U23)
+        ns = len(radii) - 1
+        Sm = S23[1:ns, 1:ns]
+        extra["diag_twothirds"] = {
+            "field": "initial k^-5/3 field, K <= 170", "dealias": "2/3 rule", "spheres": ns,
+            "shell_to_shell_sum": float(S23.sum()), "shell_to_shell_abs_sum": float(np.abs(S23).sum()),
+            "sum_rel": float(abs(S23.sum()) / np.abs(S23).sum()),
+            "antisymmetry_rel": float(np.abs(Sm + Sm.T).max() / np.abs(S23).sum()),
+            "flux_last_sphere": float(f23[-1]), "flux_max": float(np.abs(f23).max())}
+        print(f"[bench] 2/3-rule transfer at {N}: |sum S|/sum|S| = {extra['diag_twothirds']['sum_rel']:.2e}",
+              file=sys.stderr, flush=True)
+        del U23, b23
+        torch.cuda.empty_cache()
         extra["energy"] = E64
         if not args.no_adjoint:
-            del et
-            torch.cuda.empty_cache()
             extra["adjoint"] = adjoint_cost(api, torch, np)
             torch.cuda.empty_cache()
         if not args.no_rbc:
@@ -273,7 +314,18 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(N, nu, dt, force, V_init if V_init is not None else U64.cvf)
+        src = V_init if V_init is not None else U64.cvf
+        cpu, V_cpu = cpu_baseline(N, nu, dt, force, src)
+        # the same step on the GPU from the same field, against the oracle's step
+        b1, U1 = make(api.PREC_FP64)
+        U1.set_field(src)
+        api.Time_advance_incompress(U1, dt, **force).Time_advance_step(1)
+        num = sum(float(np.linalg.norm((U1.cvf[c].cpu().numpy() - V_cpu[c]).ravel()) ** 2) for c in range(3))
+        den = sum(float(np.linalg.norm(V_cpu[c].ravel()) ** 2) for c in range(3))
+        cpu["gpu_step_rel_l2_vs_oracle"] = (num / den) ** 0.5
+        print(f"[bench] GPU step vs oracle step at {N}: rel L2 {cpu['gpu_step_rel_l2_vs_oracle']:.3e}",
+              file=sys.stderr, flush=True)
+        del U1, b1
 
     out = {
         "metric": "RK4 steps/s at 512^3 (FP64 vs FP32, % HBM roofline); adjoint/forward cost",
@@ -314,8 +366,11 @@ def rbc_run(api, torch, np, prec, N, steps, warmup, Ra, Pr, dt, seed, profile):
     shape = (m - 1, rows.numel(), m)
     noise = torch.complex(torch.randn(shape, generator=gen, device="cuda", dtype=torch.float64),
                           torch.randn(shape, generator=gen, device="cuda", dtype=torch.float64))
-    T.F[1:m, rows, :m] = 1e-3 * noise
+    T.F[1:m, rows, :m] = noise
     b.Satisfy_strong_reality_condition_in_Array(T.F)
+    # real-space RMS 1e-3: energy (1/2) <theta^2> = 5e-7 with the SFF weights
+    T.F.mul_((5e-7 / T.Compute_total_energy()) ** 0.5)
+    e0 = T.Compute_total_energy()
     ta = api.Time_advance_incompress_RBC(U, T, Ra, Pr, dt)
     ta.Time_advance_step(warmup)
     torch.cuda.synchronize()
@@ -336,7 +391,7 @@ def rbc_run(api, torch, np, prec, N, steps, warmup, Ra, Pr, dt, seed, profile):
             check("tg_profile_read", L.tg_profile_read(b.plan, k, ctypes.byref(kms), ctypes.byref(cnt)))
             if cnt.value:
                 kern[name] = {"ms_per_step": round(kms.value, 3), "launches_per_step": cnt.value}
-    res = {"ms": ms, "kernels": kern, "nusselt": api.Correlation.Get_Nusselt_no(U, T, Ra, Pr),
+    res = {"ms": ms, "kernels": kern, "energy_theta_initial": e0, "nusselt": api.Correlation.Get_Nusselt_no(U, T, Ra, Pr),
            "energy_u": U.Compute_total_energy(), "energy_theta": T.Compute_total_energy(),
            "plan_GB": round(b.device_bytes() / 1e9, 1), "S": nx * ny * nzc * 16}
     del ta, U, T, b
@@ -354,7 +409,11 @@ def rbc_rate(api, torch, np, N=(512, 1024, 1024), steps=3, warmup=1, Ra=1e8, Pr=
     rel = lambda a, b: abs(a - b) / abs(b)
     print(f"[bench] rbc FP32: {r32['ms']:.1f} ms/step, Nu {r32['nusselt']:.12g}", file=sys.stderr, flush=True)
     return {"workload": f"SFF Rayleigh-Benard {N[0]}x{N[1]}x{N[2]} FP64, Ra={Ra:g}, Pr={Pr:g}, corner dealias, "
-                        "weak reality, theta noise 1e-3 in the first 32 modes",
+                        "weak reality, u = 0, theta = white noise in the first 32 modes per axis scaled to a "
+                        "real-space RMS of 1e-3",
+            "grid_note": f"{N[0]} sin/cos points along x, not 513: the transforms take powers of two (the "
+                         "reference's own SFF examples use even sizes, info/data/RBC_scaling/3d/in/para.yaml)",
+            "energy_theta_initial": r["energy_theta_initial"],
             "steps_per_s": round(1e3 / ms, 4), "ms_per_step": round(ms, 2), "steps": steps,
             "step_roofline": {"bytes_model": model, "achieved": round(model / (ms * 1e-3) / 1e9, 1),
                               "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -440,7 +499,7 @@ def measured_traffic(kernel_class, N, precision):
 
 
 def adjoint_cost(api, torch, np, n=256, nsteps=200, every=20, nu=1e-3, dt=1e-3, r_lo=8.0, r_hi=16.0,
-                 hbm_sized=True):
+                 hbm_sized=True, fd_dirs=16):
     N = (n, n, n)
     b = api.FFF_Basis(N, precision=api.PREC_FP64, dealias=api.DEALIAS_REF_CORNER, strong_reality=False)
     U = api.FluidVF(b, nu)
@@ -500,11 +559,41 @@ def adjoint_cost(api, torch, np, n=256, nsteps=200, every=20, nu=1e-3, dt=1e-3, 
     adj.Free()
     print(f"[bench] adjoint {n}^3 x {nsteps}: forward {t_fwd:.3f} s, forward+backward {t_adj:.3f} s",
           file=sys.stderr, flush=True)
+    fd = None
+    if fd_dirs:
+        # the gradient of the timed sweep against central differences of the forward map along seeded
+        # solenoidal directions (eps = 1e-6), at the full size and length
+        U.set_field(V0)
+        adj.Forward(nsteps)
+        grad = [2.0 * m * v for v in U.cvf]
+        adj.Backward(grad)
+        adj.Free()
+        Ud = api.FluidVF(b, nu)
+
+        def loss_of(Vs):
+            U.set_field(Vs)
+            ta.Time_advance_step(nsteps, chain=False)
+            return float(sum((m * (v.real ** 2 + v.imag ** 2)).sum().item() for v in U.cvf))
+
+        eps, worst, t0 = 1e-6, 0.0, time.perf_counter()
+        for seed in range(8, 8 + fd_dirs):
+            Ud.Init_cond_spectrum(kind=1, kmin=1.0, kmax=n / 3.0, energy=0.5, seed=seed)
+            d = Ud.cvf
+            lp = loss_of([v + eps * x for v, x in zip(V0, d)])
+            lm = loss_of([v - eps * x for v, x in zip(V0, d)])
+            fdv = (lp - lm) / (2 * eps)
+            adv = float(sum(torch.sum(torch.conj(g) * x).real.item() for g, x in zip(grad, d)))
+            worst = max(worst, abs(fdv - adv) / max(abs(fdv), 1e-3 * loss))
+        print(f"[bench] adjoint FD check: {fd_dirs} directions, worst rel mismatch {worst:.3e} "
+              f"({time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
+        fd = {"directions": fd_dirs, "eps": eps, "worst_rel_mismatch": worst, "target": 1e-6}
+        del Ud
     out = {"workload": f"{n}^3 FP64, {nsteps} unforced RK4 steps, checkpoints every {every}, "
                        f"loss = E in {r_lo:g} <= |k| < {r_hi:g}",
            "forward_s": round(t_fwd, 4), "forward_and_backward_s": round(t_adj, 4),
            "cost_ratio": round(t_adj / t_fwd, 3), "target_ratio": 3.0, "loss": loss, "grad_norm": gnorm,
-           "checkpoint_and_tape_GB": round(tape_bytes(every) / 1e9, 2), "segment_kernels": kern}
+           "checkpoint_and_tape_GB": round(tape_bytes(every) / 1e9, 2), "segment_kernels": kern,
+           "fd_check": fd}
     if hbm_sized:
         # the same sweep with the checkpoint interval sized to the device memory: the largest divisor of
         # nsteps whose checkpoints and tape fit in 60% of it (fewer recomputed steps, the same gradient)
@@ -534,39 +623,49 @@ def adjoint_cost(api, torch, np, n=256, nsteps=200, every=20, nu=1e-3, dt=1e-3, 
     return out
 
 
-def cpu_baseline(N, nu, dt, force, V_dev):
-    """One RK4 stage (Compute_force_TO_rhs: force, nonlinear term, pressure,
-    RHS) of the same workload on the FP64 CPU oracle (own FFT, OpenMP); a
-    step is four such stages plus O(N^3) pointwise updates, so steps/s is
-    reported as 1 / (4 x stage time)."""
-    import numpy as np
-    sys.path.insert(0, os.path.join(HERE, "oracle"))
-    import tg_oracle as to
-
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    o = to.Oracle(N, dealias=to.DEALIAS_REF_CORNER, strong_reality=True)
-    V = [np.ascontiguousarray(v.cpu().numpy()) for v in V_dev]
-    fl = to.Fluid(nu, dt, force["force_kind"], force["force_eps"], force["force_r_in"], force["force_r_out"])
-    print(f"[bench] cpu baseline: one RK4 stage of {N} on {threads} threads", file=sys.stderr, flush=True)
-    t0 = time.perf_counter()
-    F = o.compute_force(fl, V)
-    Nl = o.compute_nlin(V)
-    o.compute_rhs(Nl, F)
-    t = time.perf_counter() - t0
-    print(f"[bench] cpu baseline: {t:.2f} s/stage", file=sys.stderr, flush=True)
-    model = ""
+def _cpu_model():
+    """lscpu's model name (falls back to /proc/cpuinfo)."""
+    import subprocess
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                return line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
                 if line.startswith("model name"):
-                    model = line.split(":", 1)[1].strip()
-                    break
+                    return line.split(":", 1)[1].strip()
     except OSError:
         pass
-    return {"value": round(1.0 / (4 * t), 5), "unit": "RK4 steps/s", "cores": threads, "kind": "port",
-            "sample": f"one RK4 stage (force + nonlinear term + pressure/RHS, 9 FP64 3D transforms) of the "
-                      f"same {N[0]}^3 workload on the FP64 oracle (own FFT, OpenMP): {t:.1f} s; "
-                      f"steps/s = 1/(4 x stage)", "cpu_model": model}
+    return ""
+
+
+def cpu_baseline(N, nu, dt, force, V_dev):
+    """One full Lawson RK4 step (four stages: force, nonlinear term, pressure,
+    RHS, the stage algebra and the strong reality condition) of the same
+    workload from the same field on the FP64 CPU oracle (own FFT, OpenMP over
+    the host threads), timed."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(HERE, "oracle"))
+    import tg_oracle as to
+
+    nproc = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    threads = int(os.environ.get("OMP_NUM_THREADS", nproc))
+    o = to.Oracle(N, dealias=to.DEALIAS_REF_CORNER, strong_reality=True)
+    V = [np.ascontiguousarray(v.cpu().numpy()) for v in V_dev]
+    fl = to.Fluid(nu, dt, force["force_kind"], force["force_eps"], force["force_r_in"], force["force_r_out"])
+    print(f"[bench] cpu baseline: one RK4 step of {N} on {threads} threads", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    o.rk4_step(fl, V)
+    t = time.perf_counter() - t0
+    print(f"[bench] cpu baseline: {t:.2f} s/step", file=sys.stderr, flush=True)
+    return {"value": round(1.0 / t, 5), "unit": "RK4 steps/s", "cores": threads, "nproc": nproc, "kind": "port",
+            "sample": f"one full RK4 step (4 stages, 36 FP64 3D transforms) of the same {N[0]}^3 workload from "
+                      f"the same field on the FP64 oracle (own FFT, OpenMP, {threads} threads): {t:.1f} s",
+            "cpu_model": _cpu_model()}, V
 
 
 if __name__ == "__main__":

@@ -199,7 +199,10 @@ int tg_shell_to_shell(tg_plan *plan, double *const V_dev[3], const double *radii
  * evaluation per shell gives the shell-to-shell row S (as tg_shell_to_shell)
  * and, summed over the givers inside each sphere and the receivers outside
  * it, the flux (as tg_flux, to round-off; the nonlinear term is linear in the
- * giver).  radii[0..n], flux[0..n], S[(n+1)*(n+1)].  Dealiases V in place. */
+ * giver).  radii[0..n], flux[0..n], S[(n+1)*(n+1)].  Needs radii[0] = 0,
+ * strictly increasing radii and radii[n] >= Min_radius_outside (a last
+ * sphere enclosing every mode, e.g. inf), TG_ERR_ARG otherwise.  Dealiases V
+ * in place. */
 int tg_energy_transfer(tg_plan *plan, double *const V_dev[3], const double *radii, int n, double *flux, double *S);
 
 /* Per-kernel timing with HIP events on the plan's stream (no counterpart

@@ -1296,6 +1296,16 @@ int tg_energy_transfer(tg_plan *plan, double *const V_dev[3], const double *radi
     if (rc) return rc;
     const int nr = n + 1;
     if (nr > 1000) return fail(TG_ERR_ARG, "too many shells");
+    // the flux is assembled from the shell-to-shell givers 1..n-1: every mode
+    // must lie in one of them or in the receiver-only last shell, i.e. the
+    // first sphere is the origin and the last one encloses every mode
+    // (Min_radius_outside); radii strictly increasing
+    if (!(radii[0] == 0.0)) return fail(TG_ERR_ARG, "tg_energy_transfer needs radii[0] == 0");
+    for (int i = 1; i <= n; i++)
+        if (!(radii[i] > radii[i - 1])) return fail(TG_ERR_ARG, "radii must increase strictly");
+    if (!(radii[n] >= (double)min_radius_outside(plan->g)))
+        return fail(TG_ERR_ARG, "tg_energy_transfer needs radii[n] >= %d (a last sphere enclosing every mode)",
+                    min_radius_outside(plan->g));
     if (plan->g.dealias) HIPCHK(launch_dealias(plan->g, V_dev, 3, plan->stream));
     for (int i = 0; i < nr * nr; i++) S[i] = 0.0;
     // T[m][k]: transfer from giver shell m into receiver shell k, all modes

@@ -168,3 +168,30 @@ def test_transfer_zero_sum_twothirds(N):
         assert abs(flux[s] - pi) <= 1e-13 * scale, (s, flux[s], pi)
     assert np.abs(flux - flux_d).max() <= 1e-13 * scale
     assert rel_l2(S, S_d) < 1e-12 and rel_l2(S, S_ref) < 1e-11
+
+
+def test_energy_transfer_radii_contract():
+    """tg_energy_transfer with a finite last radius that encloses every mode
+    equals tg_flux; radii that leave modes outside every giver shell (first
+    radius above 0, last radius inside the grid) are rejected."""
+    from tarang_amd._lib import ERR_ARG, TarangGpuError
+    N = (32, 32, 32)
+    o = to.Oracle(N)
+    V = o.init_field(kind=1, k0=4.0, kmin=1.0, kmax=14.0, energy=0.5, seed=43)
+    b = api.FFF_Basis(N)
+    U = api.FluidVF(b)
+    U.set_field([dev(v) for v in V])
+    rmax = float(b.Min_radius_outside())
+    radii = np.array([0.0, 2.0, 4.0, 8.0, 12.0, 16.0, rmax])
+    flux, S = api.EnergyTr(b, radii, radii).Compute_flux_and_shell_tr(U)
+    flux_d = api.EnergyTr(b, radii, radii).Compute_flux(U)
+    flux_ref = o.flux([v.copy() for v in V], radii)
+    log(f"finite last radius {rmax}: flux vs tg_flux {np.abs(flux - flux_d).max():.2e}, vs oracle "
+        f"{rel_l2(flux, flux_ref):.2e}")
+    assert np.abs(flux - flux_d).max() <= 1e-12 * np.abs(flux_d).max()
+    assert rel_l2(flux, flux_ref) < 1e-11
+    for bad in (np.array([0.5, 2.0, 4.0, np.inf]), np.array([0.0, 2.0, 4.0, 10.0]),
+                np.array([0.0, 4.0, 2.0, np.inf])):
+        with pytest.raises(TarangGpuError) as ei:
+            api.EnergyTr(b, bad, bad).Compute_flux_and_shell_tr(U)
+        assert ei.value.code == ERR_ARG
