@@ -200,10 +200,48 @@ int tg_shell_to_shell(tg_plan *plan, double *const V_dev[3], const double *radii
  * and, summed over the givers inside each sphere and the receivers outside
  * it, the flux (as tg_flux, to round-off; the nonlinear term is linear in the
  * giver).  radii[0..n], flux[0..n], S[(n+1)*(n+1)].  Needs radii[0] = 0,
- * strictly increasing radii and radii[n] >= Min_radius_outside (a last
+ * non-decreasing radii and radii[n] >= Min_radius_outside (a last
  * sphere enclosing every mode, e.g. inf), TG_ERR_ARG otherwise.  Dealiases V
  * in place. */
 int tg_energy_transfer(tg_plan *plan, double *const V_dev[3], const double *radii, int n, double *flux, double *S);
+
+/* Anisotropic diagnostics (FFF plans).  dirn is the anisotropy axis
+ * (global.field.anisotropy_dirn, 1..3); theta is the angle of K to it, folded
+ * into [0, pi/2] for binning; edge arrays hold nsub+1 non-decreasing values
+ * (sector angles up to pi/2, or |Kpll| edges); bins are shell-major with
+ * nsub+1 sub-bins per shell (sub-bin 0 stays empty, as in the reference).
+ *
+ * Universal::Compute_ring_spectrum (universal_energy.cc:552-628, scalar form
+ * :753-786; Correlation::Compute_ring_spectrum, Correlation.cc:301-311):
+ * out[c][shell][sector] = sum m K^n |A_c|^2 over 0 < K, shell = ceil(K) <=
+ * Max_radius_inside; ncomp 1 or 3; out has ncomp (Kmax+1)(nsectors+1). */
+int tg_ring_spectrum(tg_plan *plan, const double *const A_dev[], int ncomp, int n, int dirn,
+                     const double *sector_angles, int nsectors, double *out);
+/* Universal::Compute_cylindrical_ring_spectrum (universal_energy.cc:868-
+ * 925): S1 = sum m K^n |V_pll|^2, S2 = sum m K^n (|V|^2 - |V_pll|^2) in bins
+ * shell = ceil(Kperp) <= Anis_max_Krho_radius_inside, slab of |Kpll|
+ * (Get_slab_index); S1, S2 each (Kperp_max+1)(nslabs+1). */
+int tg_cylindrical_ring_spectrum(tg_plan *plan, const double *const V_dev[3], int n, int dirn, const double *kpll,
+                                 int nslabs, double *S1, double *S2);
+/* EnergyTr::Compute_ring_tr (compute_ring_tr.cc:62-77): for the giver rings
+ * (shell 1..nshells-1 of radii[0..nshells], sector 1..nsectors;
+ * Fill_array_ring, universal_ET.cc:897-913), the transfer into every
+ * receiver ring, -sum 2m Re(N(u|u_giver) . u*) over 0 < K <=
+ * Max_radius_inside (Ring_mult_all, universal_ET.cc:205-272).
+ * out[((i (ns+1) + j) (nr+1) + k) (ns+1) + l], nr = nshells, ns = nsectors.
+ * The reference's three-component Local_ring_mult_all adds 0 (its product
+ * is commented out, universal_ET.cc:267-268); this adds the product its
+ * one-component and cylindrical forms add.  Dealiases V in place. */
+int tg_ring_to_ring(tg_plan *plan, double *const V_dev[3], int dirn, const double *radii, int nshells,
+                    const double *sector_angles, int nsectors, double *out);
+/* EnergyTr::Compute_cylindrical_ring_tr (compute_cylin_ring_tr.cc:51-64):
+ * giver rings radii[i-1] < |K| <= radii[i], kpll[j-1] < Kpll <= kpll[j]
+ * (Fill_array_cylindrical_ring, universal_ET.cc:917-933), receivers binned
+ * by the Kperp shell and the slab of |Kpll| over Kperp <=
+ * Anis_max_Krho_radius_inside (Cyl_ring_mult_all, universal_ET.cc:
+ * 335-375).  Layout as tg_ring_to_ring.  Dealiases V in place. */
+int tg_cylindrical_ring_to_ring(tg_plan *plan, double *const V_dev[3], int dirn, const double *radii, int nshells,
+                                const double *kpll, int nslabs, double *out);
 
 /* Per-kernel timing with HIP events on the plan's stream (no counterpart
  * in the reference, whose only timing is the total wall time printed by

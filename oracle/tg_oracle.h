@@ -118,6 +118,28 @@ void or_flux(const or_grid *g, double *V[3], const double *radii, int nspheres, 
 void or_shell_to_shell(const or_grid *g, double *V[3], const double *radii, int nshells, double *S);
 void or_default_radii(const or_grid *g, int n, double *radii);
 
+/* ---- anisotropic diagnostics (tg_oracle_aniso.c); dirn = anisotropy_dirn 1..3 ---- */
+double or_aniso_kpll(const or_grid *g, int lx, int ly, int lz, int dirn);
+double or_aniso_kperp(const or_grid *g, int lx, int ly, int lz, int dirn);
+double or_polar_angle(const or_grid *g, int lx, int ly, int lz, int dirn);
+int  or_aniso_max_krho_inside(const or_grid *g, int dirn);
+void or_ring_spectrum(const or_grid *g, double *const A[], int ncomp, int n, int dirn, const double *angles,
+                      int nsec, double *out);
+void or_cyl_ring_spectrum(const or_grid *g, double *const V[3], int n, int dirn, const double *kpll_array, int nslabs,
+                          double *S1, double *S2);
+void or_fill_array_ring(const or_grid *g, const double *A, double *B, double r_in, double r_out, double a_lo,
+                        double a_hi, int dirn);
+void or_fill_array_cyl_ring(const or_grid *g, const double *A, double *B, double r_in, double r_out, double h_lo,
+                            double h_hi, int dirn);
+void or_ring_mult_all(const or_grid *g, double *const A[3], double *const B[3], int dirn, const double *radii,
+                      int nshells, const double *angles, int nsec, double *out);
+void or_cyl_ring_mult_all(const or_grid *g, double *const A[3], double *const B[3], int dirn, const double *radii,
+                          int nshells, const double *kpll_array, int nslabs, double *out);
+void or_ring_to_ring(const or_grid *g, double *V[3], int dirn, const double *radii, int nshells,
+                     const double *angles, int nsec, double *out);
+void or_cyl_ring_to_ring(const or_grid *g, double *V[3], int dirn, const double *radii, int nshells,
+                         const double *kpll_array, int nslabs, double *out);
+
 /* ---- seeded initial condition (this build's own definition) ---- */
 void or_init_field(const or_grid *g, int spectrum_kind, double k0, double kmin, double kmax,
                    double energy, uint64_t seed, double *V[3]);

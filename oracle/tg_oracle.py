@@ -105,6 +105,18 @@ def lib():
             "or_compute_rhs_rbc": (None, [gp, p, p, pp, p]),
             "or_rk4_step_rbc": (None, [gp, ctypes.POINTER(Rbc), pp, p]),
             "or_nusselt": (d, [gp, ctypes.POINTER(Rbc), p, p]),
+            "or_aniso_kpll": (d, [gp, i, i, i, i]),
+            "or_aniso_kperp": (d, [gp, i, i, i, i]),
+            "or_polar_angle": (d, [gp, i, i, i, i]),
+            "or_aniso_max_krho_inside": (i, [gp, i]),
+            "or_ring_spectrum": (None, [gp, pp, i, i, i, p, i, p]),
+            "or_cyl_ring_spectrum": (None, [gp, pp, i, i, p, i, p, p]),
+            "or_fill_array_ring": (None, [gp, p, p, d, d, d, d, i]),
+            "or_fill_array_cyl_ring": (None, [gp, p, p, d, d, d, d, i]),
+            "or_ring_mult_all": (None, [gp, pp, pp, i, p, i, p, i, p]),
+            "or_cyl_ring_mult_all": (None, [gp, pp, pp, i, p, i, p, i, p]),
+            "or_ring_to_ring": (None, [gp, pp, i, p, i, p, i, p]),
+            "or_cyl_ring_to_ring": (None, [gp, pp, i, p, i, p, i, p]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(_lib, name)
@@ -137,6 +149,7 @@ class Oracle:
         if isinstance(n, int):
             n = (n, n, n)
         self.nx, self.ny, self.nz = n
+        self.kfactor = tuple(float(k) for k in kfactor)
         self.g = Grid(self.nx, self.ny, self.nz, (ctypes.c_double * 3)(*kfactor), dealias,
                       1 if strong_reality else 0)
         self.L = lib()
@@ -289,6 +302,75 @@ class Oracle:
         loss = self.L.or_adjoint_shell_energy(ctypes.byref(self.g), ctypes.byref(fluid), _pp(V0), nsteps, r_lo, r_hi,
                                               _pp(grad))
         return loss, grad
+
+    # ---- anisotropic diagnostics (dirn = anisotropy_dirn 1..3) ----
+    def polar_angle(self, lx, ly, lz, dirn=3):
+        return self.L.or_polar_angle(ctypes.byref(self.g), lx, ly, lz, dirn)
+
+    def aniso_max_krho_inside(self, dirn=3):
+        return self.L.or_aniso_max_krho_inside(ctypes.byref(self.g), dirn)
+
+    def ring_spectrum(self, A, n, angles, dirn=3):
+        """[ncomp, Max_radius_inside+1, nsectors+1]"""
+        angles = np.ascontiguousarray(angles, np.float64)
+        nsec = len(angles) - 1
+        kmax = self.max_radius_inside()
+        out = np.zeros((len(A), kmax + 1, nsec + 1))
+        self.L.or_ring_spectrum(ctypes.byref(self.g), _pp(A), len(A), n, dirn, _ptr(angles), nsec, _ptr(out))
+        return out
+
+    def cyl_ring_spectrum(self, V, n, kpll, dirn=3):
+        kpll = np.ascontiguousarray(kpll, np.float64)
+        ns = len(kpll) - 1
+        km = self.aniso_max_krho_inside(dirn)
+        S1, S2 = np.zeros((km + 1, ns + 1)), np.zeros((km + 1, ns + 1))
+        self.L.or_cyl_ring_spectrum(ctypes.byref(self.g), _pp(V), n, dirn, _ptr(kpll), ns, _ptr(S1), _ptr(S2))
+        return S1, S2
+
+    def fill_array_ring(self, A, r_in, r_out, a_lo, a_hi, dirn=3):
+        B = self.czeros()
+        self.L.or_fill_array_ring(ctypes.byref(self.g), _cptr(A), _cptr(B), r_in, r_out, a_lo, a_hi, dirn)
+        return B
+
+    def fill_array_cyl_ring(self, A, r_in, r_out, h_lo, h_hi, dirn=3):
+        B = self.czeros()
+        self.L.or_fill_array_cyl_ring(ctypes.byref(self.g), _cptr(A), _cptr(B), r_in, r_out, h_lo, h_hi, dirn)
+        return B
+
+    def ring_mult_all(self, A, B, radii, angles, dirn=3):
+        radii = np.ascontiguousarray(radii, np.float64)
+        angles = np.ascontiguousarray(angles, np.float64)
+        out = np.zeros((len(radii), len(angles)))
+        self.L.or_ring_mult_all(ctypes.byref(self.g), _pp(A), _pp(B), dirn, _ptr(radii), len(radii) - 1,
+                                _ptr(angles), len(angles) - 1, _ptr(out))
+        return out
+
+    def cyl_ring_mult_all(self, A, B, radii, kpll, dirn=3):
+        radii = np.ascontiguousarray(radii, np.float64)
+        kpll = np.ascontiguousarray(kpll, np.float64)
+        out = np.zeros((len(radii), len(kpll)))
+        self.L.or_cyl_ring_mult_all(ctypes.byref(self.g), _pp(A), _pp(B), dirn, _ptr(radii), len(radii) - 1,
+                                    _ptr(kpll), len(kpll) - 1, _ptr(out))
+        return out
+
+    def ring_to_ring(self, V, radii, angles, dirn=3):
+        """[giver shell, giver sector, receiver shell, receiver sector]"""
+        radii = np.ascontiguousarray(radii, np.float64)
+        angles = np.ascontiguousarray(angles, np.float64)
+        ns, na = len(radii), len(angles)
+        out = np.zeros((ns, na, ns, na))
+        self.L.or_ring_to_ring(ctypes.byref(self.g), _pp(V), dirn, _ptr(radii), ns - 1, _ptr(angles), na - 1,
+                               _ptr(out))
+        return out
+
+    def cyl_ring_to_ring(self, V, radii, kpll, dirn=3):
+        radii = np.ascontiguousarray(radii, np.float64)
+        kpll = np.ascontiguousarray(kpll, np.float64)
+        ns, nb = len(radii), len(kpll)
+        out = np.zeros((ns, nb, ns, nb))
+        self.L.or_cyl_ring_to_ring(ctypes.byref(self.g), _pp(V), dirn, _ptr(radii), ns - 1, _ptr(kpll), nb - 1,
+                                   _ptr(out))
+        return out
 
     # ---- SFF basis / Rayleigh-Benard ----
     def sff_forward(self, parity, real):

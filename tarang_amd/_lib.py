@@ -75,6 +75,10 @@ SIGNATURES = {
     "tg_flux": (_I, [_P, _PP, _DP, _I, _DP]),
     "tg_shell_to_shell": (_I, [_P, _PP, _DP, _I, _DP]),
     "tg_energy_transfer": (_I, [_P, _PP, _DP, _I, _DP, _DP]),
+    "tg_ring_spectrum": (_I, [_P, _PP, _I, _I, _I, _DP, _I, _DP]),
+    "tg_cylindrical_ring_spectrum": (_I, [_P, _PP, _I, _I, _DP, _I, _DP, _DP]),
+    "tg_ring_to_ring": (_I, [_P, _PP, _I, _DP, _I, _DP, _I, _DP]),
+    "tg_cylindrical_ring_to_ring": (_I, [_P, _PP, _I, _DP, _I, _DP, _I, _DP]),
     "tg_init_field": (_I, [_P, _I, _D, _D, _D, _D, ctypes.c_uint64, _PP]),
     "tg_compute_nlin_mhd": (_I, [_P, _PP, _PP, _PP, _PP]),
     "tg_rk4_step_mhd": (_I, [_P, _PP, _PP, ctypes.POINTER(TgFluid), _D, _I]),

@@ -22,7 +22,7 @@ import ctypes
 import numpy as np
 import torch
 
-from . import _lib
+from . import _lib, h5io
 from ._lib import (DEALIAS_NONE, DEALIAS_REF_CORNER, DEALIAS_TWOTHIRDS, PREC_FP32, PREC_FP64, STEP_CONTINUE,
                    TgConfig, TgFluid, check, ptr_array, BASIS_SFF, SC_COS, SC_SIN, TgRbc)
 
@@ -40,6 +40,11 @@ class FFF_Basis:
     """Periodic Fourier basis on an (Nx, Ny, Nz) grid, one plan per instance."""
 
     basis_code = 0
+    fourier_x = True   # x is a Fourier axis (HDF5 reduced-plan layout)
+
+    @property
+    def N(self):
+        return (self.Nx, self.Ny, self.Nz)
 
     def __init__(self, N, kfactor=(1.0, 1.0, 1.0), precision=PREC_FP64, dealias=DEALIAS_REF_CORNER,
                  strong_reality=True, device=0):
@@ -156,6 +161,7 @@ class SFF_Basis(FFF_Basis):
     2 weak (the reference's RBC example applies the weak condition)."""
 
     basis_code = BASIS_SFF
+    fourier_x = False
 
     def __init__(self, N, kfactor=(np.pi, np.pi / np.sqrt(2.0), np.pi / np.sqrt(2.0)), precision=PREC_FP64,
                  dealias=DEALIAS_REF_CORNER, reality=2, device=0):
@@ -190,6 +196,22 @@ class FluidSF:
     def Compute_total_energy(self):
         self.total_energy = self.basis.Get_total_energy(self.F)
         return self.total_energy
+
+    def Write_complex_field(self, out_dir, t, reduced=None):
+        """CSF::Write_complex_field: <out_dir>/time_<t>/<name>.F.h5 (reduced:
+        <out_dir>/reduced_<t>/..., the low modes of N_out_reduced)."""
+        b = self.basis
+        if reduced is None:
+            h5io.write(self.F, b.N, h5io.FULL, out_dir, h5io.time_folder(t), self.name + ".F", b.fourier_x)
+        else:
+            h5io.write(self.F, b.N, h5io.REDUCED, out_dir, "reduced_" + h5io.time_folder(t)[5:], self.name + ".F",
+                       b.fourier_x, reduced)
+
+    def Read_complex_field(self, in_dir, reduced=None):
+        """CSF::Read_complex_field (or the reduced read: N_in_reduced modes)."""
+        b = self.basis
+        plan = h5io.FULL if reduced is None else h5io.REDUCED
+        h5io.read(self.F, b.N, plan, in_dir, self.name + ".F", b.fourier_x, reduced)
 
 
 class FluidVF:
@@ -228,6 +250,36 @@ class FluidVF:
         b = self.basis
         check("tg_init_field", b.L.tg_init_field(b.plan, kind, k0, kmin, kmax, energy, seed, self.ptrs()))
 
+    def Write_complex_field(self, out_dir, t, reduced=None, vx_vy=False):
+        """CVF::Write_complex_field (Cvf.cc:494-506): <out_dir>/time_<t>/<name>.V1.h5,
+        .V2, .V3 (vx_vy: V3's kz = 0 plane only, as <name>.V3kz0); with
+        reduced = N_out_reduced, CVF::Write_reduced_complex_field into
+        reduced_<t>/."""
+        b = self.basis
+        folder = h5io.time_folder(t) if reduced is None else "reduced_" + h5io.time_folder(t)[5:]
+        full, kz0 = (h5io.FULL, h5io.KZ0_FULL) if reduced is None else (h5io.REDUCED, h5io.KZ0_REDUCED)
+        for c, a in enumerate(self.cvf):
+            if c == 2 and vx_vy:
+                h5io.write(a, b.N, kz0, out_dir, folder, self.name + ".V3kz0", b.fourier_x, reduced)
+            else:
+                h5io.write(a, b.N, full, out_dir, folder, f"{self.name}.V{c + 1}", b.fourier_x, reduced)
+
+    def Read_complex_field(self, in_dir, reduced=None, vx_vy=False):
+        """CVF::Read_complex_field (Cvf.cc:528-539) / Read_reduced_complex_field
+        (:541-556); with vx_vy, V3 is rebuilt for kz > 0 from incompressibility
+        (FluidIO::Init_cond_complex_field, init_cond_field.cc:49-56)."""
+        b = self.basis
+        full, kz0 = (h5io.FULL, h5io.KZ0_FULL) if reduced is None else (h5io.REDUCED, h5io.KZ0_REDUCED)
+        for c, a in enumerate(self.cvf):
+            if c == 2 and vx_vy:
+                if reduced is not None:
+                    a.zero_()
+                h5io.read(a, b.N, kz0, in_dir, self.name + ".V3kz0", b.fourier_x, reduced)
+            else:
+                h5io.read(a, b.N, full, in_dir, f"{self.name}.V{c + 1}", b.fourier_x, reduced)
+        if vx_vy:
+            h5io.fill_vz(self.cvf[0], self.cvf[1], self.cvf[2], b.N, b.kfactor)
+
     def Compute_total_energy(self):
         e = [self.basis.Get_total_energy(a) for a in self.cvf]
         self.total_E1, self.total_E2, self.total_E3 = e
@@ -258,6 +310,34 @@ class Nlin_incompress:
 
 
 class Correlation:
+    @staticmethod
+    def Compute_ring_spectrum(U, n=0, sector_angles=None, dirn=3):
+        """Universal::Compute_ring_spectrum of U.cvf (or of a FluidSF's F):
+        [component, shell, sector]; Correlation.cc:301-311 multiplies the n = 2
+        form by 2 for the dissipation spectrum."""
+        b = U.basis
+        arrs = U.cvf if hasattr(U, "cvf") else [U.F]
+        a = np.ascontiguousarray(sector_angles if sector_angles is not None else sector_angles_equi(4), np.float64)
+        out = np.zeros((len(arrs), max_radius_inside(b) + 1, len(a)))
+        dp = ctypes.POINTER(ctypes.c_double)
+        check("tg_ring_spectrum", b.L.tg_ring_spectrum(b.plan, ptr_array([x.data_ptr() for x in arrs]), len(arrs),
+                                                       int(n), int(dirn), a.ctypes.data_as(dp), len(a) - 1,
+                                                       out.ctypes.data_as(dp)))
+        return out
+
+    @staticmethod
+    def Compute_cylindrical_ring_spectrum(U, n=0, kpll=None, dirn=3):
+        """Universal::Compute_cylindrical_ring_spectrum: (S1, S2) [Kperp shell, slab]."""
+        b = U.basis
+        h = np.ascontiguousarray(kpll if kpll is not None else kpll_edges_equi(b, 4, dirn), np.float64)
+        km = aniso_max_krho_inside(b, dirn)
+        S1, S2 = np.zeros((km + 1, len(h))), np.zeros((km + 1, len(h)))
+        dp = ctypes.POINTER(ctypes.c_double)
+        check("tg_cylindrical_ring_spectrum",
+              b.L.tg_cylindrical_ring_spectrum(b.plan, U.ptrs(), int(n), int(dirn), h.ctypes.data_as(dp), len(h) - 1,
+                                               S1.ctypes.data_as(dp), S2.ctypes.data_as(dp)))
+        return S1, S2
+
     @staticmethod
     def Get_Nusselt_no(U, T, Ra, Pr):
         """Correlation.cc:144-160 (PRLARGE / ULARGE)."""
@@ -445,3 +525,90 @@ class EnergyTr:
                                     out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
         self.shelltoshell_self = out
         return out
+
+    def Compute_ring_tr(self, U, radii=None, sector_angles=None, dirn=3):
+        """EnergyTr::Compute_ring_tr (compute_ring_tr.cc:62-77):
+        ring_to_ring_self[giver shell, giver sector, receiver shell, receiver sector]."""
+        b = self.basis
+        r = np.ascontiguousarray(radii if radii is not None else ring_radii(b, 8), dtype=np.float64)
+        a = np.ascontiguousarray(sector_angles if sector_angles is not None else sector_angles_equi(4), np.float64)
+        out = np.zeros((len(r), len(a), len(r), len(a)))
+        dp = ctypes.POINTER(ctypes.c_double)
+        check("tg_ring_to_ring", b.L.tg_ring_to_ring(b.plan, U.ptrs(), int(dirn), r.ctypes.data_as(dp), len(r) - 1,
+                                                     a.ctypes.data_as(dp), len(a) - 1, out.ctypes.data_as(dp)))
+        self.ring_to_ring_self = out
+        return out
+
+    def Compute_cylindrical_ring_tr(self, U, radii=None, kpll=None, dirn=3):
+        """EnergyTr::Compute_cylindrical_ring_tr (compute_cylin_ring_tr.cc:51-64):
+        [giver shell, giver slab, receiver shell, receiver slab]."""
+        b = self.basis
+        r = np.ascontiguousarray(radii if radii is not None else cylindrical_ring_radii(b, 8, dirn), np.float64)
+        h = np.ascontiguousarray(kpll if kpll is not None else kpll_edges_equi(b, 4, dirn), np.float64)
+        out = np.zeros((len(r), len(h), len(r), len(h)))
+        dp = ctypes.POINTER(ctypes.c_double)
+        check("tg_cylindrical_ring_to_ring",
+              b.L.tg_cylindrical_ring_to_ring(b.plan, U.ptrs(), int(dirn), r.ctypes.data_as(dp), len(r) - 1,
+                                              h.ctypes.data_as(dp), len(h) - 1, out.ctypes.data_as(dp)))
+        self.cylindrical_ring_to_ring_self = out
+        return out
+
+
+def sector_angles_equi(nsectors, max_theta=np.pi / 2):
+    """EQUISPACED sector angles i max_theta / n, the last one max_theta (pi/2 on
+    FFF, Get_max_polar_angle) (Global_process_advanced_vars.cc:100-107)."""
+    d = max_theta / nsectors
+    return np.array([i * d for i in range(nsectors)] + [max_theta])
+
+
+def _aniso_max_kpll(basis, dirn):
+    # Anis_max_Kpll (FFF_pencil_inline.h:767-781)
+    n = (basis.Nx, basis.Ny, basis.Nz)[dirn - 1]
+    return (n // 2) * basis.kfactor[dirn - 1]
+
+
+def aniso_max_krho_inside(basis, dirn):
+    """Anis_max_Krho_radius_inside (FFF_pencil_inline.h:784-798)."""
+    h = [(basis.Nx // 2) * basis.kfactor[0], (basis.Ny // 2) * basis.kfactor[1], (basis.Nz // 2) * basis.kfactor[2]]
+    del h[dirn - 1]
+    return int(min(h))
+
+
+def max_radius_inside(basis):
+    return int(min((basis.Nx // 2) * basis.kfactor[0], (basis.Ny // 2) * basis.kfactor[1],
+                   (basis.Nz // 2) * basis.kfactor[2]))
+
+
+def kpll_edges_equi(basis, nslabs, dirn=3):
+    """EQUISPACED slab edges over [0, Anis_max_Kpll] (Global_process_advanced_vars.cc:112-123)."""
+    d = _aniso_max_kpll(basis, dirn) / nslabs
+    return np.array([i * d for i in range(nslabs + 1)])
+
+
+def ring_radii(basis, nshells):
+    """Default ring-to-ring radii (Global_process_advanced_vars.cc:180-195):
+    0, 4, 8, log-spaced, Kmax/2, Kmax, inf with Kmax = Max_radius_inside."""
+    kmax = max_radius_inside(basis)
+    r = np.zeros(nshells + 1)
+    r[1], r[2] = 4.0, 8.0
+    r[nshells - 2], r[nshells - 1], r[nshells] = kmax / 2.0, kmax, np.inf
+    if nshells > 5:
+        s = np.log2(kmax / 16.0) / (nshells - 4)
+        for i in range(3, nshells - 2):
+            r[i] = 8 * 2 ** (s * (i - 2))
+    return r
+
+
+def cylindrical_ring_radii(basis, nshells, dirn=3):
+    """Default cylindrical-ring radii (Global_process_advanced_vars.cc:224-238):
+    as ring_radii, with the log spacing over Anis_max_Krho_radius_inside."""
+    kmax = max_radius_inside(basis)
+    kc = aniso_max_krho_inside(basis, dirn)
+    r = np.zeros(nshells + 1)
+    r[1], r[2] = 4.0, 8.0
+    r[nshells - 2], r[nshells - 1], r[nshells] = kmax / 2.0, kmax, np.inf
+    if nshells > 5:
+        s = np.log2(kc / 16.0) / (nshells - 4)
+        for i in range(3, nshells - 2):
+            r[i] = 8 * 2 ** (s * (i - 2))
+    return r

@@ -2,7 +2,8 @@
 
 Compiles every HIP source under tarang_amd/csrc with hipcc for gfx950 and
 links them into tarang_amd/libtarang_amd.so, whose exported symbols are the
-C ABI declared in include/tarang_gpu.h.
+C ABI declared in include/tarang_gpu.h; builds the HDF5 field-dump library
+tarang_amd/libtarang_io.so (include/tarang_io.h) and the C++ drivers.
 """
 import concurrent.futures
 import os
@@ -16,6 +17,10 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libtarang_amd.so")
 DRIVER = os.path.join(HERE, "tarang_ifluid")
 DRIVER_RBC = os.path.join(HERE, "tarang_rbc")
+IO_LIB = os.path.join(HERE, "libtarang_io.so")
+CXX = os.environ.get("CXX", "g++")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+HDF5 = os.environ.get("TG_HDF5_ROOT", "/opt/conda")
 SOURCES = ["tg_kernels.hip", "tg_diag.hip", "tg_mhd.hip", "tg_adjoint.hip", "tg_sff.hip", "tg_api.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("TG_OFFLOAD_ARCH", "gfx950")
@@ -54,9 +59,25 @@ def build(verbose=False):
         if r.returncode != 0:
             raise RuntimeError(f"driver build failed for {src}:\n{r.stdout}\n{r.stderr}")
         os.replace(out + ".tmp", out)
+    build_io()
     if verbose:
-        print(f"built {LIB}, {DRIVER} and {DRIVER_RBC}")
+        print(f"built {LIB}, {IO_LIB}, {DRIVER} and {DRIVER_RBC}")
     return LIB
+
+
+def build_io():
+    """libtarang_io.so: the HDF5 field dumps (host C++ over the HDF5 C library
+    of the image's /opt/conda and the HIP runtime for device copies)."""
+    cmd = [CXX, "-O2", "-fPIC", "-shared", "-std=c++17", "-D__HIP_PLATFORM_AMD__",
+           "-I", os.path.join(ROOT, "include"), "-I", os.path.join(ROCM, "include"), "-I", os.path.join(HDF5, "include"),
+           os.path.join(HERE, "io", "tg_io.cpp"), "-o", IO_LIB + ".tmp",
+           "-L", os.path.join(HDF5, "lib"), "-lhdf5", "-Wl,-rpath," + os.path.join(HDF5, "lib"),
+           "-L", os.path.join(ROCM, "lib"), "-lamdhip64", "-Wl,-rpath," + os.path.join(ROCM, "lib")]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"io library build failed:\n{r.stdout}\n{r.stderr}")
+    os.replace(IO_LIB + ".tmp", IO_LIB)
+    return IO_LIB
 
 
 if __name__ == "__main__":

@@ -365,22 +365,24 @@ int run_nlin(tg_plan *p, double *const V[3], double *const out[3], const tg_flui
     return TG_OK;
 }
 
-// N_i = FT[d_j(x_j w_i)] with x the advecting field X and w = Y (filter:
-// Y restricted to r_lo < K <= r_hi, Fill_array_shell, universal_ET.cc:880-892).
+// N_i = FT[d_j(x_j w_i)] with x the advecting field X and w = Y restricted
+// by the x-inverse filter flags (0 none; 8 r_lo < K <= r_hi, Fill_array_shell,
+// universal_ET.cc:880-892; 16 ring and 32 cylindrical ring about the axis
+// dirn, Fill_array_ring / Fill_array_cylindrical_ring, :897-933).
 template <typename T>
 // receiver_ready: U[0..2] still hold the x- and y-inverse of X from the
 // previous call (the z pass only reads them), so only the giver is
 // transformed (consecutive evaluations with one receiver: flux spheres,
 // shell-to-shell givers).
-int run_giver_nlin(tg_plan *p, double *const X[3], double *const Y[3], bool filter, double r_lo, double r_hi,
-                   double *const out[3], bool receiver_ready)
+int run_giver_nlin(tg_plan *p, double *const X[3], double *const Y[3], int filter, double r_lo, double r_hi,
+                   double *const out[3], bool receiver_ready, double a_lo, double a_hi, int dirn)
 {
     Band nb{};
     {
         Prof pr(p, TG_KERNEL_X_INV);
         if (!receiver_ready) HIPCHK(launch_xinv_state<T>(p->g, nb, X, 3, p->U, nullptr, 0, p->twx, p->stream));
-        HIPCHK(launch_xinv_state<T>(p->g, nb, Y, 3, p->U + 3, nullptr, filter ? 8 : 0, p->twx, p->stream, r_lo,
-                                    r_hi));
+        HIPCHK(launch_xinv_state<T>(p->g, nb, Y, 3, p->U + 3, nullptr, filter, p->twx, p->stream, r_lo, r_hi,
+                                    a_lo, a_hi, dirn));
     }
     {
         Prof pr(p, TG_KERNEL_Y_FFT);
@@ -893,6 +895,52 @@ int min_radius_outside(const Grid &g)
     return (int)std::ceil(std::sqrt(a * a + b * b + c * c));
 }
 
+// Anis_max_Krho_radius_inside (FFF_pencil_inline.h:784-798)
+int aniso_max_krho_inside(const Grid &g, int dirn)
+{
+    const double a = (g.nx / 2) * g.kf[0], b = (g.ny / 2) * g.kf[1], c = (g.nz / 2) * g.kf[2];
+    const double k = dirn == 1 ? std::fmin(b, c) : (dirn == 2 ? std::fmin(a, c) : std::fmin(a, b));
+    return (int)k;
+}
+
+// Anisotropic binned reduction into out[q * nbins_total + bin], in windows of
+// bins that fit the per-wavefront LDS bins and the partial buffer.  radii
+// (nshell + 1 entries, may be null) and aux (nsub + 1 entries) are host
+// arrays, uploaded behind the output window in the plan's small buffer.
+int reduce_aniso_to_host(tg_plan *p, int mode, const double *const *A, const double *const *B, int ncomp, int n,
+                         int nq, const double *radii_host, int nshell, const double *aux_host, int nsub, int dirn,
+                         int nbins_total, int kcut, double *out)
+{
+    const int win = std::max(1, std::min(nbins_total, std::min(1100, 2200 / nq)));
+    const size_t off_aux = (size_t)nq * win + 8, off_radii = off_aux + (size_t)nsub + 8;
+    const size_t need = sizeof(double) * (off_radii + (size_t)nshell + 8);
+    if (need > 65536) return fail(TG_ERR_ARG, "too many shells or sub-bins (%d, %d)", nshell, nsub);
+    double *outd = p->small, *auxd = p->small + off_aux, *radd = radii_host ? p->small + off_radii : nullptr;
+    HIPCHK(hipMemcpyAsync(auxd, aux_host, sizeof(double) * (nsub + 1), hipMemcpyHostToDevice, p->stream));
+    if (radii_host)
+        HIPCHK(hipMemcpyAsync(radd, radii_host, sizeof(double) * (nshell + 1), hipMemcpyHostToDevice, p->stream));
+    std::vector<double> tmp((size_t)nq * win);
+    for (int lo = 0; lo < nbins_total; lo += win) {
+        const int nb = std::min(win, nbins_total - lo);
+        RedAniso an{auxd, nsub, nshell, dirn, lo};
+        HIPCHK(launch_reduce(p->g, mode, A, B, ncomp, n, radd, nb, nq, kcut, 0.0, 0.0, p->partial, p->red_blocks,
+                             outd, p->stream, &an));
+        HIPCHK(hipMemcpyAsync(tmp.data(), outd, sizeof(double) * nq * nb, hipMemcpyDeviceToHost, p->stream));
+        HIPCHK(hipStreamSynchronize(p->stream));
+        for (int q = 0; q < nq; q++)
+            for (int b = 0; b < nb; b++) out[(size_t)q * nbins_total + lo + b] = tmp[(size_t)q * nb + b];
+    }
+    return TG_OK;
+}
+
+int check_edges(const double *e, int n, const char *what)
+{
+    if (!e || n < 1) return fail(TG_ERR_ARG, "%s: need at least one interval", what);
+    for (int i = 1; i <= n; i++)
+        if (!(e[i] >= e[i - 1])) return fail(TG_ERR_ARG, "%s must not decrease", what);
+    return TG_OK;
+}
+
 }  // namespace
 
 #define DISPATCH_PREC(p, CALL) ((p)->prec == TG_PREC_FP32 ? CALL<float> : CALL<double>)
@@ -1183,7 +1231,7 @@ int tg_compute_nlin_giver(tg_plan *plan, double *const V_dev[3], double *const W
         HIPCHK(launch_dealias(plan->g, V_dev, 3, plan->stream));
         HIPCHK(launch_dealias(plan->g, W_dev, 3, plan->stream));
     }
-    return DISPATCH_PREC(plan, run_giver_nlin)(plan, V_dev, W_dev, false, 0.0, 0.0, nlin_dev, false);
+    return DISPATCH_PREC(plan, run_giver_nlin)(plan, V_dev, W_dev, 0, 0.0, 0.0, nlin_dev, false, 0.0, 0.0, 3);
 }
 
 int tg_rk4_step(tg_plan *plan, double *const V_dev[3], const tg_fluid *fluid, int nsteps, int flags)
@@ -1249,7 +1297,7 @@ int tg_flux(tg_plan *plan, double *const V_dev[3], const double *radii, int nsph
     if (plan->g.dealias) HIPCHK(launch_dealias(plan->g, V_dev, 3, plan->stream));
     flux[0] = 0.0;
     for (int s = 1; s <= nspheres; s++) {
-        rc = DISPATCH_PREC(plan, run_giver_nlin)(plan, V_dev, V_dev, true, 0.0, radii[s], plan->acc, s > 1);
+        rc = DISPATCH_PREC(plan, run_giver_nlin)(plan, V_dev, V_dev, 8, 0.0, radii[s], plan->acc, s > 1, 0.0, 0.0, 3);
         if (rc) return rc;
         double v = 0.0;
         const double *A[3] = {plan->acc[0], plan->acc[1], plan->acc[2]};
@@ -1275,8 +1323,8 @@ int tg_shell_to_shell(tg_plan *plan, double *const V_dev[3], const double *radii
     std::vector<double> row(nr);
     const int kin = max_radius_inside(plan->g);
     for (int m = 1; m < nshells; m++) {
-        rc = DISPATCH_PREC(plan, run_giver_nlin)(plan, V_dev, V_dev, true, radii[m - 1], radii[m], plan->acc,
-                                                 m > 1);
+        rc = DISPATCH_PREC(plan, run_giver_nlin)(plan, V_dev, V_dev, 8, radii[m - 1], radii[m], plan->acc,
+                                                 m > 1, 0.0, 0.0, 3);
         if (rc) return rc;
         const double *A[3] = {plan->acc[0], plan->acc[1], plan->acc[2]};
         const double *B[3] = {V_dev[0], V_dev[1], V_dev[2]};
@@ -1299,10 +1347,12 @@ int tg_energy_transfer(tg_plan *plan, double *const V_dev[3], const double *radi
     // the flux is assembled from the shell-to-shell givers 1..n-1: every mode
     // must lie in one of them or in the receiver-only last shell, i.e. the
     // first sphere is the origin and the last one encloses every mode
-    // (Min_radius_outside); radii strictly increasing
+    // (Min_radius_outside); radii non-decreasing (the default radii repeat
+    // a value when their log spacing is finer than the grid, leaving empty
+    // shells)
     if (!(radii[0] == 0.0)) return fail(TG_ERR_ARG, "tg_energy_transfer needs radii[0] == 0");
     for (int i = 1; i <= n; i++)
-        if (!(radii[i] > radii[i - 1])) return fail(TG_ERR_ARG, "radii must increase strictly");
+        if (!(radii[i] >= radii[i - 1])) return fail(TG_ERR_ARG, "radii must not decrease");
     if (!(radii[n] >= (double)min_radius_outside(plan->g)))
         return fail(TG_ERR_ARG, "tg_energy_transfer needs radii[n] >= %d (a last sphere enclosing every mode)",
                     min_radius_outside(plan->g));
@@ -1314,8 +1364,8 @@ int tg_energy_transfer(tg_plan *plan, double *const V_dev[3], const double *radi
     std::vector<double> row(nr), T((size_t)nr * nr, 0.0);
     const int kin = max_radius_inside(plan->g);
     for (int m = 1; m < n; m++) {
-        rc = DISPATCH_PREC(plan, run_giver_nlin)(plan, V_dev, V_dev, true, radii[m - 1], radii[m], plan->acc,
-                                                 m > 1);
+        rc = DISPATCH_PREC(plan, run_giver_nlin)(plan, V_dev, V_dev, 8, radii[m - 1], radii[m], plan->acc,
+                                                 m > 1, 0.0, 0.0, 3);
         if (rc) return rc;
         const double *A[3] = {plan->acc[0], plan->acc[1], plan->acc[2]};
         const double *B[3] = {V_dev[0], V_dev[1], V_dev[2]};
@@ -1335,6 +1385,93 @@ int tg_energy_transfer(tg_plan *plan, double *const V_dev[3], const double *radi
         flux[s] = f;
     }
     return TG_OK;
+}
+
+int tg_ring_spectrum(tg_plan *plan, const double *const A_dev[], int ncomp, int n, int dirn,
+                     const double *sector_angles, int nsectors, double *out)
+{
+    if (!plan || !A_dev || !out || n < 0 || (ncomp != 1 && ncomp != 3)) return fail(TG_ERR_ARG, "bad argument");
+    if (int rb_ = need_basis(plan, 0, "tg_ring_spectrum")) return rb_;
+    if (dirn < 1 || dirn > 3) return fail(TG_ERR_ARG, "anisotropy direction must be 1, 2 or 3");
+    int rc = check_ptrs((const void *const *)A_dev, ncomp, "A");
+    if (!rc) rc = check_edges(sector_angles, nsectors, "sector angles");
+    if (!rc) rc = set_device(plan);
+    if (rc) return rc;
+    const int kmax = max_radius_inside(plan->g);
+    return reduce_aniso_to_host(plan, RED_RING_SPEC, A_dev, nullptr, ncomp, n, ncomp, nullptr, 0, sector_angles,
+                                nsectors, dirn, (kmax + 1) * (nsectors + 1), kmax, out);
+}
+
+int tg_cylindrical_ring_spectrum(tg_plan *plan, const double *const V_dev[3], int n, int dirn, const double *kpll,
+                                 int nslabs, double *S1, double *S2)
+{
+    if (!plan || !V_dev || !S1 || !S2 || n < 0) return fail(TG_ERR_ARG, "bad argument");
+    if (int rb_ = need_basis(plan, 0, "tg_cylindrical_ring_spectrum")) return rb_;
+    if (dirn < 1 || dirn > 3) return fail(TG_ERR_ARG, "anisotropy direction must be 1, 2 or 3");
+    int rc = check_ptrs((const void *const *)V_dev, 3, "V");
+    if (!rc) rc = check_edges(kpll, nslabs, "Kpll edges");
+    if (!rc) rc = set_device(plan);
+    if (rc) return rc;
+    const int kmax = aniso_max_krho_inside(plan->g, dirn);
+    const int nb = (kmax + 1) * (nslabs + 1);
+    std::vector<double> tmp(2 * (size_t)nb);
+    rc = reduce_aniso_to_host(plan, RED_CYL_SPEC, V_dev, nullptr, 3, n, 2, nullptr, 0, kpll, nslabs, dirn, nb, kmax,
+                              tmp.data());
+    if (rc) return rc;
+    std::copy(tmp.begin(), tmp.begin() + nb, S1);
+    std::copy(tmp.begin() + nb, tmp.end(), S2);
+    return TG_OK;
+}
+
+// Ring-to-ring (mode 16) and cylindrical-ring-to-ring (mode 32) transfers:
+// one giver evaluation per (shell, sub-bin) giver ring, binned receivers.
+static int aniso_transfer(tg_plan *plan, double *const V_dev[3], int dirn, const double *radii, int nshells,
+                          const double *sub, int nsub, double *out, bool cyl, const char *what)
+{
+    if (!plan || !V_dev || !out) return fail(TG_ERR_ARG, "null argument");
+    if (int rb_ = need_basis(plan, 0, what)) return rb_;
+    if (dirn < 1 || dirn > 3) return fail(TG_ERR_ARG, "anisotropy direction must be 1, 2 or 3");
+    int rc = check_ptrs((const void *const *)V_dev, 3, "V");
+    if (!rc) rc = check_edges(radii, nshells, "radii");
+    if (!rc) rc = check_edges(sub, nsub, cyl ? "Kpll edges" : "sector angles");
+    if (!rc) rc = set_device(plan);
+    if (rc) return rc;
+    const size_t rb = (size_t)(nshells + 1) * (nsub + 1);
+    if (rb > 4096) return fail(TG_ERR_ARG, "too many rings (%zu)", rb);
+    // Compute_nlin(U, Giver) dealiases U in place (compute_nlin_ET_VF.cc:74-77)
+    if (plan->g.dealias) HIPCHK(launch_dealias(plan->g, V_dev, 3, plan->stream));
+    std::fill(out, out + rb * rb, 0.0);
+    const int kcut = cyl ? aniso_max_krho_inside(plan->g, dirn) : max_radius_inside(plan->g);
+    std::vector<double> row(rb);
+    const double *B[3] = {V_dev[0], V_dev[1], V_dev[2]};
+    const double *A[3] = {plan->acc[0], plan->acc[1], plan->acc[2]};
+    bool first = true;
+    // the last shell (outer radius infinite) gives nothing (compute_ring_tr.cc:64)
+    for (int i = 1; i < nshells; i++)
+        for (int j = 1; j <= nsub; j++) {
+            rc = DISPATCH_PREC(plan, run_giver_nlin)(plan, V_dev, V_dev, cyl ? 32 : 16, radii[i - 1], radii[i],
+                                                     plan->acc, !first, sub[j - 1], sub[j], dirn);
+            if (rc) return rc;
+            first = false;
+            rc = reduce_aniso_to_host(plan, cyl ? RED_CYL_MULT : RED_RING_MULT, A, B, 3, 0, 1, radii, nshells, sub,
+                                      nsub, dirn, (int)rb, kcut, row.data());
+            if (rc) return rc;
+            double *o = out + ((size_t)i * (nsub + 1) + j) * rb;
+            for (size_t k = 0; k < rb; k++) o[k] = -row[k];
+        }
+    return TG_OK;
+}
+
+int tg_ring_to_ring(tg_plan *plan, double *const V_dev[3], int dirn, const double *radii, int nshells,
+                    const double *sector_angles, int nsectors, double *out)
+{
+    return aniso_transfer(plan, V_dev, dirn, radii, nshells, sector_angles, nsectors, out, false, "tg_ring_to_ring");
+}
+
+int tg_cylindrical_ring_to_ring(tg_plan *plan, double *const V_dev[3], int dirn, const double *radii, int nshells,
+                                const double *kpll, int nslabs, double *out)
+{
+    return aniso_transfer(plan, V_dev, dirn, radii, nshells, kpll, nslabs, out, true, "tg_cylindrical_ring_to_ring");
 }
 
 int tg_compute_nlin_mhd(tg_plan *plan, double *const V_dev[3], double *const W_dev[3], double *const nlinV_dev[3],
@@ -1405,14 +1542,14 @@ int tg_flux_mhd(tg_plan *plan, double *const V_dev[3], double *const W_dev[3], c
     u2b[0] = b2b[0] = 0.0;
     for (int s = 1; s <= nspheres; s++) {
         // W.nlin = FT[d_j(b_j u<_i)]; u< -> b> = sum_{K > r} 2m Re(N.b*)
-        rc = DISPATCH_PREC(plan, run_giver_nlin)(plan, W_dev, V_dev, true, 0.0, radii[s], plan->acc, false);
+        rc = DISPATCH_PREC(plan, run_giver_nlin)(plan, W_dev, V_dev, 8, 0.0, radii[s], plan->acc, false, 0.0, 0.0, 3);
         if (rc) return rc;
         rc = reduce_to_host(plan, RED_SHELL_SINGLE, A, B, 3, 0, nullptr, 1, 1, 0, radii[s], INFINITY, &u2b[s]);
         if (rc) return rc;
     }
     for (int s = 1; s <= nspheres; s++) {
         // U.nlin = FT[d_j(u_j b<_i)]; b< -> b> = -sum_{K > r} 2m Re(N.b*)
-        rc = DISPATCH_PREC(plan, run_giver_nlin)(plan, V_dev, W_dev, true, 0.0, radii[s], plan->acc, false);
+        rc = DISPATCH_PREC(plan, run_giver_nlin)(plan, V_dev, W_dev, 8, 0.0, radii[s], plan->acc, false, 0.0, 0.0, 3);
         if (rc) return rc;
         double v = 0.0;
         rc = reduce_to_host(plan, RED_SHELL_SINGLE, A, B, 3, 0, nullptr, 1, 1, 0, radii[s], INFINITY, &v);

@@ -10,6 +10,7 @@
 #include "tg_device.h"
 #include "tg_kernels.h"
 #include "tg_diag.h"
+#include "tg_aniso.h"
 
 namespace tg {
 
@@ -59,6 +60,7 @@ struct RedArgs {
     Grid g;
     int mode, ncomp, n, nbins, nq, kcut;  // kcut: Max_radius_inside (mode SHELL_ALL) / Min_radius_outside
     double r_lo, r_hi;
+    RedAniso an;
 };
 
 constexpr int kRedThreads = 256;
@@ -96,6 +98,24 @@ __device__ __forceinline__ void wave_bin_add(double *bs, double *bc, int nb, int
     }
 }
 
+// wave_bin_add with a run-time number of quantities (nq <= 3)
+__device__ __forceinline__ void wave_bin_add_n(double *bs, double *bc, int nb, int nq, int bin, const double *val,
+                                               bool active)
+{
+    const int lane = threadIdx.x & 63;
+    unsigned long long pending = __ballot(active);
+    while (pending) {
+        const int leader = __ffsll((long long)pending) - 1;
+        const int b = __shfl(bin, leader, 64);
+        const bool mine = active && bin == b;
+        for (int q = 0; q < nq; q++) {
+            const double t = wave_sum(mine ? val[q] : 0.0);
+            if (lane == leader) neumaier(bs[q * nb + b], bc[q * nb + b], t);
+        }
+        pending &= ~__ballot(mine);
+    }
+}
+
 // Modes of block b: a fixed contiguous slice of the flattened mode index.
 // Scalar modes: per-lane Neumaier sums, then a fixed-order block tree.
 // Binned modes: per-wavefront compensated bins (LDS: [NW][nq][nbins] sums,
@@ -108,7 +128,9 @@ __global__ void __launch_bounds__(kRedThreads) k_reduce(RedArgs a)
     const Grid &g = a.g;
     const int wave = threadIdx.x / 64;
     const int nb = a.nbins, nq = a.nq;
-    constexpr bool BINNED = MODE == RED_SPECTRUM || MODE == RED_SHELL_ALL || MODE == RED_SHELL_ALL_NOCUT;
+    constexpr bool ANIS = MODE == RED_RING_SPEC || MODE == RED_CYL_SPEC || MODE == RED_RING_MULT ||
+                          MODE == RED_CYL_MULT;
+    constexpr bool BINNED = MODE == RED_SPECTRUM || MODE == RED_SHELL_ALL || MODE == RED_SHELL_ALL_NOCUT || ANIS;
     double *bsum = dyn + (size_t)wave * nq * nb;
     double *bcmp = dyn + (size_t)(NW + wave) * nq * nb;
     if (BINNED) {
@@ -171,6 +193,69 @@ __global__ void __launch_bounds__(kRedThreads) k_reduce(RedArgs a)
             }
             const double val[1] = {2 * multiplicity(g, lx, ly, lz) * d};
             wave_bin_add<1>(bsum, bcmp, nb, sidx, val, act);
+        } else if (ANIS) {
+            const RedAniso &an = a.an;
+            const int kx = kw(lx, g.nx), ky = kw(ly, g.ny);
+            const double K = kmag_l(g, lx, ly, lz);
+            const double m = multiplicity(g, lx, ly, lz);
+            double val[3] = {0.0, 0.0, 0.0};
+            int shell = 0, sub = 0;
+            bool act = valid;
+            if (MODE == RED_RING_SPEC || MODE == RED_RING_MULT) {
+                // Compute_local_ring_spectrum (universal_energy.cc:552-590) and
+                // Local_ring_mult_all (universal_ET.cc:205-230)
+                if (MODE == RED_RING_SPEC) {
+                    shell = (int)ceil(K);
+                    act = act && K > kAnisEps && shell <= a.kcut;
+                } else {
+                    act = act && K <= a.kcut && K > kAnisEps;
+                    shell = first_ge(a.radii, an.nshell, K);
+                }
+                sub = sector_of(aniso_polar(g, kx, ky, lz, an.dirn), an.aux, an.nsub);
+            } else {
+                // Compute_local_cylindrical_ring_spectrum (universal_energy.cc:
+                // 868-906) and Local_cyl_ring_mult_all (universal_ET.cc:335-362)
+                const double kperp = aniso_kperp(g, kx, ky, lz, an.dirn);
+                if (MODE == RED_CYL_SPEC) {
+                    shell = (int)ceil(kperp);
+                    act = act && shell <= a.kcut;
+                } else {
+                    act = act && kperp <= a.kcut;
+                    shell = first_ge(a.radii, an.nshell, kperp);
+                }
+                sub = slab_of(aniso_kpll(g, kx, ky, lz, an.dirn), an.aux, an.nsub);
+            }
+            if (MODE == RED_RING_SPEC || MODE == RED_CYL_SPEC) {
+                double kn = 1.0;
+                for (int t = 0; t < a.n; t++) kn *= K;
+                const double f = m * kn;
+                if (MODE == RED_RING_SPEC) {
+                    for (int c = 0; c < a.nq; c++) {
+                        const double *A = a.A[c];
+                        val[c] = f * (A[2 * ii] * A[2 * ii] + A[2 * ii + 1] * A[2 * ii + 1]);
+                    }
+                } else {
+                    double v2 = 0.0, p2 = 0.0;
+                    for (int c = 0; c < 3; c++) {
+                        const double *A = a.A[c];
+                        const double s2 = A[2 * ii] * A[2 * ii] + A[2 * ii + 1] * A[2 * ii + 1];
+                        v2 += s2;
+                        if (c == an.dirn - 1) p2 = s2;
+                    }
+                    val[0] = f * p2;
+                    val[1] = f * (v2 - p2);
+                }
+            } else {
+                double d = 0.0;
+                for (int c = 0; c < 3; c++) {
+                    const double *A = a.A[c], *B = a.B[c];
+                    d += A[2 * ii] * B[2 * ii] + A[2 * ii + 1] * B[2 * ii + 1];
+                }
+                val[0] = 2 * m * d;
+            }
+            const int bin = shell * (an.nsub + 1) + sub - an.bin_lo;
+            act = act && bin >= 0 && bin < nb;
+            wave_bin_add_n(bsum, bcmp, nb, nq, act ? bin : 0, val, act);
         } else if (MODE == RED_SHELL_SINGLE) {
             const double K = kmag_l(g, lx, ly, lz);
             if (valid && K > a.r_lo && K <= a.r_hi) {
@@ -219,9 +304,10 @@ __global__ void k_reduce_final(const double *partial, int nblocks, int width, do
 
 hipError_t launch_reduce(const Grid &g, int mode, const double *const *A, const double *const *B, int ncomp,
                          int n, const double *radii_dev, int nbins, int nq, int kcut, double r_lo, double r_hi,
-                         double *partial, int nblocks, double *out_dev, hipStream_t st)
+                         double *partial, int nblocks, double *out_dev, hipStream_t st, const RedAniso *an)
 {
     RedArgs a{};
+    if (an) a.an = *an;
     for (int c = 0; c < ncomp; c++) {
         a.A[c] = A[c];
         a.B[c] = B ? B[c] : nullptr;
@@ -260,6 +346,14 @@ hipError_t launch_reduce(const Grid &g, int mode, const double *const *A, const 
             hipLaunchKernelGGL(k_reduce<RED_SHELL_SINGLE>, dim3(nblocks), dim3(threads), shm, st, a);
             break;
         case RED_CROSS: hipLaunchKernelGGL(k_reduce<RED_CROSS>, dim3(nblocks), dim3(threads), shm, st, a); break;
+        case RED_RING_SPEC:
+            hipLaunchKernelGGL(k_reduce<RED_RING_SPEC>, dim3(nblocks), dim3(threads), shm, st, a);
+            break;
+        case RED_CYL_SPEC: hipLaunchKernelGGL(k_reduce<RED_CYL_SPEC>, dim3(nblocks), dim3(threads), shm, st, a); break;
+        case RED_RING_MULT:
+            hipLaunchKernelGGL(k_reduce<RED_RING_MULT>, dim3(nblocks), dim3(threads), shm, st, a);
+            break;
+        case RED_CYL_MULT: hipLaunchKernelGGL(k_reduce<RED_CYL_MULT>, dim3(nblocks), dim3(threads), shm, st, a); break;
         default: return hipErrorInvalidValue;
     }
     hipError_t e = hipGetLastError();

@@ -33,11 +33,14 @@ hipError_t launch_colfft(const Grid &g, int axis, int dir, void *const *fields, 
 
 // x-inverse of an FP64 spectral field set into the T work layout.
 //   flags: 1 = dealias on read, 2 = write band slots, 4 = reality patch
-//   columns only (kz = 0 and kz = Nz/2), 8 = keep only r_lo < K <= r_hi.
+//   columns only (kz = 0 and kz = Nz/2), 8 = keep only r_lo < K <= r_hi,
+//   16 = keep the ring r_lo < K <= r_hi, a_lo < theta <= a_hi (polar angle
+//   to the axis dirn), 32 = keep the cylindrical ring r_lo < K <= r_hi,
+//   a_lo < Kpll <= a_hi.
 template <typename T>
 hipError_t launch_xinv_state(const Grid &g, const Band &b, double *const *V, int ncomp, void *const *U,
                              double *slots, int flags, const void *twx, hipStream_t st, double r_lo = 0.0,
-                             double r_hi = 0.0);
+                             double r_hi = 0.0, double a_lo = 0.0, double a_hi = 0.0, int dirn = 3);
 
 // x-forward of T work fields into FP64 spectral outputs, scaled.
 template <typename T>

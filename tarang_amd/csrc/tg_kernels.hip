@@ -21,6 +21,7 @@
 #include "tg_device.h"
 #include "tg_kernels.h"
 #include "tg_pass.h"
+#include "tg_aniso.h"
 
 namespace tg {
 
@@ -100,6 +101,11 @@ struct XinvArgs {
     Band b;
     int ncomp, flags;
     double r_lo, r_hi;  // flags & 8: keep only r_lo < K <= r_hi (Fill_array_shell)
+    // flags & 16: ring r_lo < K <= r_hi, a_lo < theta <= a_hi (Fill_array_ring);
+    // flags & 32: cylindrical ring r_lo < K <= r_hi, a_lo < Kpll <= a_hi
+    // (Fill_array_cylindrical_ring); theta, Kpll about the axis dirn
+    double a_lo, a_hi;
+    int dirn;
 };
 
 template <typename T, int L, int CB>
@@ -143,6 +149,15 @@ __global__ void __launch_bounds__(CB *(L / 8)) k_xinv_state(XinvArgs a)
             if (a.flags & 8) {
                 const double K = kmag(g, kwave(lx, g.nx), ky, lz);
                 if (!(K > a.r_lo && K <= a.r_hi)) d = mk<double>(0, 0);
+            }
+            if (a.flags & 48) {
+                const int kx = kwave(lx, g.nx);
+                const double K = kmag(g, kx, ky, lz);
+                // universal_ET.cc:909-911 (ring) and :929-931 (cylindrical ring)
+                const double q = (a.flags & 16) ? aniso_polar(g, kx, ky, lz, a.dirn) : aniso_kpll(g, kx, ky, lz, a.dirn);
+                const bool in = K > a.r_lo && K <= a.r_hi &&
+                                ((q > a.a_lo && q <= a.a_hi) || (fabs(q) < kAnisEps && a.a_lo < kAnisEps));
+                if (!in) d = mk<double>(0, 0);
             }
             v[s] = cvt<T>(d);
         }
@@ -779,11 +794,15 @@ hipError_t launch_colfft(const Grid &g, int axis, int dir, void *const *fields, 
 
 template <typename T>
 hipError_t launch_xinv_state(const Grid &g, const Band &b, double *const *V, int ncomp, void *const *U,
-                             double *slots, int flags, const void *twx, hipStream_t st, double r_lo, double r_hi)
+                             double *slots, int flags, const void *twx, hipStream_t st, double r_lo, double r_hi,
+                             double a_lo, double a_hi, int dirn)
 {
     XinvArgs a{};
     a.r_lo = r_lo;
     a.r_hi = r_hi;
+    a.a_lo = a_lo;
+    a.a_hi = a_hi;
+    a.dirn = dirn;
     for (int i = 0; i < ncomp; i++) { a.V[i] = V[i]; a.U[i] = U[i]; }
     a.slots = slots;
     a.tw = twx;
@@ -962,7 +981,8 @@ hipError_t launch_dealias(const Grid &g, double *const *V, int ncomp, hipStream_
 #define TG_INST(T)                                                                                              \
     template hipError_t launch_colfft<T>(const Grid &, int, int, void *const *, int, const void *, hipStream_t); \
     template hipError_t launch_xinv_state<T>(const Grid &, const Band &, double *const *, int, void *const *,   \
-                                             double *, int, const void *, hipStream_t, double, double);        \
+                                             double *, int, const void *, hipStream_t, double, double, double, \
+                                             double, int);                                                     \
     template hipError_t launch_xfwd_spec<T>(const Grid &, void *const *, int, double *const *, double,          \
                                             const void *, hipStream_t);                                        \
     template hipError_t launch_z_c2r<T>(const Grid &, void *const *, int, double *const *, const void *,        \

@@ -73,3 +73,16 @@ def test_header_is_c_compatible(tmp_path):
     r = subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), str(src), "-o",
                         str(tmp_path / "t")], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_io_library_exports_every_header_symbol():
+    import re
+    from tarang_amd import h5io
+    h5io.load()
+    text = open(os.path.join(ROOT, "include", "tarang_io.h")).read()
+    syms = sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(tgio_\w+)\s*\(", text, flags=re.M)))
+    assert len(syms) == 4
+    out = subprocess.run(["nm", "-D", "--defined-only", h5io.LIB_PATH], capture_output=True, text=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if " T " in line}
+    assert set(syms) <= exported
+    assert not sorted(s for s in exported if s.startswith("tgio_") and s not in syms)
