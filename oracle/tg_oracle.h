@@ -140,6 +140,20 @@ void or_ring_to_ring(const or_grid *g, double *V[3], int dirn, const double *rad
 void or_cyl_ring_to_ring(const or_grid *g, double *V[3], int dirn, const double *radii, int nshells,
                          const double *kpll_array, int nslabs, double *out);
 
+/* ---- Chebyshev (ChFF) path (tg_oracle_chff.c) ---- */
+void or_chff_forward(const or_grid *g, const double *real_in, double *spec_out);
+void or_chff_inverse(const or_grid *g, const double *spec_in, double *real_out);
+void or_cheb_xderiv(const or_grid *g, const double *A, double *B);
+void or_helmholtz_line(int N, const double *f, double *x, double lambda, double vp, double vm);
+void or_helmholtz(const or_grid *g, const double *R, double *U, double lam_sup, const double vp[2],
+                  const double vm[2]);
+void or_chff_influence(const or_grid *g, double nu, double dt, double *pp, double *pm, double *vxp, double *vxm,
+                       double *M);
+void or_chff_correct(const or_grid *g, const double *pp, const double *pm, const double *vxp, const double *vxm,
+                     const double *M, double *P, double *V1);
+void or_compute_nlin_chff(const or_grid *g, double *const V[3], double *N[3]);
+void or_bdf1_chff(const or_grid *g, double nu, double dt, double *V[3], double *P);
+
 /* ---- seeded initial condition (this build's own definition) ---- */
 void or_init_field(const or_grid *g, int spectrum_kind, double k0, double kmin, double kmax,
                    double energy, uint64_t seed, double *V[3]);

@@ -105,6 +105,15 @@ def lib():
             "or_compute_rhs_rbc": (None, [gp, p, p, pp, p]),
             "or_rk4_step_rbc": (None, [gp, ctypes.POINTER(Rbc), pp, p]),
             "or_nusselt": (d, [gp, ctypes.POINTER(Rbc), p, p]),
+            "or_chff_forward": (None, [gp, p, p]),
+            "or_chff_inverse": (None, [gp, p, p]),
+            "or_cheb_xderiv": (None, [gp, p, p]),
+            "or_helmholtz_line": (None, [i, p, p, d, d, d]),
+            "or_helmholtz": (None, [gp, p, p, d, p, p]),
+            "or_chff_influence": (None, [gp, d, d, p, p, p, p, p]),
+            "or_chff_correct": (None, [gp, p, p, p, p, p, p, p]),
+            "or_compute_nlin_chff": (None, [gp, pp, pp]),
+            "or_bdf1_chff": (None, [gp, d, d, pp, p]),
             "or_aniso_kpll": (d, [gp, i, i, i, i]),
             "or_aniso_kperp": (d, [gp, i, i, i, i]),
             "or_polar_angle": (d, [gp, i, i, i, i]),
@@ -302,6 +311,53 @@ class Oracle:
         loss = self.L.or_adjoint_shell_energy(ctypes.byref(self.g), ctypes.byref(fluid), _pp(V0), nsteps, r_lo, r_hi,
                                               _pp(grad))
         return loss, grad
+
+    # ---- Chebyshev (ChFF): Chebyshev-Gauss points along x ----
+    def chff_forward(self, real):
+        real = np.ascontiguousarray(real, np.float64)
+        out = self.czeros()
+        self.L.or_chff_forward(ctypes.byref(self.g), _ptr(real), _ptr(out))
+        return out
+
+    def chff_inverse(self, spec):
+        out = np.zeros(self.rshape)
+        self.L.or_chff_inverse(ctypes.byref(self.g), _cptr(spec), _ptr(out))
+        return out
+
+    def cheb_xderiv(self, A):
+        B = self.czeros()
+        self.L.or_cheb_xderiv(ctypes.byref(self.g), _cptr(A), _cptr(B))
+        return B
+
+    def helmholtz_line(self, f, lam, vp, vm):
+        f = np.ascontiguousarray(f, np.float64)
+        x = np.zeros_like(f)
+        self.L.or_helmholtz_line(len(f), _ptr(f), _ptr(x), lam, vp, vm)
+        return x
+
+    def helmholtz(self, R, lam_sup, vp=0j, vm=0j):
+        U = self.czeros()
+        a = np.array([complex(vp).real, complex(vp).imag])
+        b = np.array([complex(vm).real, complex(vm).imag])
+        self.L.or_helmholtz(ctypes.byref(self.g), _cptr(R), _cptr(U), lam_sup, _ptr(a), _ptr(b))
+        return U
+
+    def chff_influence(self, nu, dt):
+        shp = self.cshape
+        pp, pm, vxp, vxm = (np.zeros(shp) for _ in range(4))
+        M = np.zeros((shp[1], shp[2], 4))
+        self.L.or_chff_influence(ctypes.byref(self.g), nu, dt, _ptr(pp), _ptr(pm), _ptr(vxp), _ptr(vxm), _ptr(M))
+        return pp, pm, vxp, vxm, M
+
+    def compute_nlin_chff(self, V):
+        N = [self.czeros() for _ in range(3)]
+        self.L.or_compute_nlin_chff(ctypes.byref(self.g), _pp(V), _pp(N))
+        return N
+
+    def bdf1_chff(self, nu, dt, V):
+        P = self.czeros()
+        self.L.or_bdf1_chff(ctypes.byref(self.g), nu, dt, _pp(V), _cptr(P))
+        return P
 
     # ---- anisotropic diagnostics (dirn = anisotropy_dirn 1..3) ----
     def polar_angle(self, lx, ly, lz, dirn=3):
