@@ -187,6 +187,20 @@ int tg_Sn(tg_plan *plan, const double *A_dev, int n, double *Sn);
 /* Universal::Compute_shell_spectrum (universal_energy.cc:279-360), Stepanov
  * normalisation; nbins >= Min_radius_outside()+1. */
 int tg_shell_spectrum(tg_plan *plan, const double *A_dev, int n, double *Sk, int nbins);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): tg_shell_spectrum without a plan and without any
+ * allocation.  The grid, kfactor and device come from cfg (FFF basis; other
+ * fields ignored); A_dev is the complex field (Nx, Ny, Nz/2+1); Sk_dev
+ * (device memory, nbins >= Min_radius_outside + 1 doubles) receives the same
+ * E(k) as tg_shell_spectrum, bit for bit.  The block partial sums live in a
+ * static 16 MiB device buffer of the library, one per GPU, held until the
+ * process ends: grids whose partials exceed it (about 590^3 at kfactor 1)
+ * return TG_ERR_ARG.  Asynchronous on `stream` (a hipStream_t, null for the
+ * default stream); makes cfg->device current.  Calls that overlap on one GPU
+ * share the buffer and race: serialise them (one stream, or events). */
+int tg_shell_spectrum_noalloc(const tg_config *cfg, const double *A_dev, int n, double *Sk_dev, int nbins,
+                              void *stream);
 /* EnergyTr::Compute_flux(FluidVF&) (compute_flux.cc:51-65): flux[0..nspheres],
  * radii[0..nspheres].  Dealiases V in place. */
 int tg_flux(tg_plan *plan, double *const V_dev[3], const double *radii, int nspheres, double *flux);

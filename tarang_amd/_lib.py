@@ -76,6 +76,7 @@ SIGNATURES = {
     "tg_shell_to_shell": (_I, [_P, _PP, _DP, _I, _DP]),
     "tg_energy_transfer": (_I, [_P, _PP, _DP, _I, _DP, _DP]),
     "tg_ring_spectrum": (_I, [_P, _PP, _I, _I, _I, _DP, _I, _DP]),
+    "tg_shell_spectrum_noalloc": (_I, [ctypes.POINTER(TgConfig), _P, _I, _P, _I, _P]),
     "tg_cylindrical_ring_spectrum": (_I, [_P, _PP, _I, _I, _DP, _I, _DP, _DP]),
     "tg_ring_to_ring": (_I, [_P, _PP, _I, _DP, _I, _DP, _I, _DP]),
     "tg_cylindrical_ring_to_ring": (_I, [_P, _PP, _I, _DP, _I, _DP, _I, _DP]),

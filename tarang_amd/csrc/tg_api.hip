@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -1666,3 +1667,100 @@ int tg_init_field(tg_plan *plan, int kind, double k0, double kmin, double kmax, 
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Allocation-free shell spectrum: the block partials and the raw bins live in
+// a static device buffer (one per GPU, held until the process ends) instead
+// of plan-owned memory.  Its device address comes from hipGetSymbolAddress
+// on each device's first call; the array's name is never used in host code
+// as a device pointer.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr size_t kScratchDoubles = size_t(1) << 21;  // 16 MiB
+constexpr int kScratchMaxDevices = 64;
+
+}  // namespace
+
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+namespace {
+
+std::mutex g_scratch_mu;
+double *g_scratch_dev[kScratchMaxDevices] = {};
+
+// the current device's copy of `scratch` (looked up once per device)
+int scratch_for_device(int dev, double **out)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (dev < 0 || dev >= kScratchMaxDevices) return fail(TG_ERR_ARG, "device %d out of range", dev);
+    if (!g_scratch_dev[dev]) {
+        void *ptr = nullptr;
+        hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(TG_ERR_HIP, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+        g_scratch_dev[dev] = (double *)ptr;
+    }
+    *out = g_scratch_dev[dev];
+    return TG_OK;
+}
+
+// Stepanov normalisation of the raw bins, as tg_shell_spectrum does on the host
+__global__ void k_spectrum_normalize(const double *raw, int kmax, double kf, double *Sk, int nbins)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nbins) return;
+    double v = 0.0;
+    if (i <= kmax) {
+        const double cnt = raw[kmax + 1 + i];
+        v = cnt <= 0 ? 0.0 : raw[i] * (4 * M_PI) / (cnt * kf);
+    }
+    Sk[i] = v;
+}
+
+}  // namespace
+
+extern "C" int tg_shell_spectrum_noalloc(const tg_config *cfg, const double *A_dev, int n, double *Sk_dev, int nbins,
+                                         void *stream)
+{
+    if (!cfg || !A_dev || !Sk_dev || n < 0) return fail(TG_ERR_ARG, "bad argument");
+    if (cfg->basis != TG_BASIS_FFF) return fail(TG_ERR_ARG, "tg_shell_spectrum_noalloc needs the FFF basis");
+    if (cfg->nx < 1 || cfg->ny < 1 || cfg->nz < 2) return fail(TG_ERR_SHAPE, "bad grid");
+    for (int i = 0; i < 3; i++)
+        if (!(cfg->kfactor[i] > 0)) return fail(TG_ERR_ARG, "kfactor[%d] must be positive", i);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || cfg->device < 0 || cfg->device >= ndev)
+        return fail(TG_ERR_ARG, "device %d out of range", cfg->device);
+    Grid g{};
+    g.nx = cfg->nx;
+    g.ny = cfg->ny;
+    g.nz = cfg->nz;
+    g.nzc = cfg->nz / 2 + 1;
+    g.nzp = g.nzc;
+    for (int i = 0; i < 3; i++) g.kf[i] = cfg->kfactor[i];
+    const int kmax = min_radius_outside(g);
+    if (nbins < kmax + 1) return fail(TG_ERR_ARG, "nbins must be >= %d", kmax + 1);
+    // the same block count as a plan's reductions, so the sums agree bit for bit
+    const size_t nc = (size_t)g.nx * g.ny * g.nzc;
+    const int nblocks = (int)std::max<size_t>(1, std::min<size_t>(2048, (nc + 1023) / 1024));
+    const size_t width = 2 * (size_t)(kmax + 1);
+    const size_t need = (size_t)nblocks * width + width;
+    if (need > kScratchDoubles)
+        return fail(TG_ERR_ARG, "grid %dx%dx%d needs %zu doubles of scratch, above the static %zu (16 MiB)", g.nx,
+                    g.ny, g.nz, need, kScratchDoubles);
+    if (kmax + 1 > 1100) return fail(TG_ERR_ARG, "too many spectrum bins (%d)", kmax + 1);
+    HIPCHK(hipSetDevice(cfg->device));
+    double *buf = nullptr;
+    int rc = scratch_for_device(cfg->device, &buf);
+    if (rc) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    double *partial = buf, *raw = buf + (size_t)nblocks * width;
+    const double *A[1] = {A_dev};
+    HIPCHK(launch_reduce(g, RED_SPECTRUM, A, nullptr, 1, n, nullptr, kmax + 1, 2, kmax, 0.0, 0.0, partial, nblocks,
+                         raw, st));
+    const double kf = g.kf[0] * g.kf[1] * g.kf[2];
+    hipLaunchKernelGGL(k_spectrum_normalize, dim3((nbins + 255) / 256), dim3(256), 0, st, raw, kmax, kf, Sk_dev,
+                       nbins);
+    HIPCHK(hipGetLastError());
+    return TG_OK;
+}
