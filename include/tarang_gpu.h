@@ -38,14 +38,20 @@ extern "C" {
 #define TG_ERR_NAN (-5)    /* a time advance wrote a non-finite value into the state */
 #define TG_ERR_STATE (-6)  /* call out of sequence (adjoint backward before a forward sweep) */
 
-/* Bases: FFF (Fourier x, y, z) and SFF (sin/cos along x, Fourier y, z;
- * SFF_pencil.cc).  For SFF, kfactor[0] = pi / L_x and strong_reality selects
+/* Bases: FFF (Fourier x, y, z), SFF (sin/cos along x, Fourier y, z;
+ * SFF_pencil.cc) and ChFF (Chebyshev along x on the Chebyshev-Gauss points
+ * x_j = cos(pi (j + 1/2) / Nx), Fourier y, z; the basis of the reference's
+ * Chebyshev solvers, time_advance/chebyshev/, whose basis class is not in
+ * the reference).  For SFF, kfactor[0] = pi / L_x and strong_reality selects
  * the reality condition applied after each RK4 step: 0 none, 1 strong, 2 weak
- * (apply_strong/weak_realitycond_alltime_switch, Time_advance.cc:85-93). */
-enum { TG_BASIS_FFF = 0, TG_BASIS_SFF = 1 };
+ * (apply_strong/weak_realitycond_alltime_switch, Time_advance.cc:85-93).
+ * ChFF plans are FP64 with strong_reality 0; kfactor[0] = 2 / L_x scales
+ * d/dx. */
+enum { TG_BASIS_FFF = 0, TG_BASIS_SFF = 1, TG_BASIS_CHFF = 2 };
 /* sincos_code of tg_forward / tg_inverse (global.program.sincostr_switch[0]):
- * TG_SC_FOURIER on FFF plans; TG_SC_SIN ('S') or TG_SC_COS ('C') on SFF plans. */
-enum { TG_SC_FOURIER = 0, TG_SC_SIN = 1, TG_SC_COS = 2 };
+ * TG_SC_FOURIER on FFF plans; TG_SC_SIN ('S') or TG_SC_COS ('C') on SFF plans;
+ * TG_SC_CHEB on ChFF plans (f(x_j) = sum_k a_k T_k(x_j)). */
+enum { TG_SC_FOURIER = 0, TG_SC_SIN = 1, TG_SC_COS = 2, TG_SC_CHEB = 3 };
 enum { TG_PREC_FP64 = 0, TG_PREC_FP32 = 1 };
 /* Dealias modes: reference corner box (FFF_pencil_basic.cc:87-90) or the
  * two-thirds rule |kx|<=Nx/3, |ky|<=Ny/3, kz<=Nz/3. */
@@ -218,6 +224,31 @@ int tg_shell_to_shell(tg_plan *plan, double *const V_dev[3], const double *radii
  * sphere enclosing every mode, e.g. inf), TG_ERR_ARG otherwise.  Dealiases V
  * in place. */
 int tg_energy_transfer(tg_plan *plan, double *const V_dev[3], const double *radii, int n, double *flux, double *S);
+
+/* ---- ChFF (Chebyshev x) path; arrays (Nx, Ny, Nz/2+1), Chebyshev
+ * coefficients along x ---- */
+/* d/dx in Chebyshev space: c_k b_k = b_{k+2} + 2 (k+1) a_{k+1} (the
+ * derivative at the walls, influence_matrix.cc:215-229, is its sum), times
+ * kfactor[0].  A and B distinct. */
+int tg_cheb_xderiv(tg_plan *plan, const double *A_dev, double *B_dev);
+/* Time_advance_incompress::Helmholtz_complex_full_array (helmholtz.cc:94-118)
+ * with Helmholtz_complex / Tridiagonal_solver (:18-64, :155-231): per
+ * (ky, kz) line, u'' - lambda u = -R (quasi-tridiagonal, Thual's algorithm,
+ * even and odd coefficients apart), lambda = Kperp^2 + lambda_supplement,
+ * u(+1) = value_plus1, u(-1) = value_minus1 ({re, im}).  R and U distinct. */
+int tg_helmholtz(tg_plan *plan, const double *R_dev, double *U_dev, double lambda_supplement,
+                 const double value_plus1[2], const double value_minus1[2]);
+/* Nlin_incompress::Compute_nlin(U) on ChFF: N_i = sum_j d_j (u_j u_i), the
+ * products at the Gauss points, x-derivatives by the recurrence (no
+ * dealiasing). */
+int tg_compute_nlin_chff(tg_plan *plan, double *const V_dev[3], double *const nlin_dev[3]);
+/* Time_advance_incompress::BDF1 (BDF1.cc:63-121), unforced, nsteps steps of V
+ * in place, P the pressure: p from the divergence of N with zero wall values,
+ * each component from (D^2 - Kperp^2 - 1/(dt nu)) u = (N_i - u_i/dt +
+ * d_i p)/(-nu) with no-slip walls, the influence-matrix correction of p and
+ * u_x after u_x (Compute_pressure_Ux_chebyshev, influence_matrix.cc:137-180;
+ * homogeneous solutions :74-133, computed once per (nu, dt)). */
+int tg_bdf1_step(tg_plan *plan, double *const V_dev[3], double *P_dev, const tg_fluid *fluid, int nsteps);
 
 /* Anisotropic diagnostics (FFF plans).  dirn is the anisotropy axis
  * (global.field.anisotropy_dirn, 1..3); theta is the angle of K to it, folded

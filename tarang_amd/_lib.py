@@ -30,8 +30,8 @@ class TgFluid(ctypes.Structure):
                 ("force_r_out", ctypes.c_double)]
 
 
-BASIS_FFF, BASIS_SFF = 0, 1
-SC_FOURIER, SC_SIN, SC_COS = 0, 1, 2
+BASIS_FFF, BASIS_SFF, BASIS_CHFF = 0, 1, 2
+SC_FOURIER, SC_SIN, SC_COS, SC_CHEB = 0, 1, 2, 3
 
 
 class TgRbc(ctypes.Structure):
@@ -76,6 +76,10 @@ SIGNATURES = {
     "tg_shell_to_shell": (_I, [_P, _PP, _DP, _I, _DP]),
     "tg_energy_transfer": (_I, [_P, _PP, _DP, _I, _DP, _DP]),
     "tg_ring_spectrum": (_I, [_P, _PP, _I, _I, _I, _DP, _I, _DP]),
+    "tg_cheb_xderiv": (_I, [_P, _P, _P]),
+    "tg_helmholtz": (_I, [_P, _P, _P, _D, _DP, _DP]),
+    "tg_compute_nlin_chff": (_I, [_P, _PP, _PP]),
+    "tg_bdf1_step": (_I, [_P, _PP, _P, ctypes.POINTER(TgFluid), _I]),
     "tg_shell_spectrum_noalloc": (_I, [ctypes.POINTER(TgConfig), _P, _I, _P, _I, _P]),
     "tg_cylindrical_ring_spectrum": (_I, [_P, _PP, _I, _I, _DP, _I, _DP, _DP]),
     "tg_ring_to_ring": (_I, [_P, _PP, _I, _DP, _I, _DP, _I, _DP]),

@@ -24,7 +24,7 @@ import torch
 
 from . import _lib, h5io
 from ._lib import (DEALIAS_NONE, DEALIAS_REF_CORNER, DEALIAS_TWOTHIRDS, PREC_FP32, PREC_FP64, STEP_CONTINUE,
-                   TgConfig, TgFluid, check, ptr_array, BASIS_SFF, SC_COS, SC_SIN, TgRbc)
+                   TgConfig, TgFluid, check, ptr_array, BASIS_CHFF, BASIS_SFF, SC_CHEB, SC_COS, SC_SIN, TgRbc)
 
 __all__ = ["FFF_Basis", "SFF_Basis", "FluidVF", "FluidSF", "Nlin_incompress", "Time_advance_incompress",
            "Time_advance_incompress_MHD", "Time_advance_incompress_RBC", "Adjoint_incompress", "Correlation",
@@ -178,6 +178,55 @@ class SFF_Basis(FFF_Basis):
     def Satisfy_weak_reality_condition_in_Array(self, *arrays):
         check("tg_weak_reality",
               self.L.tg_weak_reality(self.plan, ptr_array([a.data_ptr() for a in arrays]), len(arrays)))
+
+
+class ChFF_Basis(FFF_Basis):
+    """Chebyshev along x (walls at x = +-1 of the Chebyshev variable, kfactor[0]
+    = 2 / L_x), Fourier along y and z: the basis of the reference's Chebyshev
+    solvers (time_advance/chebyshev/).  Chebyshev-Gauss sample points; FP64."""
+
+    basis_code = BASIS_CHFF
+    fourier_x = False
+
+    def __init__(self, N, kfactor=(1.0, 1.0, 1.0), device=0):
+        super().__init__(N, kfactor=kfactor, precision=PREC_FP64, dealias=DEALIAS_NONE, strong_reality=False,
+                         device=device)
+
+    def Forward_transform(self, Ar, A):
+        check("tg_forward", self.L.tg_forward(self.plan, SC_CHEB, _vp(Ar), _vp(A)))
+
+    def Inverse_transform(self, A, Ar):
+        check("tg_inverse", self.L.tg_inverse(self.plan, SC_CHEB, _vp(A), _vp(Ar)))
+
+    def Xderiv(self, A, B):
+        """d/dx in Chebyshev space (B distinct from A)."""
+        check("tg_cheb_xderiv", self.L.tg_cheb_xderiv(self.plan, _vp(A), _vp(B)))
+
+    def Helmholtz(self, R, U, lambda_supplement, value_plus1=0j, value_minus1=0j):
+        """Helmholtz_complex_full_array: u'' - (Kperp^2 + lambda_supplement) u = -R
+        per (ky, kz) line, u(+1) = value_plus1, u(-1) = value_minus1."""
+        vp = (ctypes.c_double * 2)(complex(value_plus1).real, complex(value_plus1).imag)
+        vm = (ctypes.c_double * 2)(complex(value_minus1).real, complex(value_minus1).imag)
+        check("tg_helmholtz", self.L.tg_helmholtz(self.plan, _vp(R), _vp(U), float(lambda_supplement), vp, vm))
+
+
+class Time_advance_incompress_BDF1:
+    """Time_advance_incompress::BDF1 on a ChFF basis (BDF1.cc:63-121): U.cvf
+    advanced in place, the pressure in self.P."""
+
+    def __init__(self, U, dt):
+        self.U = U
+        self.fluid = TgFluid(U.dissipation_coefficient, float(dt), 0, 0.0, 0.0, 0.0)
+        self.P = U.basis.complex_array()
+
+    def Compute_nlin(self):
+        b = self.U.basis
+        check("tg_compute_nlin_chff", b.L.tg_compute_nlin_chff(b.plan, self.U.ptrs(), self.U.ptrs("nlin")))
+
+    def Time_advance_step(self, nsteps=1):
+        b = self.U.basis
+        check("tg_bdf1_step", b.L.tg_bdf1_step(b.plan, self.U.ptrs(), _vp(self.P), ctypes.byref(self.fluid),
+                                               int(nsteps)))
 
 
 class FluidSF:

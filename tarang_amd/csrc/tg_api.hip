@@ -14,6 +14,7 @@
 
 #include "../../include/tarang_gpu.h"
 #include "tg_adjoint.h"
+#include "tg_chff.h"
 #include "tg_diag.h"
 #include "tg_kernels.h"
 #include "tg_mhd.h"
@@ -101,6 +102,16 @@ struct tg_plan {
         double *vw = nullptr, *lg = nullptr, *acc = nullptr;
         bool recorded = false;
     } adj;
+    // ChFF (TG_BASIS_CHFF) buffers, allocated on first use: the weighted copy
+    // of an inverse transform's input, derivative/divergence and product
+    // spectra, the nonlinear term (BDF1 right-hand sides), real velocity and
+    // product arrays, Helmholtz scratch, the homogeneous solutions p+-,
+    // u_x+- and the influence matrix (for the stored nu, dt)
+    struct Chff {
+        double *tmp = nullptr, *D = nullptr, *S = nullptr, *NL[3] = {}, *R[4] = {}, *X = nullptr, *Y = nullptr;
+        double *pp = nullptr, *pm = nullptr, *vxp = nullptr, *vxm = nullptr, *M = nullptr;
+        double infl_nu = -1.0, infl_dt = -1.0;
+    } ch;
     // per-kernel HIP event timing (tg_profile_*)
     bool prof_on = false;
     struct ProfRec { int kid; hipEvent_t a, b; };
@@ -211,6 +222,10 @@ void free_plan(tg_plan *p)
     void *aptrs[] = {p->adj.ckpt, p->adj.tape, p->adj.vw, p->adj.lg, p->adj.acc, p->nonfinite_dev};
     for (void *q : aptrs)
         if (q) (void)hipFree(q);
+    void *cptrs[] = {p->ch.tmp, p->ch.D, p->ch.S, p->ch.NL[0], p->ch.NL[1], p->ch.NL[2], p->ch.R[0], p->ch.R[1],
+                     p->ch.R[2], p->ch.R[3], p->ch.X, p->ch.Y, p->ch.pp, p->ch.pm, p->ch.vxp, p->ch.vxm, p->ch.M};
+    for (void *q : cptrs)
+        if (q) (void)hipFree(q);
     if (p->nonfinite_host) (void)hipHostFree(p->nonfinite_host);
     for (auto &r : p->prof_recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
     for (auto e : p->prof_pool) (void)hipEventDestroy(e);
@@ -294,6 +309,8 @@ int run_forward(tg_plan *p, const double *real_dev, double *spec_dev, int sincos
         const double scale = 1.0 / (2.0 * p->g.nx * (double)p->g.ny * p->g.nz);
         Prof pr(p, TG_KERNEL_X_SC);
         HIPCHK(launch_xfwd_sc<T>(p->g, W, 1, sincos == TG_SC_SIN ? 1 : 0, O, scale, p->twx, p->twq, p->stream));
+        // Chebyshev coefficients: a_0 = A_0, a_k = 2 A_k of the cosine transform
+        if (sincos == TG_SC_CHEB) HIPCHK(launch_cheb_scale(p->g, spec_dev, spec_dev, 2.0, p->stream));
         return TG_OK;
     }
     const double scale = 1.0 / ((double)p->g.nx * p->g.ny * p->g.nz);
@@ -307,6 +324,11 @@ int run_inverse(tg_plan *p, const double *spec_dev, double *real_dev, int sincos
     double *V[1] = {const_cast<double *>(spec_dev)};
     void *W[1] = {p->U[0]};
     Band nb{};
+    if (sincos == TG_SC_CHEB) {
+        // the cosine coefficients A_0 = a_0, A_k = a_k / 2, into a copy (input preserved)
+        HIPCHK(launch_cheb_scale(p->g, spec_dev, p->ch.tmp, 0.5, p->stream));
+        V[0] = p->ch.tmp;
+    }
     if (p->g.basis == 1) {
         Prof pr(p, TG_KERNEL_X_SC);
         HIPCHK(launch_xinv_sc<T>(p->g, V, 1, sincos == TG_SC_SIN ? 1 : 0, W, 0, p->twx, p->twq, p->stream));
@@ -483,8 +505,6 @@ int run_rk4_mhd(tg_plan *p, double *const V[3], double *const W[3], const tg_flu
     int rc = ensure_mhd(p);
     if (rc) return rc;
     double *accW[3] = {p->mhd[0], p->mhd[1], p->mhd[2]};
-    double *Np[3] = {p->mhd[3], p->mhd[4], p->mhd[5]};
-    double *Nm[3] = {p->mhd[6], p->mhd[7], p->mhd[8]};
     double *Zp[3] = {p->mhd[9], p->mhd[10], p->mhd[11]};
     double *Zm[3] = {p->mhd[12], p->mhd[13], p->mhd[14]};
     p->unext_valid = false;
@@ -495,17 +515,34 @@ int run_rk4_mhd(tg_plan *p, double *const V[3], double *const W[3], const tg_flu
         rc = mhd_xinv<T>(p, Zp, Zm);
         if (rc) return rc;
         for (int s = 1; s <= 4; s++) {
-            rc = mhd_nonlinear<T>(p, Np, Nm);
-            if (rc) return rc;
+            { Prof pr(p, TG_KERNEL_Y_FFT); HIPCHK(launch_colfft<T>(p->g, 1, +1, p->U, 6, p->twy, p->stream)); }
             {
-                Prof pr(p, TG_KERNEL_X_NLIN_RK);
-                HIPCHK(launch_mhd_rk(p->g, s, f->dt, f->nu, eta, V, W, p->acc, accW, Np, Nm, Zp, Zm, p->stream,
-                                     s == 4 ? p->nonfinite_dev : nullptr));
+                Prof pr(p, TG_KERNEL_Z_PRODUCTS);
+                HIPCHK(launch_z_products<T>(p->g, p->U, p->P, 1, p->twm, p->twn, p->stream));
             }
-            if (s < 4) {
-                rc = mhd_xinv<T>(p, Zp, Zm);
-                if (rc) return rc;
+            { Prof pr(p, TG_KERNEL_Y_FFT); HIPCHK(launch_colfft<T>(p->g, 1, -1, p->P, 9, p->twy, p->stream)); }
+            // one x pass: both nonlinear terms, the stage for u and b, the next Z+- x-inverse
+            XmhdArgs xa{};
+            for (int i = 0; i < 9; i++) xa.P[i] = p->P[i];
+            for (int c = 0; c < 3; c++) {
+                xa.V[c] = V[c];
+                xa.W[c] = W[c];
+                xa.accV[c] = p->acc[c];
+                xa.accW[c] = accW[c];
             }
+            for (int i = 0; i < 6; i++) xa.U[i] = p->U[i];
+            xa.tw = p->twx;
+            xa.g = p->g;
+            xa.stage = s;
+            xa.dt = f->dt;
+            xa.fh_u = -f->nu * 0.5 * f->dt;
+            xa.f1_u = -f->nu * 1.0 * f->dt;
+            xa.fh_b = -eta * 0.5 * f->dt;
+            xa.f1_b = -eta * 1.0 * f->dt;
+            xa.scale = 1.0 / ((double)p->g.nx * p->g.ny * p->g.nz);
+            xa.nonfinite = s == 4 ? p->nonfinite_dev : nullptr;
+            Prof pr(p, TG_KERNEL_X_NLIN_RK);
+            HIPCHK(launch_x_mhd<T>(xa, p->stream));
         }
         if (p->cfg.strong_reality) {
             Prof pr(p, TG_KERNEL_REALITY);
@@ -956,8 +993,10 @@ int tg_plan_create(const tg_config *cfg, tg_plan **plan)
 {
     if (!cfg || !plan) return fail(TG_ERR_ARG, "null argument");
     *plan = nullptr;
-    if (cfg->basis != TG_BASIS_FFF && cfg->basis != TG_BASIS_SFF)
+    if (cfg->basis != TG_BASIS_FFF && cfg->basis != TG_BASIS_SFF && cfg->basis != TG_BASIS_CHFF)
         return fail(TG_ERR_ARG, "unsupported basis %d", cfg->basis);
+    if (cfg->basis == TG_BASIS_CHFF && (cfg->precision != TG_PREC_FP64 || cfg->strong_reality != 0))
+        return fail(TG_ERR_ARG, "ChFF plans are FP64 with strong_reality 0");
     if (cfg->strong_reality < 0 || cfg->strong_reality > (cfg->basis == TG_BASIS_SFF ? 2 : 1))
         return fail(TG_ERR_ARG, "strong_reality %d out of range for this basis", cfg->strong_reality);
     if (!pow2_in(cfg->nx, 8, 1024) || !pow2_in(cfg->ny, 8, 1024) || !pow2_in(cfg->nz, 16, 1024))
@@ -980,7 +1019,8 @@ int tg_plan_create(const tg_config *cfg, tg_plan **plan)
     p->g.nzc = cfg->nz / 2 + 1;
     for (int i = 0; i < 3; i++) p->g.kf[i] = cfg->kfactor[i];
     p->g.dealias = cfg->dealias;
-    p->g.basis = cfg->basis == TG_BASIS_SFF ? 1 : 0;
+    // ChFF runs its x transforms on the cosine passes of the SFF basis
+    p->g.basis = cfg->basis == TG_BASIS_FFF ? 0 : 1;
     p->prec = cfg->precision;
     p->elem = p->prec == TG_PREC_FP32 ? 8 : 16;
     {
@@ -1064,18 +1104,30 @@ int tg_get_shapes(const tg_plan *plan, int64_t cshape[3], int64_t rshape[3])
 
 int64_t tg_plan_device_bytes(const tg_plan *plan) { return plan ? plan->dev_bytes : 0; }
 
-static int check_sincos(const tg_plan *p, int code)
+static int ensure_chff(tg_plan *p);
+
+static int check_sincos(tg_plan *p, int code)
 {
-    if (p->g.basis == 0 && code != TG_SC_FOURIER) return fail(TG_ERR_ARG, "FFF basis takes sincos_code 0");
-    if (p->g.basis == 1 && code != TG_SC_SIN && code != TG_SC_COS)
+    if (p->cfg.basis == TG_BASIS_FFF && code != TG_SC_FOURIER) return fail(TG_ERR_ARG, "FFF basis takes sincos_code 0");
+    if (p->cfg.basis == TG_BASIS_SFF && code != TG_SC_SIN && code != TG_SC_COS)
         return fail(TG_ERR_ARG, "SFF basis takes sincos_code TG_SC_SIN or TG_SC_COS");
+    if (p->cfg.basis == TG_BASIS_CHFF) {
+        if (code != TG_SC_CHEB) return fail(TG_ERR_ARG, "ChFF basis takes sincos_code TG_SC_CHEB");
+        return ensure_chff(p);
+    }
     return TG_OK;
 }
 
 static int need_basis(const tg_plan *p, int basis, const char *what)
 {
-    if (p->g.basis != basis)
-        return fail(TG_ERR_ARG, "%s needs a plan with basis %s", what, basis ? "TG_BASIS_SFF" : "TG_BASIS_FFF");
+    static const char *names[3] = {"TG_BASIS_FFF", "TG_BASIS_SFF", "TG_BASIS_CHFF"};
+    if (p->cfg.basis != basis) return fail(TG_ERR_ARG, "%s needs a plan with basis %s", what, names[basis]);
+    return TG_OK;
+}
+
+static int not_chff(const tg_plan *p, const char *what)
+{
+    if (p->cfg.basis == TG_BASIS_CHFF) return fail(TG_ERR_ARG, "%s is not defined on ChFF plans", what);
     return TG_OK;
 }
 
@@ -1109,6 +1161,7 @@ int tg_inverse(tg_plan *plan, int sincos_code, const double *spec_dev, double *r
 int tg_dealias(tg_plan *plan, double *const V_dev[], int ncomp)
 {
     if (!plan || ncomp < 1 || ncomp > 3) return fail(TG_ERR_ARG, "bad argument");
+    if (int rb_ = not_chff(plan, "tg_dealias")) return rb_;
     int rc = check_ptrs((const void *const *)V_dev, ncomp, "tg_dealias");
     if (rc) return rc;
     rc = set_device(plan);
@@ -1120,6 +1173,7 @@ int tg_dealias(tg_plan *plan, double *const V_dev[], int ncomp)
 int tg_strong_reality(tg_plan *plan, double *const V_dev[], int ncomp)
 {
     if (!plan || ncomp < 1 || ncomp > 3) return fail(TG_ERR_ARG, "bad argument");
+    if (int rb_ = not_chff(plan, "tg_strong_reality")) return rb_;
     int rc = check_ptrs((const void *const *)V_dev, ncomp, "tg_strong_reality");
     if (rc) return rc;
     rc = set_device(plan);
@@ -1133,6 +1187,7 @@ int tg_strong_reality(tg_plan *plan, double *const V_dev[], int ncomp)
 int tg_weak_reality(tg_plan *plan, double *const V_dev[], int ncomp)
 {
     if (!plan || ncomp < 1 || ncomp > 4) return fail(TG_ERR_ARG, "bad argument");
+    if (int rb_ = not_chff(plan, "tg_weak_reality")) return rb_;
     int rc = check_ptrs((const void *const *)V_dev, ncomp, "tg_weak_reality");
     if (rc) return rc;
     rc = set_device(plan);
@@ -1250,6 +1305,7 @@ int tg_rk4_step(tg_plan *plan, double *const V_dev[3], const tg_fluid *fluid, in
 int tg_energy(tg_plan *plan, const double *A_dev, double *energy)
 {
     if (!plan || !A_dev || !energy) return fail(TG_ERR_ARG, "null argument");
+    if (int rb_ = not_chff(plan, "tg_energy")) return rb_;
     int rc = set_device(plan);
     if (rc) return rc;
     const double *A[1] = {A_dev};
@@ -1663,6 +1719,179 @@ int tg_init_field(tg_plan *plan, int kind, double k0, double kmin, double kmax, 
     }
     HIPCHK(launch_scale_fields(plan->g, V_dev, tgt, plan->stream));
     HIPCHK(hipStreamSynchronize(plan->stream));
+    return TG_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// ChFF: Chebyshev x-derivative, Helmholtz solves, nonlinear term, BDF1.
+// ---------------------------------------------------------------------------
+static int ensure_chff(tg_plan *p)
+{
+    const size_t nc16 = p->nc * 16, nr8 = (size_t)p->g.nx * p->g.ny * (p->g.nz + 2) * 8;
+    const size_t ncol = (size_t)p->g.ny * p->g.nzc;
+    struct { double **q; size_t bytes; } need[] = {
+        {&p->ch.tmp, nc16}, {&p->ch.D, nc16}, {&p->ch.S, nc16}, {&p->ch.NL[0], nc16}, {&p->ch.NL[1], nc16},
+        {&p->ch.NL[2], nc16}, {&p->ch.R[0], nr8}, {&p->ch.R[1], nr8}, {&p->ch.R[2], nr8}, {&p->ch.R[3], nr8},
+        {&p->ch.X, nc16}, {&p->ch.Y, nc16}, {&p->ch.pp, p->nc * 8}, {&p->ch.pm, p->nc * 8},
+        {&p->ch.vxp, p->nc * 8}, {&p->ch.vxm, p->nc * 8}, {&p->ch.M, ncol * 4 * 8}};
+    for (auto &n : need)
+        if (!*n.q) {
+            int rc = dmalloc(p, (void **)n.q, n.bytes);
+            if (rc) return rc;
+        }
+    return TG_OK;
+}
+
+static HelmArgs helm_args(tg_plan *p, const double *R, double *U, double lam_sup, const double vp[2],
+                          const double vm[2], int E, int pressure_switch)
+{
+    HelmArgs h{};
+    h.R = R;
+    h.U = U;
+    h.X = p->ch.X;
+    h.Y = p->ch.Y;
+    h.g = p->g;
+    h.lam_sup = lam_sup;
+    for (int i = 0; i < 2; i++) {
+        h.vp[i] = vp[i];
+        h.vm[i] = vm[i];
+    }
+    h.E = E;
+    h.pressure_switch = pressure_switch;
+    return h;
+}
+
+// N_i = sum_j d_j (u_j u_i), Chebyshev x-derivatives (no dealiasing)
+static int chff_nlin(tg_plan *p, double *const V[3], double *const N[3])
+{
+    for (int c = 0; c < 3; c++) {
+        int rc = run_inverse<double>(p, V[c], p->ch.R[c], TG_SC_CHEB);
+        if (rc) return rc;
+    }
+    static const int pa[6] = {0, 1, 2, 0, 1, 2}, pb[6] = {0, 1, 2, 1, 2, 0};
+    const size_t nr = (size_t)p->g.nx * p->g.ny * (p->g.nz + 2);
+    bool assigned[3] = {false, false, false};
+    for (int q = 0; q < 6; q++) {
+        const int a = pa[q], b = pb[q];
+        HIPCHK(launch_real_product(p->ch.R[a], p->ch.R[b], p->ch.R[3], nr, p->stream));
+        int rc = run_forward<double>(p, p->ch.R[3], p->ch.S, TG_SC_CHEB);
+        if (rc) return rc;
+        for (int t = 0; t < (a == b ? 1 : 2); t++) {
+            const int dst = t == 0 ? a : b, dir = t == 0 ? b : a;
+            if (dir == 0) HIPCHK(launch_cheb_deriv(p->g, p->ch.S, p->ch.D, true, p->stream));
+            HIPCHK(launch_add_deriv(p->g, N[dst], p->ch.S, p->ch.D, dir, !assigned[dst], p->stream));
+            assigned[dst] = true;
+        }
+    }
+    return TG_OK;
+}
+
+// Compute_homgeneous_soln_influence_matrix (influence_matrix.cc:74-133)
+static int chff_influence(tg_plan *p, double nu, double dt)
+{
+    if (p->ch.infl_nu == nu && p->ch.infl_dt == dt) return TG_OK;
+    double *zero = p->ch.R[3];
+    HIPCHK(launch_fill_double(zero, p->nc, 0.0, p->stream));
+    const double one[2] = {1.0, 0.0}, nil[2] = {0.0, 0.0};
+    double *tmp = p->ch.D;  // real scratch (nc doubles)
+    double *hp[2] = {p->ch.pp, p->ch.pm}, *hv[2] = {p->ch.vxp, p->ch.vxm};
+    for (int s = 0; s < 2; s++) {
+        HIPCHK(launch_helmholtz(helm_args(p, zero, hp[s], 0.0, s == 0 ? one : nil, s == 0 ? nil : one, 1, 1),
+                                p->stream));
+        HIPCHK(launch_cheb_deriv(p->g, hp[s], tmp, false, p->stream));
+        HIPCHK(launch_scale_real(tmp, p->nc, -nu, p->stream));
+        HIPCHK(launch_helmholtz(helm_args(p, tmp, hv[s], 1 / (dt * nu), nil, nil, 1, 0), p->stream));
+    }
+    HIPCHK(launch_influence(p->g, p->ch.vxp, p->ch.vxm, p->ch.M, p->stream));
+    p->ch.infl_nu = nu;
+    p->ch.infl_dt = dt;
+    return TG_OK;
+}
+
+extern "C" {
+
+int tg_cheb_xderiv(tg_plan *plan, const double *A_dev, double *B_dev)
+{
+    if (!plan || !A_dev || !B_dev || A_dev == B_dev) return fail(TG_ERR_ARG, "bad argument");
+    if (int rb_ = need_basis(plan, TG_BASIS_CHFF, "tg_cheb_xderiv")) return rb_;
+    int rc = set_device(plan);
+    if (rc) return rc;
+    HIPCHK(launch_cheb_deriv(plan->g, A_dev, B_dev, true, plan->stream));
+    return TG_OK;
+}
+
+int tg_helmholtz(tg_plan *plan, const double *R_dev, double *U_dev, double lambda_supplement,
+                 const double value_plus1[2], const double value_minus1[2])
+{
+    if (!plan || !R_dev || !U_dev || !value_plus1 || !value_minus1 || R_dev == U_dev)
+        return fail(TG_ERR_ARG, "bad argument");
+    if (int rb_ = need_basis(plan, TG_BASIS_CHFF, "tg_helmholtz")) return rb_;
+    if (plan->g.nx < 4) return fail(TG_ERR_SHAPE, "tg_helmholtz needs Nx >= 4");
+    int rc = set_device(plan);
+    if (!rc) rc = ensure_chff(plan);
+    if (rc) return rc;
+    HIPCHK(launch_helmholtz(helm_args(plan, R_dev, U_dev, lambda_supplement, value_plus1, value_minus1, 2, 0),
+                            plan->stream));
+    return TG_OK;
+}
+
+int tg_compute_nlin_chff(tg_plan *plan, double *const V_dev[3], double *const nlin_dev[3])
+{
+    if (!plan) return fail(TG_ERR_ARG, "null plan");
+    if (int rb_ = need_basis(plan, TG_BASIS_CHFF, "tg_compute_nlin_chff")) return rb_;
+    int rc = check_ptrs((const void *const *)V_dev, 3, "V");
+    if (!rc) rc = check_ptrs((const void *const *)nlin_dev, 3, "nlin");
+    if (!rc) rc = set_device(plan);
+    if (!rc) rc = ensure_chff(plan);
+    if (rc) return rc;
+    return chff_nlin(plan, V_dev, nlin_dev);
+}
+
+int tg_bdf1_step(tg_plan *plan, double *const V_dev[3], double *P_dev, const tg_fluid *fluid, int nsteps)
+{
+    if (!plan || !P_dev || !fluid || nsteps < 0) return fail(TG_ERR_ARG, "bad argument");
+    if (int rb_ = need_basis(plan, TG_BASIS_CHFF, "tg_bdf1_step")) return rb_;
+    if (!(fluid->dt > 0) || !(fluid->nu > 0)) return fail(TG_ERR_ARG, "need dt > 0 and nu > 0");
+    if (fluid->force_kind != 0) return fail(TG_ERR_ARG, "forcing is not supported for ChFF");
+    if (plan->g.nx < 4) return fail(TG_ERR_SHAPE, "tg_bdf1_step needs Nx >= 4");
+    int rc = check_ptrs((const void *const *)V_dev, 3, "V");
+    if (!rc) rc = set_device(plan);
+    if (!rc) rc = ensure_chff(plan);
+    if (!rc) rc = chff_influence(plan, fluid->nu, fluid->dt);
+    if (rc) return rc;
+    const double nu = fluid->nu, dt = fluid->dt, nil[2] = {0.0, 0.0};
+    tg_plan::Chff &c = plan->ch;
+    for (int step = 0; step < nsteps; step++) {
+        rc = chff_nlin(plan, V_dev, c.NL);
+        if (rc) return rc;
+        // div N, then the pressure with zero wall values (BDF1.cc:69-75)
+        HIPCHK(launch_cheb_deriv(plan->g, c.NL[0], c.D, true, plan->stream));
+        HIPCHK(launch_add_deriv(plan->g, c.D, c.NL[1], nullptr, 1, false, plan->stream));
+        HIPCHK(launch_add_deriv(plan->g, c.D, c.NL[2], nullptr, 2, false, plan->stream));
+        HIPCHK(launch_helmholtz(helm_args(plan, c.D, P_dev, 0.0, nil, nil, 2, 0), plan->stream));
+        for (int k = 0; k < 3; k++) {
+            if (k == 0) HIPCHK(launch_cheb_deriv(plan->g, P_dev, c.D, true, plan->stream));
+            HIPCHK(launch_bdf1_rhs(plan->g, c.NL[k], V_dev[k], P_dev, c.D, k, dt, nu, plan->stream));
+            HIPCHK(launch_helmholtz(helm_args(plan, c.NL[k], V_dev[k], 1 / (dt * nu), nil, nil, 2, 0), plan->stream));
+            if (k == 0) {
+                InflArgs ia{};
+                ia.P = P_dev;
+                ia.V1 = V_dev[0];
+                ia.pp = c.pp;
+                ia.pm = c.pm;
+                ia.vxp = c.vxp;
+                ia.vxm = c.vxm;
+                ia.M = c.M;
+                ia.nx = plan->g.nx;
+                ia.ny = plan->g.ny;
+                ia.nzc = plan->g.nzc;
+                ia.kf = plan->g.kf[0];
+                HIPCHK(launch_influence_correct(ia, plan->stream));
+            }
+        }
+    }
     return TG_OK;
 }
 

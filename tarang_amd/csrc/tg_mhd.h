@@ -14,4 +14,24 @@ hipError_t launch_mhd_rk(const Grid &g, int stage, double dt, double nu, double 
                          const double *const *Nm, double *const *Zp, double *const *Zm, hipStream_t st,
                          int *nonfinite = nullptr);
 
+// Fused x pass of an MHD RK stage (1..4): P[3i+j] = FT[Z-_j Z+_i] (work
+// layout), state V (u), W (b) and accumulators accV, accW in the FP64
+// boundary layout; stages 1-3 write the x-inverse of the next stage's Z- into
+// U[0..2] and Z+ into U[3..5].
+struct XmhdArgs {
+    void *P[9];
+    double *V[3], *W[3];
+    double *accV[3], *accW[3];
+    void *U[6];
+    const void *tw;
+    Grid g;
+    int stage;
+    double dt, fh_u, f1_u, fh_b, f1_b;  // fh = -nu dt / 2, f1 = -nu dt (u: nu, b: eta)
+    double scale;                       // 1 / (Nx Ny Nz)
+    int *nonfinite;                     // stage 4: set to 1 on a non-finite state value
+};
+
+template <typename T>
+hipError_t launch_x_mhd(const XmhdArgs &a, hipStream_t st);
+
 }  // namespace tg

@@ -10,8 +10,15 @@
 //                   RHS_u = -P N_u, RHS_b = -N_b (Tadvance_functions.cc:
 //                   262-270), the Lawson RK4 stage for u (nu) and b (eta)
 //                   (RK_VF_MHD.cc:126-190), and the next stage's Z+-.
+//   k_x_mhd         the fused x pass of an RK stage: x-forward of the nine
+//                   products, N+ and N- together, the combination, pressure
+//                   and Lawson stage above for u and b, and the x-inverse of
+//                   the next stage's Z+- (one read of the products per stage)
+#include <type_traits>
+
 #include "tg_device.h"
 #include "tg_kernels.h"
+#include "tg_pass.h"
 #include "tg_mhd.h"
 
 namespace tg {
@@ -151,6 +158,206 @@ __global__ void k_mhd_combine(MhdArgs a)
             *mm = 0.5 * (p - m);
         }
 }
+
+// ---------------------------------------------------------------------------
+// Fused MHD x pass (stages 1-4).  Tile staging as the NS x pass: CB columns
+// of the padded (ly, lz) space by all Nx, the products arriving by
+// direct-to-LDS loads into tile B while the previous one is transformed.
+// ---------------------------------------------------------------------------
+__constant__ int kMhdOrder[9] = {0, 4, 8, 3, 1, 7, 5, 2, 6};  // giver-form product order
+
+template <typename T, int L, int CB>
+__global__ void __launch_bounds__(CB *(L / 8), 1) k_x_mhd(XmhdArgs a)
+{
+    constexpr int TPL = L / 8;
+    constexpr int LP = lpad_len(L);
+    __shared__ __attribute__((aligned(16))) char rawA[CB * LP * sizeof(cpx<double>)];
+    __shared__ __attribute__((aligned(16))) char rawB[CB * LP * sizeof(cpx<double>)];
+    __shared__ int colw[CB], cols[CB];
+    __shared__ cpx<T> twl[L];
+    cpx<double> *ldd = (cpx<double> *)rawA;
+    cpx<T> *ldt = (cpx<T> *)rawA;
+    cpx<double> *ldb = (cpx<double> *)rawB;
+    const Grid &g = a.g;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    if (tile >= x_tiles(g, CB, false)) return;
+    tile_columns<CB>(colw, cols, g, false, tile);
+    lds_barrier();
+    const int c = threadIdx.x / TPL, j = threadIdx.x % TPL;
+    const int cw = colw[c];
+    const int ly = cw >= 0 ? cw / g.nzp : 0, lz = cw >= 0 ? cw % g.nzp : 0;
+    const double Ky = kwave(ly, g.ny) * g.kf[1], Kz = lz * g.kf[2];
+    for (int i = threadIdx.x; i < L; i += CB * TPL) twl[i] = ((const cpx<T> *)a.tw)[i];
+    const cpx<T> *tw = twl;
+    const int ncw = g.ny * g.nzp, ncs = g.ny * g.nzc;
+    const T sc = (T)a.scale;
+    const T Kyt = (T)Ky, Kzt = (T)Kz;
+
+    // --- x-forward of the products P[3i+j] = FT[Z-_j Z+_i]: N+_i += i K_j q
+    // (giver form), N-_j += i K_i q (transposed), branch-free over the
+    // components (a zero K adds a signed zero) ---
+    cpx<T> Np[3][8], Nm[3][8];
+#pragma unroll
+    for (int s = 0; s < 8; s++) Np[0][s] = Np[1][s] = Np[2][s] = Nm[0][s] = Nm[1][s] = Nm[2][s] = mk<T>(0, 0);
+    constexpr bool DMA = tile_dma_ok<T, L, CB>();
+    const int cw0 = tile * CB;
+    if constexpr (DMA) dma_tile<T, L, CB>((const cpx<T> *)a.P[kMhdOrder[0]], cw0, ncw, rawB);
+#pragma unroll 1
+    for (int pi = 0; pi < 9; pi++) {
+        const int p = kMhdOrder[pi];
+        const int pii = p / 3, pjj = p % 3;
+        int jj = j;
+        asm volatile("" : "+v"(jj));
+        cpx<T> v[8];
+        if constexpr (DMA) {
+            dma_wait();
+            lds_barrier();
+#pragma unroll
+            for (int s = 0; s < 8; s++) v[s] = dma_tile_read<T, L, CB>(rawB, jj + TPL * s, c);
+            lds_barrier();
+            if (pi + 1 < 9) dma_tile<T, L, CB>((const cpx<T> *)a.P[kMhdOrder[pi + 1]], cw0, ncw, rawB);
+        } else {
+            stage_in<L, CB>(ldt, (const cpx<T> *)a.P[p], colw, ncw);
+            lds_barrier();
+#pragma unroll
+            for (int s = 0; s < 8; s++) v[s] = ldt[c * LP + lpad(j + TPL * s)];
+            lds_barrier();
+        }
+        fft_line<L, -1>(v, ldt + c * LP, tw, jj);
+#pragma unroll
+        for (int s = 0; s < 8; s++) {
+            const T Kx = (T)(kwave(jj + TPL * s, g.nx) * g.kf[0]);
+            const cpx<T> q = sc * v[s];
+            const T Kj = pjj == 0 ? Kx : (pjj == 1 ? Kyt : Kzt);
+            const T Ki = pii == 0 ? Kx : (pii == 1 ? Kyt : Kzt);
+            Np[0][s] = Np[0][s] + mul_i(pii == 0 ? Kj : (T)0, q);
+            Np[1][s] = Np[1][s] + mul_i(pii == 1 ? Kj : (T)0, q);
+            Np[2][s] = Np[2][s] + mul_i(pii == 2 ? Kj : (T)0, q);
+            Nm[0][s] = Nm[0][s] + mul_i(pjj == 0 ? Ki : (T)0, q);
+            Nm[1][s] = Nm[1][s] + mul_i(pjj == 1 ? Ki : (T)0, q);
+            Nm[2][s] = Nm[2][s] + mul_i(pjj == 2 ? Ki : (T)0, q);
+        }
+        if constexpr (!DMA) lds_barrier();
+    }
+    if constexpr (DMA) lds_barrier_async();
+
+    // --- N_u = (N+ + N-)/2, N_b = (N+ - N-)/2, pressure on N_u, right-hand
+    // sides RHS_u = -(N_u + i K p), RHS_b = -N_b (in place: Np <- RHS_u,
+    // Nm <- RHS_b) ---
+#pragma unroll
+    for (int s = 0; s < 8; s++) {
+        const int lx = j + TPL * s;
+        const double K0 = kwave(lx, g.nx) * g.kf[0];
+        cpx<double> nu[3], nb[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const cpx<double> pp = cvt<double>(Np[k][s]), mm = cvt<double>(Nm[k][s]);
+            nu[k] = 0.5 * (pp + mm);
+            nb[k] = 0.5 * (pp - mm);
+        }
+        const double kxs = K0 * K0;
+        const double ks = (kxs + Ky * Ky) + Kz * Kz;
+        cpx<double> div = mul_i(K0, nu[0]);
+        div = div + mul_i(Ky, nu[1]);
+        div = div + mul_i(Kz, nu[2]);
+        cpx<double> pr = mk<double>(0, 0);
+        if (!(lx == 0 && ly == 0 && lz == 0)) pr = mk<double>(div.x / ks, div.y / ks);
+        const double Kc[3] = {K0, Ky, Kz};
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const cpx<double> ru = nu[k] + mul_i(Kc[k], pr);
+            Np[k][s] = cvt<T>(mk<double>(-ru.x, -ru.y));
+            Nm[k][s] = cvt<T>(mk<double>(-nb[k].x, -nb[k].y));
+        }
+    }
+
+    // --- Lawson stage for u (nu) and b (eta), next Z+- (stages 1-3) ---
+    const int stage = a.stage;
+    bool bad = false;
+    auto comp = [&](auto kc) {
+        constexpr int K = decltype(kc)::value;
+        cpx<double> un[8];
+        // u
+        stage_in<L, CB>(ldd, (const cpx<double> *)a.V[K], cols, ncs);
+        if (stage >= 2) stage_in<L, CB>(ldb, (const cpx<double> *)a.accV[K], cols, ncs);
+        lds_barrier();
+#pragma unroll
+        for (int s = 0; s < 8; s++) {
+            const int lx = j + TPL * s;
+            const int li = c * LP + lpad(lx);
+            const double K0 = kwave(lx, g.nx) * g.kf[0];
+            const double ks = (K0 * K0 + Ky * Ky) + Kz * Kz;
+            cpx<double> v0 = ldd[li];
+            if (stage == 1 && dealias_zero(g, lx, ly, lz)) v0 = mk<double>(0, 0);
+            cpx<double> ac = stage >= 2 ? ldb[li] : mk<double>(0, 0), vn;
+            lawson(stage, v0, cvt<double>(Np[K][s]), exp(a.fh_u * ks), exp(a.f1_u * ks), a.dt, ac, vn);
+            ldb[li] = mk<double>(stage == 4 ? vn.x : ac.x, stage == 4 ? vn.y : ac.y);
+            if (stage == 4 && !(isfinite(vn.x) && isfinite(vn.y))) bad = true;
+            un[s] = vn;
+        }
+        lds_barrier();
+        stage_out<L, CB>(ldb, (cpx<double> *)(stage == 4 ? a.V[K] : a.accV[K]), cols, ncs);
+        lds_barrier();
+        // b
+        stage_in<L, CB>(ldd, (const cpx<double> *)a.W[K], cols, ncs);
+        if (stage >= 2) stage_in<L, CB>(ldb, (const cpx<double> *)a.accW[K], cols, ncs);
+        lds_barrier();
+        cpx<T> zp[8], zm[8];
+#pragma unroll
+        for (int s = 0; s < 8; s++) {
+            const int lx = j + TPL * s;
+            const int li = c * LP + lpad(lx);
+            const double K0 = kwave(lx, g.nx) * g.kf[0];
+            const double ks = (K0 * K0 + Ky * Ky) + Kz * Kz;
+            const bool z = dealias_zero(g, lx, ly, lz);
+            cpx<double> w0 = ldd[li];
+            if (stage == 1 && z) w0 = mk<double>(0, 0);
+            cpx<double> ac = stage >= 2 ? ldb[li] : mk<double>(0, 0), bn;
+            lawson(stage, w0, cvt<double>(Nm[K][s]), exp(a.fh_b * ks), exp(a.f1_b * ks), a.dt, ac, bn);
+            ldb[li] = mk<double>(stage == 4 ? bn.x : ac.x, stage == 4 ? bn.y : ac.y);
+            if (stage == 4 && !(isfinite(bn.x) && isfinite(bn.y))) bad = true;
+            cpx<double> u = un[s];
+            if (z) u = bn = mk<double>(0, 0);
+            zp[s] = cvt<T>(u + bn);
+            zm[s] = cvt<T>(u - bn);
+        }
+        lds_barrier();
+        stage_out<L, CB>(ldb, (cpx<double> *)(stage == 4 ? a.W[K] : a.accW[K]), cols, ncs);
+        lds_barrier();
+        if (stage == 4) return;  // the next step starts from the state (after its reality condition)
+        // x-inverse of the next stage input: Z+ into U[3+K], Z- into U[K]
+        fft_line<L, +1>(zp, ldt + c * LP, tw, j);
+#pragma unroll
+        for (int s = 0; s < 8; s++) ldt[c * LP + lpad(j + TPL * s)] = zp[s];
+        lds_barrier();
+        stage_out<L, CB>(ldt, (cpx<T> *)a.U[3 + K], colw, ncw);
+        lds_barrier();
+        fft_line<L, +1>(zm, ldt + c * LP, tw, j);
+#pragma unroll
+        for (int s = 0; s < 8; s++) ldt[c * LP + lpad(j + TPL * s)] = zm[s];
+        lds_barrier();
+        stage_out<L, CB>(ldt, (cpx<T> *)a.U[K], colw, ncw);
+        lds_barrier();
+    };
+    comp(std::integral_constant<int, 0>{});
+    comp(std::integral_constant<int, 1>{});
+    comp(std::integral_constant<int, 2>{});
+    if (bad && a.nonfinite) *a.nonfinite = 1;
+}
+
+template <typename T>
+hipError_t launch_x_mhd(const XmhdArgs &a, hipStream_t st)
+{
+    const Grid &g = a.g;
+    TG_DISPATCH_L(g.nx, {
+        constexpr int CB = col_block<T>(LL, 2 * sizeof(cpx<double>));
+        hipLaunchKernelGGL((k_x_mhd<T, LL, CB>), dim3(x_grid(x_tiles(g, CB, false))), dim3(CB * (LL / 8)), 0, st, a);
+    });
+    return hipGetLastError();
+}
+
+template hipError_t launch_x_mhd<double>(const XmhdArgs &, hipStream_t);
+template hipError_t launch_x_mhd<float>(const XmhdArgs &, hipStream_t);
 
 static MhdArgs mhd_args(const Grid &g, double *const *V, double *const *W)
 {

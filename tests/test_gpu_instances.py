@@ -195,3 +195,28 @@ def test_energy_transfer_radii_contract():
         with pytest.raises(TarangGpuError) as ei:
             api.EnergyTr(b, bad, bad).Compute_flux_and_shell_tr(U)
         assert ei.value.code == ERR_ARG
+
+
+@pytest.mark.parametrize("N", [(512, 8, 16), (16, 8, 512)])
+@pytest.mark.parametrize("prec", [api.PREC_FP64, api.PREC_FP32])
+def test_mhd_timed_instances(N, prec):
+    """The fused MHD x pass at 512-point x lines and the MHD z pass at
+    512-point z lines: two RK4 steps against the oracle."""
+    log(f"test_mhd_timed_instances N={N} prec={prec}")
+    tol = TOL[prec]
+    o = to.Oracle(N, dealias=to.DEALIAS_REF_CORNER, strong_reality=True)
+    V = o.init_field(kind=1, k0=4.0, kmin=1.0, kmax=max(N) / 3, energy=0.5, seed=33)
+    W = o.init_field(kind=1, k0=4.0, kmin=1.0, kmax=max(N) / 3, energy=0.25, seed=34)
+    nu, eta, dt = 5e-3, 3e-3, 1e-3
+    b = api.FFF_Basis(N, precision=prec, dealias=to.DEALIAS_REF_CORNER, strong_reality=True)
+    U, B = api.FluidVF(b, nu), api.FluidVF(b, eta)
+    U.set_field([dev(v) for v in V])
+    B.set_field([dev(w) for w in W])
+    ta = api.Time_advance_incompress_MHD(U, B, dt)
+    fl = to.Fluid(nu, dt, 0, 0, 0, 0)
+    for s in range(2):
+        o.rk4_step_mhd(fl, eta, V, W)
+        ta.Time_advance_step(1)
+        e = rel_l2_vec([host(v) for v in U.cvf] + [host(w) for w in B.cvf], V + W)
+        log(f"  MHD step {s + 1}: rel L2 {e:.3e}")
+        assert e < tol
