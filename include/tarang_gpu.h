@@ -60,7 +60,9 @@ enum { TG_STEP_CONTINUE = 1 };
 
 typedef struct tg_config {
     int basis;          /* TG_BASIS_FFF or TG_BASIS_SFF */
-    int nx, ny, nz;     /* grid; powers of two, nx,ny in [8,1024], nz in [16,1024] */
+    int nx, ny, nz;     /* grid: nx, ny in [8,1024], nz/2 in [8,512], each 2^a, or in FP64
+                           also 3*2^a, 5*2^a, 15*2^a (multiples of 8: 24, 40, 48, ..., 960);
+                           other grids return TG_ERR_SHAPE */
     double kfactor[3];  /* 2 pi / L per axis (global.field.kfactor[1..3]) */
     int precision;      /* TG_PREC_FP64 or TG_PREC_FP32 (transform/product precision) */
     int dealias;        /* TG_DEALIAS_* (global.program.alias_option) */

@@ -22,6 +22,12 @@ CXX = os.environ.get("CXX", "g++")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HDF5 = os.environ.get("TG_HDF5_ROOT", "/opt/conda")
 SOURCES = ["tg_kernels.hip", "tg_diag.hip", "tg_mhd.hip", "tg_adjoint.hip", "tg_sff.hip", "tg_chff.hip", "tg_api.hip"]
+# Sources whose kernels are templated on the transform length: compiled once
+# more per length unit (TG_LEN_UNIT 1: power-of-two FP64, 2: power-of-two
+# FP32, 3 and 4: radix-3/5 lengths FP64), so that the instances compile in
+# parallel; the plain compilation holds their launchers.
+LENGTH_SOURCES = ["tg_kernels.hip", "tg_mhd.hip", "tg_adjoint.hip", "tg_sff.hip"]
+LENGTH_UNITS = (1, 2, 3, 4)
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("TG_OFFLOAD_ARCH", "gfx950")
 FLAGS = [
@@ -32,9 +38,11 @@ FLAGS = [
 ]
 
 
-def _compile(src):
-    out = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
-    cmd = [HIPCC] + FLAGS + ["-c", os.path.join(CSRC, src), "-o", out]
+def _compile(job):
+    src, unit = job
+    stem = os.path.splitext(src)[0] + (f"_u{unit}" if unit else "")
+    out = os.path.join(OBJ, stem + ".o")
+    cmd = [HIPCC] + FLAGS + ([f"-DTG_LEN_UNIT={unit}"] if unit else []) + ["-c", os.path.join(CSRC, src), "-o", out]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{r.stdout}\n{r.stderr}")
@@ -43,8 +51,10 @@ def _compile(src):
 
 def build(verbose=False):
     os.makedirs(OBJ, exist_ok=True)
-    with concurrent.futures.ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
-        objs = list(ex.map(_compile, SOURCES))
+    jobs = [(src, u) for src in LENGTH_SOURCES for u in LENGTH_UNITS] + [(src, 0) for src in SOURCES]
+    # largest first; one hipcc per host core (each takes 1-3 GB)
+    with concurrent.futures.ThreadPoolExecutor(max_workers=max(1, min(len(jobs), os.cpu_count() or 8))) as ex:
+        objs = list(ex.map(_compile, jobs))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs
     r = subprocess.run(cmd, capture_output=True, text=True)

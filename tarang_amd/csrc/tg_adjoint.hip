@@ -283,6 +283,42 @@ __global__ void __launch_bounds__(CB *(L / 8)) k_xfwd_adj(XadjArgs a)
         xsym_tile<T, L, CB>(g, raw, colw, tw, a.S, [&](int k) { stage_put<L, CB>(ldd, gn[k], cols); });
 }
 
+template <typename T, int LL>
+hipError_t xinv_sym_len(const XsymArgs &a, hipStream_t st)
+{
+    constexpr int CB = col_block<T>(LL, sizeof(cpx<double>));
+    hipLaunchKernelGGL((k_xinv_sym<T, LL, CB>), dim3(x_grid(x_tiles(a.g, CB, false))), dim3(CB * (LL / 8)), 0, st, a);
+    return hipGetLastError();
+}
+
+template <typename T, int MM>
+hipError_t z_adj_len(const ZArgs &a, hipStream_t st)
+{
+    constexpr int LPB = z_lines_per_block<MM>();
+    const size_t nlines = (size_t)a.g.nx * a.g.ny;
+    dim3 grid((unsigned)((nlines + LPB - 1) / LPB));
+    hipLaunchKernelGGL((k_z_adj<T, MM>), grid, dim3(LPB * (MM / 8)), 0, st, a);
+    return hipGetLastError();
+}
+
+template <typename T, int LL>
+hipError_t xfwd_adj_len(const XadjArgs &a, bool fuse, hipStream_t st)
+{
+    constexpr int CB = col_block<T>(LL, sizeof(cpx<double>));
+    const dim3 grid(x_grid(x_tiles(a.g, CB, false))), block(CB * (LL / 8));
+    if (fuse) hipLaunchKernelGGL((k_xfwd_adj<T, LL, CB, true>), grid, block, 0, st, a);
+    else hipLaunchKernelGGL((k_xfwd_adj<T, LL, CB, false>), grid, block, 0, st, a);
+    return hipGetLastError();
+}
+
+#define TG_ADJ_L(N, T, P)                                                     \
+    P template hipError_t xinv_sym_len<T, N>(const XsymArgs &, hipStream_t); \
+    P template hipError_t xfwd_adj_len<T, N>(const XadjArgs &, bool, hipStream_t);
+#define TG_ADJ_M(N, T, P) P template hipError_t z_adj_len<T, N>(const ZArgs &, hipStream_t);
+
+#if TG_LEN_UNIT == 0
+TG_ALL_UNITS(TG_ADJ_L, TG_ADJ_M, extern)
+
 template <typename T>
 hipError_t launch_xinv_sym(const Grid &g, double *const *G, void *const *S, const void *twx, hipStream_t st)
 {
@@ -291,12 +327,8 @@ hipError_t launch_xinv_sym(const Grid &g, double *const *G, void *const *S, cons
     for (int i = 0; i < 6; i++) a.S[i] = S[i];
     a.tw = twx;
     a.g = g;
-    TG_DISPATCH_L(g.nx, {
-        constexpr int CB = col_block<T>(LL, sizeof(cpx<double>));
-        hipLaunchKernelGGL((k_xinv_sym<T, LL, CB>), dim3(x_grid(x_tiles(g, CB, false))), dim3(CB * (LL / 8)), 0, st,
-                           a);
-    });
-    return hipGetLastError();
+    TG_DISPATCH_L(T, g.nx, return (xinv_sym_len<T, LL>)(a, st));
+    return hipErrorInvalidValue;
 }
 
 template <typename T>
@@ -312,13 +344,8 @@ hipError_t launch_z_adj(const Grid &g, void *const *S, void *const *Q, void *con
     a.twm = twm;
     a.twn = twn;
     a.g = g;
-    const size_t nlines = (size_t)g.nx * g.ny;
-    TG_DISPATCH_M(g.nz / 2, {
-        constexpr int LPB = z_lines_per_block<MM>();
-        dim3 grid((unsigned)((nlines + LPB - 1) / LPB));
-        hipLaunchKernelGGL((k_z_adj<T, MM>), grid, dim3(LPB * (MM / 8)), 0, st, a);
-    });
-    return hipGetLastError();
+    TG_DISPATCH_M(T, g.nz / 2, return (z_adj_len<T, MM>)(a, st));
+    return hipErrorInvalidValue;
 }
 
 template <typename T>
@@ -338,16 +365,8 @@ hipError_t launch_xfwd_adj(const Grid &g, const AdjParams &ap, void *const *Q, d
     a.tw = twx;
     a.g = g;
     a.ap = ap;
-    TG_DISPATCH_L(g.nx, {
-        constexpr int CB = col_block<T>(LL, sizeof(cpx<double>));
-        if (S_next)
-            hipLaunchKernelGGL((k_xfwd_adj<T, LL, CB, true>), dim3(x_grid(x_tiles(g, CB, false))),
-                               dim3(CB * (LL / 8)), 0, st, a);
-        else
-            hipLaunchKernelGGL((k_xfwd_adj<T, LL, CB, false>), dim3(x_grid(x_tiles(g, CB, false))),
-                               dim3(CB * (LL / 8)), 0, st, a);
-    });
-    return hipGetLastError();
+    TG_DISPATCH_L(T, g.nx, return (xfwd_adj_len<T, LL>)(a, S_next != nullptr, st));
+    return hipErrorInvalidValue;
 }
 
 #define TG_ADJ_INST(T)                                                                                           \
@@ -360,5 +379,8 @@ hipError_t launch_xfwd_adj(const Grid &g, const AdjParams &ap, void *const *Q, d
 
 TG_ADJ_INST(double)
 TG_ADJ_INST(float)
+#else
+TG_THIS_UNIT(TG_ADJ_L, TG_ADJ_M)
+#endif  // TG_LEN_UNIT
 
 }  // namespace tg

@@ -18,6 +18,7 @@
 #include "tg_diag.h"
 #include "tg_kernels.h"
 #include "tg_mhd.h"
+#include "tg_pass.h"
 #include "tg_sff.h"
 
 using namespace tg;
@@ -44,7 +45,6 @@ int fail(int code, const char *fmt, ...)
                                           __FILE__, __LINE__);                                         \
     } while (0)
 
-bool pow2_in(int n, int lo, int hi) { return n >= lo && n <= hi && (n & (n - 1)) == 0; }
 
 }  // namespace
 
@@ -999,11 +999,20 @@ int tg_plan_create(const tg_config *cfg, tg_plan **plan)
         return fail(TG_ERR_ARG, "ChFF plans are FP64 with strong_reality 0");
     if (cfg->strong_reality < 0 || cfg->strong_reality > (cfg->basis == TG_BASIS_SFF ? 2 : 1))
         return fail(TG_ERR_ARG, "strong_reality %d out of range for this basis", cfg->strong_reality);
-    if (!pow2_in(cfg->nx, 8, 1024) || !pow2_in(cfg->ny, 8, 1024) || !pow2_in(cfg->nz, 16, 1024))
-        return fail(TG_ERR_SHAPE, "grid %dx%dx%d unsupported (powers of two, nx,ny in [8,1024], nz in [16,1024])",
-                    cfg->nx, cfg->ny, cfg->nz);
     if (cfg->precision != TG_PREC_FP64 && cfg->precision != TG_PREC_FP32)
         return fail(TG_ERR_ARG, "unknown precision %d", cfg->precision);
+    {
+        // transform lengths with compiled kernels: Nx, Ny in 8..1024 and
+        // Nz/2 in 8..512 of the form 2^a, and in FP64 also 3, 5 or 15 times
+        // a power of two (multiples of 8: 24, 40, 48, ..., 960)
+        const bool fp32 = cfg->precision == TG_PREC_FP32;
+        if (!l_compiled(cfg->nx, fp32) || !l_compiled(cfg->ny, fp32) || cfg->nz % 2 != 0 ||
+            !m_compiled(cfg->nz / 2, fp32))
+            return fail(TG_ERR_SHAPE,
+                        "grid %dx%dx%d unsupported: Nx, Ny in [8, 1024] and Nz/2 in [8, 512], each 2^a (FP32), or "
+                        "2^a, 3*2^a, 5*2^a or 15*2^a and a multiple of 8 (FP64)",
+                        cfg->nx, cfg->ny, cfg->nz);
+    }
     if (cfg->dealias < 0 || cfg->dealias > 2) return fail(TG_ERR_ARG, "unknown dealias mode %d", cfg->dealias);
     for (int i = 0; i < 3; i++)
         if (!(cfg->kfactor[i] > 0)) return fail(TG_ERR_ARG, "kfactor[%d] must be positive", i);

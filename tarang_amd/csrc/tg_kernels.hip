@@ -379,11 +379,11 @@ __device__ double band_alpha(const Grid &g, const Band &b, const double *slots)
 // Order in which the product spectra are folded into N, following the
 // reference's diag/offdiag accumulation (compute_diag_offdiag.cc:47-112 for
 // NS, :329-414 for the giver form with P[3i+j] = FT[u_j w_i]).
-__constant__ int kNsOrder[6] = {0, 1, 2, 3, 4, 5};
+static __constant__ int kNsOrder[6] = {0, 1, 2, 3, 4, 5};
 // NS products xx, yy, zz, xy, yz, zx: derivative applied for N_0, N_1, N_2
 // (Compute_nlin_diag, then Compute_nlin_offdiag's Add_*deriv calls).
-__constant__ int kNsSel[6][3] = {{1, 0, 0}, {0, 2, 0}, {0, 0, 3}, {2, 1, 0}, {0, 3, 2}, {3, 0, 1}};
-__constant__ int kGiverOrder[9] = {0, 4, 8, 3, 1, 7, 5, 2, 6};
+static __constant__ int kNsSel[6][3] = {{1, 0, 0}, {0, 2, 0}, {0, 0, 3}, {2, 1, 0}, {0, 3, 2}, {3, 0, 1}};
+static __constant__ int kGiverOrder[9] = {0, 4, 8, 3, 1, 7, 5, 2, 6};
 
 template <typename T, int L, int CB, int MODE>
 __global__ void __launch_bounds__(CB *(L / 8), 2) k_x_nlin(XnlArgs a)
@@ -708,6 +708,7 @@ __global__ void __launch_bounds__(CB *(L / 8), 2) k_x_nlin(XnlArgs a)
 }
 #undef TG_TAKE
 
+#if TG_LEN_UNIT == 0
 // ---------------------------------------------------------------------------
 // Strong reality condition (FFF_pencil_basic.cc:139-179) and dealias.
 // ---------------------------------------------------------------------------
@@ -754,9 +755,111 @@ __global__ void k_dealias(PlaneArgs a)
     }
 }
 
+#endif  // TG_LEN_UNIT == 0
+
 // ---------------------------------------------------------------------------
-// Launch helpers with size dispatch.
+// Per-length launch functions (instantiated in the length units) and the
+// launchers with their size dispatch.
 // ---------------------------------------------------------------------------
+template <typename T, int LL>
+hipError_t colfft_len(const ColArgs &a, int dir, int nfields, hipStream_t st)
+{
+    // at most 512 threads a block, except FP32 lines of 512 points and
+    // more: 1024-thread blocks (16 columns at 512 points, 8 at 1024) run
+    // 2% and 9% faster than the 512-thread tiles of half the width
+    constexpr int CB = std::min(col_block<T>(LL, sizeof(cpx<T>)),
+                                std::max(1, (sizeof(T) == 4 && LL >= 512 ? 1024 : 512) / (LL / 8)));
+    dim3 grid(x_grid(a.nouter * ((a.ncol + CB - 1) / CB)), nfields);
+    if (dir < 0) hipLaunchKernelGGL((k_colfft<T, LL, -1, CB>), grid, dim3(CB * (LL / 8)), 0, st, a);
+    else hipLaunchKernelGGL((k_colfft<T, LL, +1, CB>), grid, dim3(CB * (LL / 8)), 0, st, a);
+    return hipGetLastError();
+}
+
+template <typename T, int LL>
+hipError_t xinv_state_len(const XinvArgs &a, hipStream_t st)
+{
+    constexpr int CB = col_block<T>(LL, sizeof(cpx<double>));
+    dim3 grid(x_grid(x_tiles(a.g, CB, (a.flags & 4) != 0)));
+    hipLaunchKernelGGL((k_xinv_state<T, LL, CB>), grid, dim3(CB * (LL / 8)), 0, st, a);
+    return hipGetLastError();
+}
+
+template <typename T, int LL>
+hipError_t xfwd_spec_len(const XfwdArgs &a, hipStream_t st)
+{
+    constexpr int CB = col_block<T>(LL, sizeof(cpx<double>));
+    dim3 grid(x_grid(x_tiles(a.g, CB, false)));
+    hipLaunchKernelGGL((k_xfwd_spec<T, LL, CB>), grid, dim3(CB * (LL / 8)), 0, st, a);
+    return hipGetLastError();
+}
+
+template <int MM>
+inline dim3 z_grid(const Grid &g)
+{
+    constexpr int LPB = z_lines_per_block<MM>();
+    const size_t nlines = (size_t)g.nx * g.ny;
+    return dim3((unsigned)((nlines + LPB - 1) / LPB));
+}
+
+template <typename T, int MM>
+hipError_t z_c2r_len(const ZArgs &a, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_z_c2r<T, MM>), z_grid<MM>(a.g), dim3(z_lines_per_block<MM>() * (MM / 8)), 0, st, a);
+    return hipGetLastError();
+}
+
+template <typename T, int MM>
+hipError_t z_r2c_len(const ZArgs &a, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_z_r2c<T, MM>), z_grid<MM>(a.g), dim3(z_lines_per_block<MM>() * (MM / 8)), 0, st, a);
+    return hipGetLastError();
+}
+
+template <typename T, int MM>
+hipError_t z_products_len(const ZArgs &a, int giver, hipStream_t st)
+{
+    const dim3 grid = z_grid<MM>(a.g), block(z_lines_per_block<MM>() * (MM / 8));
+    if (giver) hipLaunchKernelGGL((k_z_products<T, MM, true>), grid, block, 0, st, a);
+    else hipLaunchKernelGGL((k_z_products<T, MM, false>), grid, block, 0, st, a);
+    return hipGetLastError();
+}
+
+#define TG_XNL(CBV, M) \
+    hipLaunchKernelGGL((k_x_nlin<T, LL, CBV, M>), dim3(x_grid(x_tiles(a.g, CBV, false))), dim3(CBV * (LL / 8)), 0, st, a)
+
+template <typename T, int LL>
+hipError_t x_nlin_len(const XnlArgs &a, int mode, int x_tile_columns, hipStream_t st)
+{
+    // two FP64 tile buffers per block: the LDS-budget width (two blocks
+    // per CU), or twice that width (one block per CU)
+    constexpr int CB = col_block<T>(LL, 2 * sizeof(cpx<double>));
+    constexpr int CBW = 2 * CB * (LL / 8) <= 1024 ? 2 * CB : CB;
+    const bool wide = x_tile_columns == CBW && CBW != CB;
+    if (wide) {
+        if (mode == 1) TG_XNL(CBW, 1);
+        else if (mode == 2) TG_XNL(CBW, 2);
+        else TG_XNL(CBW, 0);
+    } else {
+        if (mode == 1) TG_XNL(CB, 1);
+        else if (mode == 2) TG_XNL(CB, 2);
+        else TG_XNL(CB, 0);
+    }
+    return hipGetLastError();
+}
+#undef TG_XNL
+
+#define TG_KERNELS_L(N, T, P)                                                                  \
+    P template hipError_t colfft_len<T, N>(const ColArgs &, int, int, hipStream_t);           \
+    P template hipError_t xinv_state_len<T, N>(const XinvArgs &, hipStream_t);                \
+    P template hipError_t xfwd_spec_len<T, N>(const XfwdArgs &, hipStream_t);                 \
+    P template hipError_t x_nlin_len<T, N>(const XnlArgs &, int, int, hipStream_t);
+#define TG_KERNELS_M(N, T, P)                                                                  \
+    P template hipError_t z_c2r_len<T, N>(const ZArgs &, hipStream_t);                        \
+    P template hipError_t z_r2c_len<T, N>(const ZArgs &, hipStream_t);                        \
+    P template hipError_t z_products_len<T, N>(const ZArgs &, int, hipStream_t);
+
+#if TG_LEN_UNIT == 0
+TG_ALL_UNITS(TG_KERNELS_L, TG_KERNELS_M, extern)
 
 template <typename T>
 hipError_t launch_colfft(const Grid &g, int axis, int dir, void *const *fields, int nfields, const void *tw,
@@ -779,17 +882,8 @@ hipError_t launch_colfft(const Grid &g, int axis, int dir, void *const *fields, 
         a.row_stride = (long long)g.ny * g.nzp;
         a.ncol = g.ny * g.nzp;
     }
-    TG_DISPATCH_L(L, {
-        // at most 512 threads a block, except FP32 lines of 512 points and
-        // more: 1024-thread blocks (16 columns at 512 points, 8 at 1024) run
-        // 2% and 9% faster than the 512-thread tiles of half the width
-        constexpr int CB = std::min(col_block<T>(LL, sizeof(cpx<T>)),
-                                    std::max(1, (sizeof(T) == 4 && LL >= 512 ? 1024 : 512) / (LL / 8)));
-        dim3 grid(x_grid(a.nouter * ((a.ncol + CB - 1) / CB)), nfields);
-        if (dir < 0) hipLaunchKernelGGL((k_colfft<T, LL, -1, CB>), grid, dim3(CB * (LL / 8)), 0, st, a);
-        else hipLaunchKernelGGL((k_colfft<T, LL, +1, CB>), grid, dim3(CB * (LL / 8)), 0, st, a);
-    });
-    return hipGetLastError();
+    TG_DISPATCH_L(T, L, return (colfft_len<T, LL>)(a, dir, nfields, st));
+    return hipErrorInvalidValue;
 }
 
 template <typename T>
@@ -810,12 +904,8 @@ hipError_t launch_xinv_state(const Grid &g, const Band &b, double *const *V, int
     a.b = b;
     a.ncomp = ncomp;
     a.flags = flags;
-    TG_DISPATCH_L(g.nx, {
-        constexpr int CB = col_block<T>(LL, sizeof(cpx<double>));
-        dim3 grid(x_grid(x_tiles(g, CB, (flags & 4) != 0)));
-        hipLaunchKernelGGL((k_xinv_state<T, LL, CB>), grid, dim3(CB * (LL / 8)), 0, st, a);
-    });
-    return hipGetLastError();
+    TG_DISPATCH_L(T, g.nx, return (xinv_state_len<T, LL>)(a, st));
+    return hipErrorInvalidValue;
 }
 
 template <typename T>
@@ -828,14 +918,9 @@ hipError_t launch_xfwd_spec(const Grid &g, void *const *W, int ncomp, double *co
     a.g = g;
     a.scale = scale;
     a.ncomp = ncomp;
-    TG_DISPATCH_L(g.nx, {
-        constexpr int CB = col_block<T>(LL, sizeof(cpx<double>));
-        dim3 grid(x_grid(x_tiles(g, CB, false)));
-        hipLaunchKernelGGL((k_xfwd_spec<T, LL, CB>), grid, dim3(CB * (LL / 8)), 0, st, a);
-    });
-    return hipGetLastError();
+    TG_DISPATCH_L(T, g.nx, return (xfwd_spec_len<T, LL>)(a, st));
+    return hipErrorInvalidValue;
 }
-
 
 template <typename T>
 hipError_t launch_z_c2r(const Grid &g, void *const *W, int ncomp, double *const *R, const void *twm, const void *twn,
@@ -847,13 +932,8 @@ hipError_t launch_z_c2r(const Grid &g, void *const *W, int ncomp, double *const 
     a.twn = twn;
     a.g = g;
     a.ncomp = ncomp;
-    const size_t nlines = (size_t)g.nx * g.ny;
-    TG_DISPATCH_M(g.nz / 2, {
-        constexpr int LPB = z_lines_per_block<MM>();
-        dim3 grid((unsigned)((nlines + LPB - 1) / LPB));
-        hipLaunchKernelGGL((k_z_c2r<T, MM>), grid, dim3(LPB * (MM / 8)), 0, st, a);
-    });
-    return hipGetLastError();
+    TG_DISPATCH_M(T, g.nz / 2, return (z_c2r_len<T, MM>)(a, st));
+    return hipErrorInvalidValue;
 }
 
 template <typename T>
@@ -866,13 +946,8 @@ hipError_t launch_z_r2c(const Grid &g, const double *const *R, int ncomp, void *
     a.twn = twn;
     a.g = g;
     a.ncomp = ncomp;
-    const size_t nlines = (size_t)g.nx * g.ny;
-    TG_DISPATCH_M(g.nz / 2, {
-        constexpr int LPB = z_lines_per_block<MM>();
-        dim3 grid((unsigned)((nlines + LPB - 1) / LPB));
-        hipLaunchKernelGGL((k_z_r2c<T, MM>), grid, dim3(LPB * (MM / 8)), 0, st, a);
-    });
-    return hipGetLastError();
+    TG_DISPATCH_M(T, g.nz / 2, return (z_r2c_len<T, MM>)(a, st));
+    return hipErrorInvalidValue;
 }
 
 template <typename T>
@@ -887,18 +962,9 @@ hipError_t launch_z_products(const Grid &g, void *const *U, void *const *P, int 
     a.twm = twm;
     a.twn = twn;
     a.g = g;
-    const size_t nlines = (size_t)g.nx * g.ny;
-    TG_DISPATCH_M(g.nz / 2, {
-        constexpr int LPB = z_lines_per_block<MM>();
-        dim3 grid((unsigned)((nlines + LPB - 1) / LPB));
-        if (giver) hipLaunchKernelGGL((k_z_products<T, MM, true>), grid, dim3(LPB * (MM / 8)), 0, st, a);
-        else hipLaunchKernelGGL((k_z_products<T, MM, false>), grid, dim3(LPB * (MM / 8)), 0, st, a);
-    });
-    return hipGetLastError();
+    TG_DISPATCH_M(T, g.nz / 2, return (z_products_len<T, MM>)(a, giver, st));
+    return hipErrorInvalidValue;
 }
-
-#define TG_XNL(CBV, M) \
-    hipLaunchKernelGGL((k_x_nlin<T, LL, CBV, M>), dim3(x_grid(x_tiles(g, CBV, false))), dim3(CBV * (LL / 8)), 0, st, a)
 
 template <typename T>
 hipError_t launch_x_nlin(const Grid &g, const Band &b, const RkParams &rk, void *const *P, double *const *V,
@@ -921,28 +987,12 @@ hipError_t launch_x_nlin(const Grid &g, const Band &b, const RkParams &rk, void 
     a.g = g;
     a.b = b;
     a.rk = rk;
-    TG_DISPATCH_L(g.nx, {
-        // two FP64 tile buffers per block: the LDS-budget width (two blocks
-        // per CU), or twice that width (one block per CU)
-        constexpr int CB = col_block<T>(LL, 2 * sizeof(cpx<double>));
-        constexpr int CBW = 2 * CB * (LL / 8) <= 1024 ? 2 * CB : CB;
-        const bool wide = x_tile_columns == CBW && CBW != CB;
-        const int mode = giver;
-        if (mode) {
-            a.rk.stage = 0;
-            a.rk.write_rhs = 0;
-        }
-        if (wide) {
-            if (mode == 1) TG_XNL(CBW, 1);
-            else if (mode == 2) TG_XNL(CBW, 2);
-            else TG_XNL(CBW, 0);
-        } else {
-            if (mode == 1) TG_XNL(CB, 1);
-            else if (mode == 2) TG_XNL(CB, 2);
-            else TG_XNL(CB, 0);
-        }
-    });
-    return hipGetLastError();
+    if (giver) {
+        a.rk.stage = 0;
+        a.rk.write_rhs = 0;
+    }
+    TG_DISPATCH_L(T, g.nx, return (x_nlin_len<T, LL>)(a, giver, x_tile_columns, st));
+    return hipErrorInvalidValue;
 }
 
 __global__ void k_fill_int(int *p, int n, int v)
@@ -997,5 +1047,8 @@ hipError_t launch_dealias(const Grid &g, double *const *V, int ncomp, hipStream_
 
 TG_INST(double)
 TG_INST(float)
+#else
+TG_THIS_UNIT(TG_KERNELS_L, TG_KERNELS_M)
+#endif  // TG_LEN_UNIT
 
 }  // namespace tg

@@ -9,11 +9,6 @@ hipError_t launch_mhd_elsasser(const Grid &g, double *const *V, double *const *W
                                double *const *Zm, hipStream_t st);
 // NU <- (NU + NW)/2, NW <- (NU - NW)/2 in place (MHD::Elsasser_to_UB_nlin)
 hipError_t launch_mhd_combine(const Grid &g, double *const *NU, double *const *NW, hipStream_t st);
-hipError_t launch_mhd_rk(const Grid &g, int stage, double dt, double nu, double eta, double *const *V,
-                         double *const *W, double *const *accV, double *const *accW, const double *const *Np,
-                         const double *const *Nm, double *const *Zp, double *const *Zm, hipStream_t st,
-                         int *nonfinite = nullptr);
-
 // Fused x pass of an MHD RK stage (1..4): P[3i+j] = FT[Z-_j Z+_i] (work
 // layout), state V (u), W (b) and accumulators accV, accW in the FP64
 // boundary layout; stages 1-3 write the x-inverse of the next stage's Z- into

@@ -6,14 +6,14 @@
 // N-_j = sum_i d_i(Z+_i Z-_j) (transposed giver form).  The kernels here are
 // the pointwise parts, one thread per mode of the FP64 boundary layout:
 //   k_mhd_elsasser  Z+ = D(u) + D(b), Z- = D(u) - D(b)   (stage-1 input)
-//   k_mhd_rk        N_u = (N+ + N-)/2, N_b = (N+ - N-)/2 (MHD.cc:150-163),
-//                   RHS_u = -P N_u, RHS_b = -N_b (Tadvance_functions.cc:
-//                   262-270), the Lawson RK4 stage for u (nu) and b (eta)
-//                   (RK_VF_MHD.cc:126-190), and the next stage's Z+-.
+//   k_mhd_combine   N_u = (N+ + N-)/2, N_b = (N+ - N-)/2 (MHD.cc:150-163)
+//                   for Compute_nlin
 //   k_x_mhd         the fused x pass of an RK stage: x-forward of the nine
-//                   products, N+ and N- together, the combination, pressure
-//                   and Lawson stage above for u and b, and the x-inverse of
-//                   the next stage's Z+- (one read of the products per stage)
+//                   products, N+ and N- together, their combination as
+//                   above, RHS_u = -P N_u, RHS_b = -N_b (Tadvance_functions.cc:
+//                   262-270), the Lawson RK4 stage for u (nu) and b (eta)
+//                   (RK_VF_MHD.cc:126-190), and the x-inverse of the next
+//                   stage's Z+- (one read of the products per stage)
 #include <type_traits>
 
 #include "tg_device.h"
@@ -52,24 +52,6 @@ struct MhdArgs {
     int *nonfinite;  // stage 4: set to 1 if the new state holds a non-finite value
 };
 
-__global__ void k_mhd_elsasser(MhdArgs a)
-{
-    const Grid &g = a.g;
-    const size_t n = (size_t)g.nx * g.ny * g.nzc;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        const int lz = i % g.nzc;
-        const int ly = (i / g.nzc) % g.ny;
-        const int lx = i / ((size_t)g.nzc * g.ny);
-        const bool z = dealias_zero_m(g, lx, ly, lz);
-        for (int c = 0; c < 3; c++) {
-            cpx<double> u = ((cpx<double> *)a.V[c])[i], b = ((cpx<double> *)a.W[c])[i];
-            if (z) u = b = mk<double>(0, 0);
-            ((cpx<double> *)a.Zp[c])[i] = u + b;
-            ((cpx<double> *)a.Zm[c])[i] = u - b;
-        }
-    }
-}
-
 __device__ __forceinline__ void lawson(int stage, const cpx<double> &v0in, const cpx<double> &r, double Eh, double E1,
                                        double dt, cpx<double> &ac, cpx<double> &vn)
 {
@@ -90,7 +72,8 @@ __device__ __forceinline__ void lawson(int stage, const cpx<double> &v0in, const
     }
 }
 
-__global__ void k_mhd_rk(MhdArgs a)
+#if TG_LEN_UNIT == 0
+__global__ void k_mhd_elsasser(MhdArgs a)
 {
     const Grid &g = a.g;
     const size_t n = (size_t)g.nx * g.ny * g.nzc;
@@ -98,53 +81,16 @@ __global__ void k_mhd_rk(MhdArgs a)
         const int lz = i % g.nzc;
         const int ly = (i / g.nzc) % g.ny;
         const int lx = i / ((size_t)g.nzc * g.ny);
-        const double K[3] = {kwm(lx, g.nx) * g.kf[0], kwm(ly, g.ny) * g.kf[1], lz * g.kf[2]};
-        const double kxs = K[0] * K[0];
-        const double ks = (kxs + K[1] * K[1]) + K[2] * K[2];
-        cpx<double> nu[3], nb[3];
-        for (int c = 0; c < 3; c++) {
-            const cpx<double> p = ((const cpx<double> *)a.Np[c])[i], m = ((const cpx<double> *)a.Nm[c])[i];
-            nu[c] = 0.5 * (p + m);
-            nb[c] = 0.5 * (p - m);
-        }
-        // pressure on N_u only (Pressure::Compute_pressure(U), Pressure.cc:56-72)
-        cpx<double> div = mul_i(K[0], nu[0]);
-        div = div + mul_i(K[1], nu[1]);
-        div = div + mul_i(K[2], nu[2]);
-        cpx<double> pr = mk<double>(0, 0);
-        if (!(lx == 0 && ly == 0 && lz == 0)) pr = mk<double>(div.x / ks, div.y / ks);
         const bool z = dealias_zero_m(g, lx, ly, lz);
-        const double Ehu = exp(a.fh_u * ks), E1u = exp(a.f1_u * ks);
-        const double Ehb = exp(a.fh_b * ks), E1b = exp(a.f1_b * ks);
         for (int c = 0; c < 3; c++) {
-            cpx<double> ru = nu[c] + mul_i(K[c], pr);
-            ru = mk<double>(-ru.x, -ru.y);
-            const cpx<double> rb = mk<double>(-nb[c].x, -nb[c].y);
-            cpx<double> u0 = ((cpx<double> *)a.V[c])[i], b0 = ((cpx<double> *)a.W[c])[i];
-            if (a.stage == 1 && z) u0 = b0 = mk<double>(0, 0);  // state dealiased in place at stage 1
-            cpx<double> au = mk<double>(0, 0), ab = mk<double>(0, 0);
-            if (a.stage >= 2) {
-                au = ((cpx<double> *)a.accV[c])[i];
-                ab = ((cpx<double> *)a.accW[c])[i];
-            }
-            cpx<double> un, bn;
-            lawson(a.stage, u0, ru, Ehu, E1u, a.dt, au, un);
-            lawson(a.stage, b0, rb, Ehb, E1b, a.dt, ab, bn);
-            if (a.stage == 4) {
-                ((cpx<double> *)a.V[c])[i] = un;
-                ((cpx<double> *)a.W[c])[i] = bn;
-                if (!(isfinite(un.x) && isfinite(un.y) && isfinite(bn.x) && isfinite(bn.y)) && a.nonfinite)
-                    *a.nonfinite = 1;
-            } else {
-                ((cpx<double> *)a.accV[c])[i] = au;
-                ((cpx<double> *)a.accW[c])[i] = ab;
-            }
-            if (z) un = bn = mk<double>(0, 0);
-            ((cpx<double> *)a.Zp[c])[i] = un + bn;
-            ((cpx<double> *)a.Zm[c])[i] = un - bn;
+            cpx<double> u = ((cpx<double> *)a.V[c])[i], b = ((cpx<double> *)a.W[c])[i];
+            if (z) u = b = mk<double>(0, 0);
+            ((cpx<double> *)a.Zp[c])[i] = u + b;
+            ((cpx<double> *)a.Zm[c])[i] = u - b;
         }
     }
 }
+
 
 __global__ void k_mhd_combine(MhdArgs a)
 {
@@ -159,12 +105,14 @@ __global__ void k_mhd_combine(MhdArgs a)
         }
 }
 
+#endif  // TG_LEN_UNIT == 0
+
 // ---------------------------------------------------------------------------
 // Fused MHD x pass (stages 1-4).  Tile staging as the NS x pass: CB columns
 // of the padded (ly, lz) space by all Nx, the products arriving by
 // direct-to-LDS loads into tile B while the previous one is transformed.
 // ---------------------------------------------------------------------------
-__constant__ int kMhdOrder[9] = {0, 4, 8, 3, 1, 7, 5, 2, 6};  // giver-form product order
+static __constant__ int kMhdOrder[9] = {0, 4, 8, 3, 1, 7, 5, 2, 6};  // giver-form product order
 
 template <typename T, int L, int CB>
 __global__ void __launch_bounds__(CB *(L / 8), 1) k_x_mhd(XmhdArgs a)
@@ -345,15 +293,24 @@ __global__ void __launch_bounds__(CB *(L / 8), 1) k_x_mhd(XmhdArgs a)
     if (bad && a.nonfinite) *a.nonfinite = 1;
 }
 
+template <typename T, int LL>
+hipError_t x_mhd_len(const XmhdArgs &a, hipStream_t st)
+{
+    constexpr int CB = col_block<T>(LL, 2 * sizeof(cpx<double>));
+    hipLaunchKernelGGL((k_x_mhd<T, LL, CB>), dim3(x_grid(x_tiles(a.g, CB, false))), dim3(CB * (LL / 8)), 0, st, a);
+    return hipGetLastError();
+}
+
+#define TG_MHD_L(N, T, P) P template hipError_t x_mhd_len<T, N>(const XmhdArgs &, hipStream_t);
+
+#if TG_LEN_UNIT == 0
+TG_ALL_UNITS(TG_MHD_L, TG_NO_LENGTHS, extern)
+
 template <typename T>
 hipError_t launch_x_mhd(const XmhdArgs &a, hipStream_t st)
 {
-    const Grid &g = a.g;
-    TG_DISPATCH_L(g.nx, {
-        constexpr int CB = col_block<T>(LL, 2 * sizeof(cpx<double>));
-        hipLaunchKernelGGL((k_x_mhd<T, LL, CB>), dim3(x_grid(x_tiles(g, CB, false))), dim3(CB * (LL / 8)), 0, st, a);
-    });
-    return hipGetLastError();
+    TG_DISPATCH_L(T, a.g.nx, return (x_mhd_len<T, LL>)(a, st));
+    return hipErrorInvalidValue;
 }
 
 template hipError_t launch_x_mhd<double>(const XmhdArgs &, hipStream_t);
@@ -389,29 +346,8 @@ hipError_t launch_mhd_combine(const Grid &g, double *const *NU, double *const *N
     return hipGetLastError();
 }
 
-hipError_t launch_mhd_rk(const Grid &g, int stage, double dt, double nu, double eta, double *const *V,
-                         double *const *W, double *const *accV, double *const *accW, const double *const *Np,
-                         const double *const *Nm, double *const *Zp, double *const *Zm, hipStream_t st,
-                         int *nonfinite)
-{
-    MhdArgs a = mhd_args(g, V, W);
-    for (int c = 0; c < 3; c++) {
-        a.accV[c] = accV[c];
-        a.accW[c] = accW[c];
-        a.Np[c] = Np[c];
-        a.Nm[c] = Nm[c];
-        a.Zp[c] = Zp[c];
-        a.Zm[c] = Zm[c];
-    }
-    a.stage = stage;
-    a.dt = dt;
-    a.fh_u = -nu * 0.5 * dt;
-    a.f1_u = -nu * 1.0 * dt;
-    a.fh_b = -eta * 0.5 * dt;
-    a.f1_b = -eta * 1.0 * dt;
-    a.nonfinite = nonfinite;
-    hipLaunchKernelGGL(k_mhd_rk, dim3(8192), dim3(256), 0, st, a);
-    return hipGetLastError();
-}
+#else
+TG_THIS_UNIT(TG_MHD_L, TG_NO_LENGTHS)
+#endif  // TG_LEN_UNIT
 
 }  // namespace tg

@@ -329,7 +329,7 @@ __device__ __forceinline__ void zline_r2c_post(cpx<T> *v, cpx<T> *__restrict__ X
     for (int s = 0; s < 8; s++) {
         const int k = j + TPL * s;
         const cpx<T> zk = v[s];
-        const cpx<T> zm = lds[lpad((M - k) & (M - 1))];
+        const cpx<T> zm = lds[lpad(k == 0 ? 0 : M - k)];
         const cpx<T> a = zk + conj(zm);
         const cpx<T> d = zk - conj(zm);
         const cpx<T> w = twn[k];
@@ -356,31 +356,87 @@ __host__ __device__ constexpr int z_lines_per_block()
 }
 
 // ---------------------------------------------------------------------------
-// Size dispatch of the launchers (transform lengths are compile-time).
+// Transform lengths with compiled kernels, and the size dispatch of the
+// launchers (lengths are compile-time).  x and y lines take L, z lines the
+// half length M = Nz/2.  Power-of-two lengths in both precisions; lengths
+// with radix-3/5 factors in FP64.
+//
+// Each launcher fills its argument block and calls a per-length function
+// (NAME_len<T, L>) that launches the kernels of that length.  The per-length
+// functions are instantiated in length units -- the same source compiled
+// once per unit with TG_LEN_UNIT = 1..4 (1: power-of-two FP64, 2:
+// power-of-two FP32, 3 and 4: radix-3/5 lengths FP64) -- so that the kernel
+// instances compile in parallel; unit 0 (the plain compilation) holds the
+// launchers and declares the per-length functions extern.
 // ---------------------------------------------------------------------------
-#define TG_DISPATCH_L(Lval, CALL) \
-    switch (Lval) {               \
-        case 8: { constexpr int LL = 8; CALL; } break;     \
-        case 16: { constexpr int LL = 16; CALL; } break;   \
-        case 32: { constexpr int LL = 32; CALL; } break;   \
-        case 64: { constexpr int LL = 64; CALL; } break;   \
-        case 128: { constexpr int LL = 128; CALL; } break; \
-        case 256: { constexpr int LL = 256; CALL; } break; \
-        case 512: { constexpr int LL = 512; CALL; } break; \
-        case 1024: { constexpr int LL = 1024; CALL; } break; \
-        default: return hipErrorInvalidValue;              \
+#define TG_L_POW2(F, T, P) F(8, T, P) F(16, T, P) F(32, T, P) F(64, T, P) F(128, T, P) F(256, T, P) F(512, T, P) \
+    F(1024, T, P)
+#define TG_L_ODDA(F, T, P) F(24, T, P) F(40, T, P) F(48, T, P) F(80, T, P) F(96, T, P) F(120, T, P) F(160, T, P) \
+    F(192, T, P)
+#define TG_L_ODDB(F, T, P) F(240, T, P) F(320, T, P) F(384, T, P) F(480, T, P) F(640, T, P) F(768, T, P) F(960, T, P)
+#define TG_M_POW2(F, T, P) F(8, T, P) F(16, T, P) F(32, T, P) F(64, T, P) F(128, T, P) F(256, T, P) F(512, T, P)
+#define TG_M_ODDA(F, T, P) F(24, T, P) F(40, T, P) F(48, T, P) F(80, T, P) F(96, T, P) F(120, T, P) F(160, T, P) \
+    F(192, T, P)
+#define TG_M_ODDB(F, T, P) F(240, T, P) F(320, T, P) F(384, T, P) F(480, T, P)
+
+#define TG_UNIT_1(FL, FM, P) TG_L_POW2(FL, double, P) TG_M_POW2(FM, double, P)
+#define TG_UNIT_2(FL, FM, P) TG_L_POW2(FL, float, P) TG_M_POW2(FM, float, P)
+#define TG_UNIT_3(FL, FM, P) TG_L_ODDA(FL, double, P) TG_M_ODDA(FM, double, P)
+#define TG_UNIT_4(FL, FM, P) TG_L_ODDB(FL, double, P) TG_M_ODDB(FM, double, P)
+#define TG_ALL_UNITS(FL, FM, P) TG_UNIT_1(FL, FM, P) TG_UNIT_2(FL, FM, P) TG_UNIT_3(FL, FM, P) TG_UNIT_4(FL, FM, P)
+#define TG_CAT_(a, b) a##b
+#define TG_CAT(a, b) TG_CAT_(a, b)
+// the instantiations of this compilation's length unit
+#define TG_THIS_UNIT(FL, FM) TG_CAT(TG_UNIT_, TG_LEN_UNIT)(FL, FM, )
+#define TG_NO_LENGTHS(N, T, P)
+
+#ifndef TG_LEN_UNIT
+#define TG_LEN_UNIT 0
+#endif
+
+#define TG_CASE_P2(N, T, CALL) \
+    case N: { constexpr int LL = N, MM = N; (void)LL; (void)MM; CALL; } break;
+#define TG_CASE_ODD(N, T, CALL)                        \
+    case N: {                                          \
+        constexpr int LL = N, MM = N;                  \
+        (void)LL;                                      \
+        (void)MM;                                      \
+        if constexpr (sizeof(T) == 8) { CALL; }        \
+        else return hipErrorInvalidValue;              \
+    } break;
+// CALL (no top-level commas: parenthesise template argument lists) is
+// compiled once per length with the constant LL (x/y lines) or MM (z half
+// lines); a length without kernels returns hipErrorInvalidValue.
+#define TG_DISPATCH_L(T, Lval, CALL)                                                                  \
+    switch (Lval) {                                                                                   \
+        TG_L_POW2(TG_CASE_P2, T, CALL) TG_L_ODDA(TG_CASE_ODD, T, CALL) TG_L_ODDB(TG_CASE_ODD, T, CALL) \
+        default: return hipErrorInvalidValue;                                                         \
+    }
+#define TG_DISPATCH_M(T, Mval, CALL)                                                                  \
+    switch (Mval) {                                                                                   \
+        TG_M_POW2(TG_CASE_P2, T, CALL) TG_M_ODDA(TG_CASE_ODD, T, CALL) TG_M_ODDB(TG_CASE_ODD, T, CALL) \
+        default: return hipErrorInvalidValue;                                                         \
     }
 
-#define TG_DISPATCH_M(Mval, CALL) \
-    switch (Mval) {               \
-        case 8: { constexpr int MM = 8; CALL; } break;     \
-        case 16: { constexpr int MM = 16; CALL; } break;   \
-        case 32: { constexpr int MM = 32; CALL; } break;   \
-        case 64: { constexpr int MM = 64; CALL; } break;   \
-        case 128: { constexpr int MM = 128; CALL; } break; \
-        case 256: { constexpr int MM = 256; CALL; } break; \
-        case 512: { constexpr int MM = 512; CALL; } break; \
-        default: return hipErrorInvalidValue;              \
+// Host checks of a length against the compiled ones (x/y lines L, z half
+// lines M; FP32 takes powers of two only).
+#define TG_OK_P2(N, T, P) case N: return true;
+#define TG_OK_ODD(N, T, P) case N: return !fp32;
+inline bool l_compiled(int n, bool fp32)
+{
+    switch (n) {
+        TG_L_POW2(TG_OK_P2, , ) TG_L_ODDA(TG_OK_ODD, , ) TG_L_ODDB(TG_OK_ODD, , )
+        default: return false;
     }
+}
+inline bool m_compiled(int n, bool fp32)
+{
+    switch (n) {
+        TG_M_POW2(TG_OK_P2, , ) TG_M_ODDA(TG_OK_ODD, , ) TG_M_ODDB(TG_OK_ODD, , )
+        default: return false;
+    }
+}
+#undef TG_OK_P2
+#undef TG_OK_ODD
 
 }  // namespace tg

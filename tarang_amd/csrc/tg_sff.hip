@@ -54,7 +54,7 @@ __device__ __forceinline__ void sc_forward_core(cpx<T> *v, cpx<T> *col, const cp
 #pragma unroll
     for (int s = 0; s < 8; s++) {
         const int k = j + TPL * s;
-        const cpx<T> vn = col[lpad((L - k) & (L - 1))];
+        const cpx<T> vn = col[lpad(k == 0 ? 0 : L - k)];
         const cpx<T> w = twq(j, s);
         const cpx<T> p = w * v[s];
         const cpx<T> q = conj(w) * vn;
@@ -586,6 +586,7 @@ struct SffPlaneArgs {
     int ncomp, mode;
 };
 
+#if TG_LEN_UNIT == 0
 __global__ void k_sff_reality(SffPlaneArgs a)
 {
     const Grid &g = a.g;
@@ -606,9 +607,57 @@ __global__ void k_sff_reality(SffPlaneArgs a)
     }
 }
 
+#endif  // TG_LEN_UNIT == 0
+
 // ---------------------------------------------------------------------------
-// Launchers
+// Per-length launch functions and launchers
 // ---------------------------------------------------------------------------
+template <typename T, int LL>
+hipError_t xinv_sc_len(const XscArgs &a, hipStream_t st)
+{
+    constexpr int CB = col_block<T>(LL, sizeof(cpx<double>));
+    hipLaunchKernelGGL((k_xinv_sc<T, LL, CB>), dim3(x_grid(x_tiles(a.g, CB, (a.flags & 4) != 0))),
+                       dim3(CB * (LL / 8)), 0, st, a);
+    return hipGetLastError();
+}
+
+template <typename T, int LL>
+hipError_t xfwd_sc_len(const XscArgs &a, hipStream_t st)
+{
+    constexpr int CB = col_block<T>(LL, sizeof(cpx<double>));
+    hipLaunchKernelGGL((k_xfwd_sc<T, LL, CB>), dim3(x_grid(x_tiles(a.g, CB, false))), dim3(CB * (LL / 8)), 0, st, a);
+    return hipGetLastError();
+}
+
+template <typename T, int MM>
+hipError_t z_rbc_len(const ZArgs &a, hipStream_t st)
+{
+    constexpr int LPB = z_lines_per_block<MM>();
+    const size_t nlines = (size_t)a.g.nx * a.g.ny;
+    dim3 grid((unsigned)((nlines + LPB - 1) / LPB));
+    hipLaunchKernelGGL((k_z_rbc<T, MM>), grid, dim3(LPB * (MM / 8)), 0, st, a);
+    return hipGetLastError();
+}
+
+template <typename T, int LL>
+hipError_t x_rbc_len(const XrbcArgs &a, hipStream_t st)
+{
+    constexpr int CB = col_block<T>(LL, 2 * sizeof(cpx<double>));
+    const dim3 grid(x_grid(x_tiles(a.g, CB, false)));
+    hipLaunchKernelGGL((k_x_rbc<T, LL, CB, 0>), grid, dim3(CB * (LL / 8)), 0, st, a);
+    hipLaunchKernelGGL((k_x_rbc<T, LL, CB, 1>), grid, dim3(CB * (LL / 8)), 0, st, a);
+    return hipGetLastError();
+}
+
+#define TG_SFF_L(N, T, P)                                                   \
+    P template hipError_t xinv_sc_len<T, N>(const XscArgs &, hipStream_t); \
+    P template hipError_t xfwd_sc_len<T, N>(const XscArgs &, hipStream_t); \
+    P template hipError_t x_rbc_len<T, N>(const XrbcArgs &, hipStream_t);
+#define TG_SFF_M(N, T, P) P template hipError_t z_rbc_len<T, N>(const ZArgs &, hipStream_t);
+
+#if TG_LEN_UNIT == 0
+TG_ALL_UNITS(TG_SFF_L, TG_SFF_M, extern)
+
 template <typename T>
 hipError_t launch_xinv_sc(const Grid &g, double *const *V, int ncomp, int parity, void *const *U, int flags,
                           const void *twx, const void *twq, hipStream_t st)
@@ -624,12 +673,8 @@ hipError_t launch_xinv_sc(const Grid &g, double *const *V, int ncomp, int parity
     a.ncomp = ncomp;
     a.parity = parity;
     a.flags = flags;
-    TG_DISPATCH_L(g.nx, {
-        constexpr int CB = col_block<T>(LL, sizeof(cpx<double>));
-        hipLaunchKernelGGL((k_xinv_sc<T, LL, CB>), dim3(x_grid(x_tiles(g, CB, (flags & 4) != 0))),
-                           dim3(CB * (LL / 8)), 0, st, a);
-    });
-    return hipGetLastError();
+    TG_DISPATCH_L(T, g.nx, return (xinv_sc_len<T, LL>)(a, st));
+    return hipErrorInvalidValue;
 }
 
 template <typename T>
@@ -647,12 +692,8 @@ hipError_t launch_xfwd_sc(const Grid &g, void *const *W, int ncomp, int parity, 
     a.ncomp = ncomp;
     a.parity = parity;
     a.scale = scale;
-    TG_DISPATCH_L(g.nx, {
-        constexpr int CB = col_block<T>(LL, sizeof(cpx<double>));
-        hipLaunchKernelGGL((k_xfwd_sc<T, LL, CB>), dim3(x_grid(x_tiles(g, CB, false))), dim3(CB * (LL / 8)), 0, st,
-                           a);
-    });
-    return hipGetLastError();
+    TG_DISPATCH_L(T, g.nx, return (xfwd_sc_len<T, LL>)(a, st));
+    return hipErrorInvalidValue;
 }
 
 template <typename T>
@@ -665,13 +706,8 @@ hipError_t launch_z_rbc(const Grid &g, void *const *U, void *const *P, const voi
     a.twm = twm;
     a.twn = twn;
     a.g = g;
-    const size_t nlines = (size_t)g.nx * g.ny;
-    TG_DISPATCH_M(g.nz / 2, {
-        constexpr int LPB = z_lines_per_block<MM>();
-        dim3 grid((unsigned)((nlines + LPB - 1) / LPB));
-        hipLaunchKernelGGL((k_z_rbc<T, MM>), grid, dim3(LPB * (MM / 8)), 0, st, a);
-    });
-    return hipGetLastError();
+    TG_DISPATCH_M(T, g.nz / 2, return (z_rbc_len<T, MM>)(a, st));
+    return hipErrorInvalidValue;
 }
 
 template <typename T>
@@ -695,13 +731,8 @@ hipError_t launch_x_rbc(const Grid &g, const RbcParams &rp, void *const *P, doub
     a.twq = twq;
     a.g = g;
     a.rp = rp;
-    TG_DISPATCH_L(g.nx, {
-        constexpr int CB = col_block<T>(LL, 2 * sizeof(cpx<double>));
-        const dim3 grid(x_grid(x_tiles(g, CB, false)));
-        hipLaunchKernelGGL((k_x_rbc<T, LL, CB, 0>), grid, dim3(CB * (LL / 8)), 0, st, a);
-        hipLaunchKernelGGL((k_x_rbc<T, LL, CB, 1>), grid, dim3(CB * (LL / 8)), 0, st, a);
-    });
-    return hipGetLastError();
+    TG_DISPATCH_L(T, g.nx, return (x_rbc_len<T, LL>)(a, st));
+    return hipErrorInvalidValue;
 }
 
 hipError_t launch_sff_reality(const Grid &g, double *const *V, int ncomp, int mode, hipStream_t st)
@@ -729,5 +760,8 @@ hipError_t launch_sff_reality(const Grid &g, double *const *V, int ncomp, int mo
 
 TG_SFF_INST(double)
 TG_SFF_INST(float)
+#else
+TG_THIS_UNIT(TG_SFF_L, TG_SFF_M)
+#endif  // TG_LEN_UNIT
 
 }  // namespace tg

@@ -10,7 +10,7 @@ import os
 
 import numpy as np
 
-SHAPES = [(8, 8, 8), (16, 8, 32), (4, 8, 16), (16, 16, 16)]
+SHAPES = [(8, 8, 8), (16, 8, 32), (4, 8, 16), (16, 16, 16), (6, 10, 12), (15, 9, 20)]
 
 
 def main():

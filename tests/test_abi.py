@@ -39,7 +39,8 @@ def _cfg(**kw):
 
 
 @pytest.mark.parametrize("kw,code", [
-    (dict(nx=48), -2), (dict(nz=8), -2), (dict(ny=2048), -2), (dict(basis=3), -1), (dict(precision=7), -1),
+    (dict(nx=48, precision=1), -2), (dict(nx=56), -2), (dict(ny=72), -2), (dict(nx=12), -2), (dict(nz=8), -2),
+    (dict(nz=36), -2), (dict(nz=1280), -2), (dict(ny=2048), -2), (dict(basis=3), -1), (dict(precision=7), -1),
     (dict(dealias=5), -1), (dict(kfactor=(1.0, 0.0, 1.0)), -1),
 ])
 def test_plan_create_rejects_bad_config(kw, code):
