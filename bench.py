@@ -60,6 +60,8 @@ def parse():
     ap.add_argument("--no-mhd", action="store_true", help="skip the MHD precision measurement")
     ap.add_argument("--mhd-only", action="store_true", help="run only the MHD precision measurement")
     ap.add_argument("--copy-calib", action="store_true", help="run a plain 1 GiB device copy (counter calibration)")
+    ap.add_argument("--fp32-only", action="store_true",
+                    help="run only the FP32-transform RK4 steps of the headline workload (for profiling)")
     return ap.parse_args()
 
 
@@ -141,6 +143,9 @@ def main():
         return
     if args.mhd_only:
         print(json.dumps({"mhd": mhd_precision(api, torch, np)}))
+        return
+    if args.fp32_only:
+        print(json.dumps({"fp32_only": fp32_steps(api, torch, args.n, args.steps, args.warmup)}))
         return
 
     hbm_copy = copy_bandwidth(torch)
@@ -350,6 +355,27 @@ def main():
     }
     out.update(extra)
     print(json.dumps(out))
+
+
+def fp32_steps(api, torch, n, steps, warmup, nu=1e-3, dt=1e-3):
+    """The headline workload with FP32 transforms alone (same field, forcing
+    and dealias), timed over `steps` RK4 steps: the run the FP32 profiles
+    come from."""
+    N = (n, n, n)
+    b = api.FFF_Basis(N, precision=api.PREC_FP32, dealias=api.DEALIAS_REF_CORNER, strong_reality=True)
+    U = api.FluidVF(b, nu)
+    U.Init_cond_spectrum(kind=1, kmin=2.0, kmax=170.0 if n >= 512 else n / 3.0, energy=0.5, seed=2)
+    ta = api.Time_advance_incompress(U, dt, force_kind=1, force_eps=0.1, force_r_in=2.0, force_r_out=3.0)
+    for _ in range(warmup):
+        ta.Time_advance_step(1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ta.Time_advance_step(1)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / max(steps, 1)
+    print(f"[bench] fp32 {steps} steps: {ms:.3f} ms/step", file=sys.stderr, flush=True)
+    return {"grid": list(N), "steps": steps, "warmup": warmup, "ms_per_step": round(ms, 4)}
 
 
 def rbc_run(api, torch, np, prec, N, steps, warmup, Ra, Pr, dt, seed, profile):
