@@ -203,6 +203,16 @@ class FluidVF {
         for (auto &f : cvf) e += basis.Get_total_energy(f.data());
         return e;
     }
+    // FluidVF::Get_dt (FluidVF.cc:546-572): dt_fixed (dt_option 0) or the CFL
+    // step of the dealiased field (dt_option 1; dealiases cvf in place)
+    double Get_dt(double courant, double dt_fixed, int dt_option = 1)
+    {
+        if (dt_option == 0) return dt_fixed;
+        auto v = V();
+        double dt = 0;
+        check("tg_get_dt", tg_get_dt(basis.plan(), v.data(), courant, dt_fixed, &dt, nullptr));
+        return dt;
+    }
     // CVF::Compute_total_k2energy (Cvf.cc:224-227)
     double Compute_total_k2energy()
     {
@@ -218,6 +228,10 @@ class FluidVF {
 
     Basis &basis;
     double dissipation_coefficient;
+    // hyper_dissipation_coefficient / _exponent (FluidVF.cc:72-84; exponent 0:
+    // none), taken by the RK4 step and the adjoint
+    double hyper_dissipation_coefficient = 0.0;
+    int hyper_dissipation_exponent = 0;
     std::array<DeviceField, 3> cvf, nlin;
 };
 
@@ -275,6 +289,8 @@ class Time_advance_incompress {
     {
         fluid.nu = U.dissipation_coefficient;
         fluid.dt = dt;
+        fluid.nu_hyper = U.hyper_dissipation_coefficient;
+        fluid.hyper_exponent = U.hyper_dissipation_exponent;
     }
     void Set_energy_supply(double eps, double r_in, double r_out)
     {
@@ -382,6 +398,8 @@ class Adjoint_incompress {
     {
         fluid.nu = U.dissipation_coefficient;
         fluid.dt = dt;
+        fluid.nu_hyper = U.hyper_dissipation_coefficient;
+        fluid.hyper_exponent = U.hyper_dissipation_exponent;
     }
     ~Adjoint_incompress() { (void)tg_adjoint_free(U.basis.plan()); }
     void Forward(int nsteps)

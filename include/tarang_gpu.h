@@ -78,6 +78,14 @@ typedef struct tg_fluid {
     double force_eps;   /* energy injection rate */
     double force_r_in;  /* forcing band (r_in, r_out] */
     double force_r_out;
+    /* FluidVF hyper_dissipation_coefficient / _exponent (FluidVF.cc:72-84):
+     * the viscous factors become exp((-nu - nu_hyper K^(q-2)) K^2 a dt)
+     * (Universal::Array_exp_ksqr, universal_basic.cc:265-291), K^(q-2) =
+     * (K^2)^((q-2)/2) with integer division; q = hyper_exponent, 0 = off,
+     * else 2..16.  Taken by tg_rk4_step and the adjoint; the MHD and BDF1
+     * steps return TG_ERR_ARG for q != 0. */
+    double nu_hyper;
+    int hyper_exponent;
 } tg_fluid;
 
 /* Rayleigh-Benard parameters, Pr_option PRLARGE with Uscaling ULARGE
@@ -118,6 +126,17 @@ int tg_forward(tg_plan *plan, int sincos_code, const double *real_dev, double *s
  * input preserved.  SFF (SFF_pencil_tr.cc:58-67): f_j = sum_m w_m A_m
  * phi_m(x_j), w_0 = 1, w_m = 2. */
 int tg_inverse(tg_plan *plan, int sincos_code, const double *spec_dev, double *real_dev);
+
+/* FluidVF::Get_dt with dt_option 1 (FluidVF.cc:546-572), as Compute_nlin
+ * calls it at the start of a step (compute_nlin.cc:60-73): V dealiased in
+ * place, umax_i = max |u_i| over the real-space grid (Inverse_transform +
+ * Get_local_max_real_space, FluidVF.cc:499-509), dt = min(courant / (umax_x
+ * / Dx + umax_y / Dy + umax_z / Dz), dt_fixed), D_i = L_i / N_i with L_i =
+ * 2 pi / kfactor_i (FFF_pencil.cc:64-72; SFF: L_x = pi / kfactor_x,
+ * SFF_pencil.cc:77-86, u_x a sine and u_y, u_z cosine series).  FFF and
+ * SFF plans; umax (3 doubles) may be null.  Synchronises the plan's stream;
+ * TG_ERR_NAN if a maximum is not finite.  dt_option 0 is dt_fixed itself. */
+int tg_get_dt(tg_plan *plan, double *const V_dev[3], double courant, double dt_fixed, double *dt, double umax[3]);
 
 /* FFF_PENCIL::Dealias (FFF_pencil_basic.cc:87-90) in place, per the plan's mode. */
 int tg_dealias(tg_plan *plan, double *const V_dev[], int ncomp);

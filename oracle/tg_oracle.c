@@ -570,22 +570,53 @@ void or_compute_rhs(const or_grid *g, double *N[3], double *const F[3])
             }
 }
 
-/* Array_exp_ksqr (universal_basic.cc:235-255): A *= exp(factor*K^2) */
-static void exp_ksqr(const or_grid *g, double *spec, double factor)
+/* The exponent of Array_exp_ksqr: factor K^2 (universal_basic.cc:235-255),
+ * or with hyperviscosity (factor + hyper_factor K^(q-2)) K^2
+ * (universal_basic.cc:265-291), K^(q-2) = (K^2)^((q-2)/2) with integer
+ * division (q = 4: K^2), the power by repeated multiplication; q = 0: none. */
+typedef struct { double f, fh; int q; } lfac;
+
+static double lexpo(const lfac *L, double ks)
+{
+    if (L->q == 0) return L->f * ks;
+    const int n = L->q == 4 ? 1 : (L->q - 2) / 2;
+    double kp = 1.0;
+    for (int i = 0; i < n; i++) kp *= ks;
+    return (L->f + L->fh * kp) * ks;
+}
+
+/* the factors of a fluid for a multiple a of dt: (-nu a dt, -nu_hyper a dt, q) */
+static lfac fluid_lfac(const or_fluid *f, double a)
+{
+    lfac L = {-f->nu * a * f->dt, -f->nu_hyper * a * f->dt, f->hyper_exponent};
+    return L;
+}
+
+/* Array_exp_ksqr: A *= exp(exponent) */
+static void exp_ksqr_l(const or_grid *g, double *spec, const lfac *L)
 {
     cplx *A = (cplx *)spec;
     const int nx = g->nx, ny = g->ny, nzc = g->nz / 2 + 1;
 #pragma omp parallel for collapse(2) schedule(static)
     for (int lx = 0; lx < nx; lx++)
         for (int ly = 0; ly < ny; ly++)
-            for (int lz = 0; lz < nzc; lz++) A[cidx(g, lx, ly, lz)] *= exp(factor * ksqr(g, lx, ly, lz));
+            for (int lz = 0; lz < nzc; lz++) A[cidx(g, lx, ly, lz)] *= exp(lexpo(L, ksqr(g, lx, ly, lz)));
+}
+
+/* Array_exp_ksqr (universal_basic.cc:235-255): A *= exp(factor*K^2) */
+static void exp_ksqr(const or_grid *g, double *spec, double factor)
+{
+    lfac L = {factor, 0.0, 0};
+    exp_ksqr_l(g, spec, &L);
 }
 
 /* FluidVF::Mult_field_exp_ksqr_dt / Mult_nlin_exp_ksqr_dt (FluidVF.cc:339-389) */
 static void mult_exp(const or_grid *g, const or_fluid *f, double *A[3], double a)
 {
-    if (fabs(a) > 1e-15)
-        for (int c = 0; c < 3; c++) exp_ksqr(g, A[c], -f->nu * a * f->dt);
+    if (fabs(a) > 1e-15) {
+        lfac L = fluid_lfac(f, a);
+        for (int c = 0; c < 3; c++) exp_ksqr_l(g, A[c], &L);
+    }
 }
 
 /* FluidVF::Add_nlin_factor_dt (FluidVF.cc:308-316): Y += (factor*dt)*N */
@@ -997,16 +1028,16 @@ void or_rhs_vjp(const or_grid *g, double *const V[3], double *const Lam[3], doub
     for (int t = 0; t < 6; t++) free(sig[t]);
 }
 
-static void scale_add(const or_grid *g, cplx *y, const cplx *x, double a, const double *fac)
+static void scale_add(const or_grid *g, cplx *y, const cplx *x, double a, const lfac *fac)
 {
-    /* y += a * fac(K) * x, fac = NULL for 1 */
+    /* y += a * exp(exponent of fac) * x, fac = NULL for 1 */
     const int nx = g->nx, ny = g->ny, nzc = g->nz / 2 + 1;
 #pragma omp parallel for collapse(2) schedule(static)
     for (int lx = 0; lx < nx; lx++)
         for (int ly = 0; ly < ny; ly++)
             for (int lz = 0; lz < nzc; lz++) {
                 size_t i = cidx(g, lx, ly, lz);
-                double f = fac ? exp(fac[0] * ksqr(g, lx, ly, lz)) : 1.0;
+                double f = fac ? exp(lexpo(fac, ksqr(g, lx, ly, lz))) : 1.0;
                 y[i] += a * f * x[i];
             }
 }
@@ -1017,7 +1048,7 @@ void or_rk4_step_vjp(const or_grid *g, const or_fluid *f, double *const V[3], do
 {
     const size_t nc = ncplx(g);
     const double dt = f->dt;
-    double fh = -f->nu * 0.5 * dt, f1 = -f->nu * 1.0 * dt;
+    const lfac fh = fluid_lfac(f, 0.5), f1 = fluid_lfac(f, 1.0);
     double *St[4][3], *G[3], *lg[3], *lv[3][3];
     /* stage inputs V0..V3 */
     for (int s = 0; s < 4; s++)
@@ -1037,14 +1068,14 @@ void or_rk4_step_vjp(const or_grid *g, const or_fluid *f, double *const V[3], do
             cplx *v1 = (cplx *)St[1][c];
             memcpy(v1, W[c], sizeof(cplx) * nc);  /* W is D V0 after compute_nlin */
             scale_add(g, v1, (cplx *)G[c], 0.5 * dt, NULL);
-            exp_ksqr(g, St[1][c], fh);
+            exp_ksqr_l(g, St[1][c], &fh);
         }
         /* V2 = Eh V0 + dt/2 f(V1) */
         for (int c = 0; c < 3; c++) memcpy(W[c], St[1][c], sizeof(cplx) * nc);
         or_compute_nlin(g, W, G); or_compute_rhs(g, G, NULL);
         for (int c = 0; c < 3; c++) {
             memcpy(St[2][c], V[c], sizeof(cplx) * nc);
-            exp_ksqr(g, St[2][c], fh);
+            exp_ksqr_l(g, St[2][c], &fh);
             scale_add(g, (cplx *)St[2][c], (cplx *)G[c], 0.5 * dt, NULL);
         }
         /* V3 = Eh (Eh V0 + dt f(V2)) */
@@ -1052,9 +1083,9 @@ void or_rk4_step_vjp(const or_grid *g, const or_fluid *f, double *const V[3], do
         or_compute_nlin(g, W, G); or_compute_rhs(g, G, NULL);
         for (int c = 0; c < 3; c++) {
             memcpy(St[3][c], V[c], sizeof(cplx) * nc);
-            exp_ksqr(g, St[3][c], fh);
+            exp_ksqr_l(g, St[3][c], &fh);
             scale_add(g, (cplx *)St[3][c], (cplx *)G[c], dt, NULL);
-            exp_ksqr(g, St[3][c], fh);
+            exp_ksqr_l(g, St[3][c], &fh);
         }
         for (int c = 0; c < 3; c++) free(W[c]);
     }
@@ -1091,7 +1122,7 @@ void or_rk4_step_vjp(const or_grid *g, const or_fluid *f, double *const V[3], do
         cplx *out = (cplx *)lam[c];
         memset(out, 0, sizeof(cplx) * nc);
         scale_add(g, out, (cplx *)l4[c], 1.0, &f1);
-        double f2 = 2 * fh;
+        const lfac f2 = {2 * fh.f, 2 * fh.fh, fh.q};
         scale_add(g, out, (cplx *)lv[2][c], 1.0, &f2);
         scale_add(g, out, (cplx *)lv[1][c], 1.0, &fh);
         double *t = (double *)malloc(sizeof(cplx) * nc);

@@ -48,6 +48,8 @@ typedef struct or_fluid {
     double force_eps;    /* energy injection rate */
     double force_r_in;   /* band (r_in, r_out] */
     double force_r_out;
+    double nu_hyper;     /* hyper_dissipation_coefficient */
+    int hyper_exponent;  /* q: exp((-nu - nu_hyper K^(q-2)) K^2 a dt); 0 = none */
 } or_fluid;
 
 /* ---- 1D FFT (unnormalised, in place), sign = -1 forward, +1 backward ---- */
@@ -91,6 +93,7 @@ typedef struct or_rbc {
 void or_sff_forward(const or_grid *g, int parity, const double *real_in, double *spec_out);
 void or_sff_inverse(const or_grid *g, int parity, const double *spec_in, double *real_out);
 void or_sff_dealias(const or_grid *g, double *spec);
+double or_get_dt(const or_grid *g, int sff, double *V[3], double courant, double dt_fixed, double *umax);
 void or_sff_reality(const or_grid *g, double *spec, int mode);
 double or_sff_energy(const or_grid *g, const double *A, const double *B);
 double or_sff_kmag(const or_grid *g, int lx, int ly, int lz);

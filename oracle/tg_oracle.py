@@ -29,7 +29,7 @@ class Grid(ctypes.Structure):
 class Fluid(ctypes.Structure):
     _fields_ = [("nu", ctypes.c_double), ("dt", ctypes.c_double), ("force_kind", ctypes.c_int),
                 ("force_eps", ctypes.c_double), ("force_r_in", ctypes.c_double),
-                ("force_r_out", ctypes.c_double)]
+                ("force_r_out", ctypes.c_double), ("nu_hyper", ctypes.c_double), ("hyper_exponent", ctypes.c_int)]
 
 
 class Rbc(ctypes.Structure):
@@ -98,6 +98,7 @@ def lib():
             "or_sff_forward": (None, [gp, i, p, p]),
             "or_sff_inverse": (None, [gp, i, p, p]),
             "or_sff_dealias": (None, [gp, p]),
+            "or_get_dt": (d, [gp, i, pp, d, d, p]),
             "or_sff_reality": (None, [gp, p, i]),
             "or_sff_energy": (d, [gp, p, p]),
             "or_sff_kmag": (d, [gp, i, i, i]),
@@ -440,6 +441,12 @@ class Oracle:
         out = np.zeros(self.rshape, np.float64)
         self.L.or_sff_inverse(ctypes.byref(self.g), parity, _cptr(spec), _ptr(out))
         return out
+
+    def get_dt(self, V, courant, dt_fixed, sff=False):
+        """FluidVF::Get_dt with dt_option 1 (dealiases V in place): (dt, umax[3])."""
+        um = np.zeros(3)
+        dt = self.L.or_get_dt(ctypes.byref(self.g), 1 if sff else 0, _pp(V), courant, dt_fixed, _ptr(um))
+        return dt, um
 
     def sff_dealias(self, A):
         self.L.or_sff_dealias(ctypes.byref(self.g), _cptr(A))

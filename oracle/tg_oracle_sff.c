@@ -386,3 +386,42 @@ double or_nusselt(const or_grid *g, const or_rbc *prm, const double *V1, const d
 {
     return 1.0 + 2.0 * sqrt(prm->Ra * prm->Pr) * or_sff_energy(g, V1, T);
 }
+
+/* FluidVF::Get_dt with dt_option 1 (FluidVF.cc:546-572), as Compute_nlin
+ * calls it (compute_nlin.cc:60-73): V dealiased in place, its real-space
+ * field (Inverse_transform), umax_i = max |u_i| over the grid
+ * (Get_local_max_real_space, FluidVF.cc:499-509), sum = umax_x / Dx +
+ * umax_z / Dz (+ umax_y / Dy for Ny > 1), dt = min(Courant / sum, dt_fixed),
+ * Dx_i = L_i / N_i with L_i = 2 pi / kfactor_i (FFF, FFF_pencil.cc:64-72)
+ * or L_x = pi / kfactor_x (SFF, SFF_pencil.cc:77-86).  sff: the SFF basis
+ * (u_x 'S', u_y, u_z 'C').  umax may be NULL. */
+double or_get_dt(const or_grid *g, int sff, double *V[3], double courant, double dt_fixed, double *umax)
+{
+    const size_t nr = (size_t)g->nx * g->ny * (g->nz + 2);
+    double *R = (double *)malloc(sizeof(double) * nr);
+    double um[3];
+    for (int c = 0; c < 3; c++) {
+        if (sff) {
+            or_sff_dealias(g, V[c]);
+            or_sff_inverse(g, c == 0 ? OR_SIN : OR_COS, V[c], R);
+        } else {
+            or_dealias(g, V[c]);
+            or_inverse(g, V[c], R);
+        }
+        double m = 0.0;
+        for (size_t r = 0; r < (size_t)g->nx * g->ny; r++)
+            for (int z = 0; z < g->nz; z++) {
+                const double v = fabs(R[r * (size_t)(g->nz + 2) + z]);
+                if (v > m || v != v) m = v;
+            }
+        um[c] = m;
+        if (umax) umax[c] = m;
+    }
+    free(R);
+    const double L[3] = {(sff ? M_PI : 2 * M_PI) / g->kfactor[0], 2 * M_PI / g->kfactor[1], 2 * M_PI / g->kfactor[2]};
+    const double dx = L[0] / g->nx, dy = L[1] / g->ny, dz = L[2] / g->nz;
+    double sum = um[0] / dx + um[2] / dz;
+    if (g->ny > 1) sum += um[1] / dy;
+    const double dt = courant / sum;
+    return dt < dt_fixed ? dt : dt_fixed;
+}

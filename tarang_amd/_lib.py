@@ -27,7 +27,7 @@ class TgConfig(ctypes.Structure):
 class TgFluid(ctypes.Structure):
     _fields_ = [("nu", ctypes.c_double), ("dt", ctypes.c_double), ("force_kind", ctypes.c_int),
                 ("force_eps", ctypes.c_double), ("force_r_in", ctypes.c_double),
-                ("force_r_out", ctypes.c_double)]
+                ("force_r_out", ctypes.c_double), ("nu_hyper", ctypes.c_double), ("hyper_exponent", ctypes.c_int)]
 
 
 BASIS_FFF, BASIS_SFF, BASIS_CHFF = 0, 1, 2
@@ -63,6 +63,7 @@ SIGNATURES = {
     "tg_plan_device_bytes": (ctypes.c_int64, [_P]),
     "tg_forward": (_I, [_P, _I, _P, _P]),
     "tg_inverse": (_I, [_P, _I, _P, _P]),
+    "tg_get_dt": (_I, [_P, _PP, _D, _D, _DP, _DP]),
     "tg_dealias": (_I, [_P, _PP, _I]),
     "tg_strong_reality": (_I, [_P, _PP, _I]),
     "tg_compute_nlin": (_I, [_P, _PP, _PP]),

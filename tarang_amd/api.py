@@ -210,13 +210,20 @@ class ChFF_Basis(FFF_Basis):
         check("tg_helmholtz", self.L.tg_helmholtz(self.plan, _vp(R), _vp(U), float(lambda_supplement), vp, vm))
 
 
+def _fluid(U, dt, force_kind=0, force_eps=0.0, force_r_in=0.0, force_r_out=0.0):
+    """tg_fluid of a FluidVF: dissipation, dt, forcing band, hyperviscosity."""
+    return TgFluid(U.dissipation_coefficient, float(dt), int(force_kind), float(force_eps), float(force_r_in),
+                   float(force_r_out), getattr(U, "hyper_dissipation_coefficient", 0.0),
+                   getattr(U, "hyper_dissipation_exponent", 0))
+
+
 class Time_advance_incompress_BDF1:
     """Time_advance_incompress::BDF1 on a ChFF basis (BDF1.cc:63-121): U.cvf
     advanced in place, the pressure in self.P."""
 
     def __init__(self, U, dt):
         self.U = U
-        self.fluid = TgFluid(U.dissipation_coefficient, float(dt), 0, 0.0, 0.0, 0.0)
+        self.fluid = _fluid(U, dt)
         self.P = U.basis.complex_array()
 
     def Compute_nlin(self):
@@ -266,10 +273,15 @@ class FluidSF:
 class FluidVF:
     """Solver-level vector field: cvf (V1, V2, V3), nlin1..3, dissipation."""
 
-    def __init__(self, basis, dissipation_coefficient=0.0, force_switch=False, name="U"):
+    def __init__(self, basis, dissipation_coefficient=0.0, force_switch=False, name="U",
+                 hyper_dissipation_coefficient=0.0, hyper_dissipation_exponent=0):
+        """hyper_dissipation_*: FluidVF.cc:72-84 (exponent 0: none); taken by
+        the RK4 step and the adjoint."""
         self.basis = basis
         self.name = name
         self.dissipation_coefficient = float(dissipation_coefficient)
+        self.hyper_dissipation_coefficient = float(hyper_dissipation_coefficient)
+        self.hyper_dissipation_exponent = int(hyper_dissipation_exponent)
         self.force_switch = force_switch
         self.cvf = [basis.complex_array() for _ in range(3)]
         self.nlin = [basis.complex_array() for _ in range(3)]
@@ -293,6 +305,19 @@ class FluidVF:
     def set_field(self, V):
         for dst, src in zip(self.cvf, V):
             dst.copy_(torch.as_tensor(src, dtype=torch.complex128))
+
+    def Get_dt(self, courant, dt_fixed, dt_option=1):
+        """FluidVF::Get_dt (FluidVF.cc:546-572): dt_fixed (dt_option 0), or the
+        CFL step min(Courant / sum_i max|u_i| / Delta_x_i, dt_fixed) of the
+        dealiased field (dt_option 1; dealiases cvf in place).  Returns dt."""
+        if dt_option == 0:
+            return float(dt_fixed)
+        b = self.basis
+        dt = ctypes.c_double()
+        um = (ctypes.c_double * 3)()
+        check("tg_get_dt", b.L.tg_get_dt(b.plan, self.ptrs(), float(courant), float(dt_fixed), ctypes.byref(dt), um))
+        self.umax = tuple(um)
+        return dt.value
 
     def Init_cond_spectrum(self, kind=0, k0=4.0, kmin=2.0, kmax=170.0, energy=0.5, seed=1):
         """Seeded solenoidal field (kind 0: K^4 exp(-2(K/k0)^2); kind 1: K^-5/3)."""
@@ -409,8 +434,7 @@ class Time_advance_incompress:
 
     def __init__(self, U, dt, force_kind=0, force_eps=0.0, force_r_in=0.0, force_r_out=0.0):
         self.U = U
-        self.fluid = TgFluid(U.dissipation_coefficient, float(dt), int(force_kind), float(force_eps),
-                             float(force_r_in), float(force_r_out))
+        self.fluid = _fluid(U, dt, force_kind, force_eps, force_r_in, force_r_out)
         self._continue = False
         self._versions = None
 
@@ -448,7 +472,7 @@ class Time_advance_incompress_MHD:
 
     def __init__(self, U, W, dt):
         self.U, self.W = U, W
-        self.fluid = TgFluid(U.dissipation_coefficient, float(dt), 0, 0.0, 0.0, 0.0)
+        self.fluid = _fluid(U, dt)
 
     def Time_advance_step(self, nsteps=1):
         b = self.U.basis
@@ -495,7 +519,7 @@ class Adjoint_incompress:
 
     def __init__(self, U, dt, checkpoint_every=20):
         self.U = U
-        self.fluid = TgFluid(U.dissipation_coefficient, float(dt), 0, 0.0, 0.0, 0.0)
+        self.fluid = _fluid(U, dt)
         self.checkpoint_every = int(checkpoint_every)
 
     def Forward(self, nsteps):

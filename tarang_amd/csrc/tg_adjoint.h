@@ -18,6 +18,8 @@ enum { ADJ_RHS = -1, ADJ_S0 = 0, ADJ_S1 = 1, ADJ_S2 = 2, ADJ_S3 = 3 };
 struct AdjParams {
     int mode;
     double dt, fh, f1;  // fh = -nu*dt/2, f1 = -nu*dt
+    double fhh, f1h;    // the same with nu_hyper (hyperviscosity)
+    int hq;             // hyper_exponent q (0: none)
     double scale;       // 1/(Nx Ny Nz)
 };
 

@@ -256,20 +256,20 @@ __global__ void __launch_bounds__(CB *(L / 8)) k_xfwd_adj(XadjArgs a)
                 const double ks = (Kx * Kx + Ky * Ky) + Kz * Kz;
                 cpx<double> nx;
                 if (ap.mode == ADJ_S3) {
-                    const double Eh = exp(ap.fh * ks);
+                    const double Eh = visc_exp(ap.fh, ap.fhh, ap.hq, ks);
                     const cpx<double> lv = c_sixth * r;
                     acc[i] = (Eh * Eh) * lv;
                     nx = c_third * (Eh * l4[ii]) + ap.dt * (Eh * lv);
                 } else if (ap.mode == ADJ_S2) {
-                    const double Eh = exp(ap.fh * ks);
+                    const double Eh = visc_exp(ap.fh, ap.fhh, ap.hq, ks);
                     acc[i] = a4[ii] + Eh * r;
                     nx = c_third * (Eh * l4[ii]) + c_half * r;
                 } else if (ap.mode == ADJ_S1) {
-                    const double Eh = exp(ap.fh * ks), E1 = exp(ap.f1 * ks);
+                    const double Eh = visc_exp(ap.fh, ap.fhh, ap.hq, ks), E1 = visc_exp(ap.f1, ap.f1h, ap.hq, ks);
                     acc[i] = a4[ii] + Eh * r;
                     nx = c_sixth * (E1 * l4[ii]) + c_half * (Eh * r);
                 } else {
-                    const double E1 = exp(ap.f1 * ks);
+                    const double E1 = visc_exp(ap.f1, ap.f1h, ap.hq, ks);
                     nx = (E1 * l4[ii] + a4[ii]) + r;
                     lam[i] = nx;
                 }

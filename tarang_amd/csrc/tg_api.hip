@@ -69,6 +69,8 @@ struct tg_plan {
     size_t partial_bytes = 0;
     double *small = nullptr;  // device scratch for reduction outputs / radii
     int red_blocks = 0;
+    // tg_get_dt: one real field (Nx, Ny, Nz+2) and the three maxima, on first use
+    double *rfield = nullptr, *umax_dev = nullptr;
     // chaining of RK4 steps: valid for the same state pointers and band
     bool unext_valid = false;
     double *unext_V[3] = {};
@@ -219,7 +221,8 @@ void free_plan(tg_plan *p)
         if (q) (void)hipFree(q);
     for (void *q : p->mhd)
         if (q) (void)hipFree(q);
-    void *aptrs[] = {p->adj.ckpt, p->adj.tape, p->adj.vw, p->adj.lg, p->adj.acc, p->nonfinite_dev};
+    void *aptrs[] = {p->adj.ckpt, p->adj.tape, p->adj.vw, p->adj.lg, p->adj.acc, p->nonfinite_dev, p->rfield,
+                     p->umax_dev};
     for (void *q : aptrs)
         if (q) (void)hipFree(q);
     void *cptrs[] = {p->ch.tmp, p->ch.D, p->ch.S, p->ch.NL[0], p->ch.NL[1], p->ch.NL[2], p->ch.R[0], p->ch.R[1],
@@ -352,6 +355,18 @@ int ns_middle(tg_plan *p)
     return TG_OK;
 }
 
+// Hyperviscosity parameters: q = 0 (none) or 2..16 with nu_hyper >= 0;
+// `allowed` false for the steps that take none (MHD, BDF1).
+int check_hyper(const tg_fluid *f, bool allowed)
+{
+    if (f->hyper_exponent == 0) return TG_OK;
+    if (!allowed) return fail(TG_ERR_ARG, "hyperviscosity (hyper_exponent %d) is taken by tg_rk4_step and the "
+                                          "adjoint only", f->hyper_exponent);
+    if (f->hyper_exponent < 2 || f->hyper_exponent > 16 || !(f->nu_hyper >= 0))
+        return fail(TG_ERR_ARG, "need hyper_exponent 0 or 2..16 and nu_hyper >= 0");
+    return TG_OK;
+}
+
 RkParams rk_params(const tg_plan *p, const tg_fluid *f, int stage)
 {
     RkParams rk{};
@@ -362,6 +377,10 @@ RkParams rk_params(const tg_plan *p, const tg_fluid *f, int stage)
     // Mult_field_exp_ksqr_dt factor = -dissipation_coefficient*a*dt (FluidVF.cc:342)
     rk.fh = -nu * 0.5 * dt;
     rk.f1 = -nu * 1.0 * dt;
+    // and hyper_factor = -hyper_dissipation_coefficient*a*dt (FluidVF.cc:350-355)
+    rk.hq = f ? f->hyper_exponent : 0;
+    rk.fhh = f ? -f->nu_hyper * 0.5 * dt : 0.0;
+    rk.f1h = f ? -f->nu_hyper * 1.0 * dt : 0.0;
     rk.scale = 1.0 / ((double)p->g.nx * p->g.ny * p->g.nz);
     return rk;
 }
@@ -680,6 +699,9 @@ AdjParams adj_params(const tg_plan *p, const tg_fluid *f, int mode)
     ap.dt = f ? f->dt : 0.0;
     ap.fh = f ? -f->nu * 0.5 * f->dt : 0.0;
     ap.f1 = f ? -f->nu * 1.0 * f->dt : 0.0;
+    ap.hq = f ? f->hyper_exponent : 0;
+    ap.fhh = f ? -f->nu_hyper * 0.5 * f->dt : 0.0;
+    ap.f1h = f ? -f->nu_hyper * 1.0 * f->dt : 0.0;
     ap.scale = 1.0 / ((double)p->g.nx * p->g.ny * p->g.nz);
     return ap;
 }
@@ -1167,6 +1189,46 @@ int tg_inverse(tg_plan *plan, int sincos_code, const double *spec_dev, double *r
     return DISPATCH_PREC(plan, run_inverse)(plan, spec_dev, real_dev, sincos_code);
 }
 
+int tg_get_dt(tg_plan *plan, double *const V_dev[3], double courant, double dt_fixed, double *dt, double umax[3])
+{
+    if (!plan || !dt) return fail(TG_ERR_ARG, "null argument");
+    if (int rb_ = not_chff(plan, "tg_get_dt")) return rb_;
+    if (!(courant > 0) || !(dt_fixed > 0)) return fail(TG_ERR_ARG, "need courant > 0 and dt_fixed > 0");
+    int rc = check_ptrs((const void *const *)V_dev, 3, "V");
+    if (!rc) rc = set_device(plan);
+    if (rc) return rc;
+    tg_plan *p = plan;
+    if (!p->rfield) rc = dmalloc(p, (void **)&p->rfield, (size_t)p->g.nx * p->g.ny * (p->g.nz + 2) * 8);
+    if (!rc && !p->umax_dev) rc = dmalloc(p, (void **)&p->umax_dev, 256);
+    if (rc) return rc;
+    // Compute_nlin dealiases U.cvf before its Inverse_transform (compute_nlin.cc:60-64)
+    if (p->g.dealias) HIPCHK(launch_dealias(p->g, V_dev, 3, p->stream));
+    p->unext_valid = false;
+    const bool sff = p->g.basis == 1;
+    for (int c = 0; c < 3; c++) {
+        const int sc = sff ? (c == 0 ? TG_SC_SIN : TG_SC_COS) : 0;
+        rc = DISPATCH_PREC(p, run_inverse)(p, V_dev[c], p->rfield, sc);
+        if (rc) return rc;
+        HIPCHK(launch_max_abs_real(p->g, p->rfield, p->umax_dev + c, p->stream));
+    }
+    double um[3];
+    HIPCHK(hipMemcpyAsync(um, p->umax_dev, sizeof(um), hipMemcpyDeviceToHost, p->stream));
+    HIPCHK(hipStreamSynchronize(p->stream));
+    if (umax)
+        for (int c = 0; c < 3; c++) umax[c] = um[c];
+    for (int c = 0; c < 3; c++)
+        if (!std::isfinite(um[c])) return fail(TG_ERR_NAN, "tg_get_dt: the real-space velocity is not finite");
+    // Delta_x_i = L_i / N_i, L_i = 2 pi / kfactor_i (FFF_pencil.cc:64-72; SFF L_x = pi / kfactor_x,
+    // SFF_pencil.cc:77-86)
+    const double L[3] = {(sff ? M_PI : 2 * M_PI) / p->g.kf[0], 2 * M_PI / p->g.kf[1], 2 * M_PI / p->g.kf[2]};
+    const double dx = L[0] / p->g.nx, dy = L[1] / p->g.ny, dz = L[2] / p->g.nz;
+    double sum = um[0] / dx + um[2] / dz;
+    if (p->g.ny > 1) sum += um[1] / dy;
+    const double d = courant / sum;
+    *dt = d < dt_fixed ? d : dt_fixed;
+    return TG_OK;
+}
+
 int tg_dealias(tg_plan *plan, double *const V_dev[], int ncomp)
 {
     if (!plan || ncomp < 1 || ncomp > 3) return fail(TG_ERR_ARG, "bad argument");
@@ -1305,6 +1367,7 @@ int tg_rk4_step(tg_plan *plan, double *const V_dev[3], const tg_fluid *fluid, in
     if (int rb_ = need_basis(plan, 0, "tg_rk4_step")) return rb_;
     if (nsteps < 0) return fail(TG_ERR_ARG, "nsteps must be >= 0");
     if (!(fluid->dt > 0) || !(fluid->nu >= 0)) return fail(TG_ERR_ARG, "need dt > 0 and nu >= 0");
+    if (int rh_ = check_hyper(fluid, true)) return rh_;
     int rc = check_ptrs((const void *const *)V_dev, 3, "V");
     if (!rc) rc = set_device(plan);
     if (rc) return rc;
@@ -1561,6 +1624,7 @@ int tg_rk4_step_mhd(tg_plan *plan, double *const V_dev[3], double *const W_dev[3
     if (int rb_ = need_basis(plan, 0, "tg_rk4_step_mhd")) return rb_;
     if (nsteps < 0) return fail(TG_ERR_ARG, "nsteps must be >= 0");
     if (!(fluid->dt > 0) || !(fluid->nu >= 0) || !(eta >= 0)) return fail(TG_ERR_ARG, "need dt > 0, nu, eta >= 0");
+    if (int rh_ = check_hyper(fluid, false)) return rh_;
     if (fluid->force_kind != 0) return fail(TG_ERR_ARG, "forcing is not supported for MHD");
     int rc = check_ptrs((const void *const *)V_dev, 3, "V");
     if (!rc) rc = check_ptrs((const void *const *)W_dev, 3, "W");
@@ -1643,6 +1707,7 @@ int tg_adjoint_forward(tg_plan *plan, double *const V_dev[3], const tg_fluid *fl
     if (int rb_ = need_basis(plan, 0, "tg_adjoint_forward")) return rb_;
     if (nsteps < 1 || checkpoint_every < 1) return fail(TG_ERR_ARG, "need nsteps >= 1 and checkpoint_every >= 1");
     if (!(fluid->dt > 0) || !(fluid->nu >= 0)) return fail(TG_ERR_ARG, "need dt > 0 and nu >= 0");
+    if (int rh_ = check_hyper(fluid, true)) return rh_;
     if (fluid->force_kind != 0) return fail(TG_ERR_ARG, "the adjoint supports force_kind 0 only");
     if (plan->cfg.strong_reality) return fail(TG_ERR_ARG, "the adjoint needs a plan with strong_reality 0");
     int rc = check_ptrs((const void *const *)V_dev, 3, "V");
@@ -1863,6 +1928,7 @@ int tg_bdf1_step(tg_plan *plan, double *const V_dev[3], double *P_dev, const tg_
     if (!plan || !P_dev || !fluid || nsteps < 0) return fail(TG_ERR_ARG, "bad argument");
     if (int rb_ = need_basis(plan, TG_BASIS_CHFF, "tg_bdf1_step")) return rb_;
     if (!(fluid->dt > 0) || !(fluid->nu > 0)) return fail(TG_ERR_ARG, "need dt > 0 and nu > 0");
+    if (int rh_ = check_hyper(fluid, false)) return rh_;
     if (fluid->force_kind != 0) return fail(TG_ERR_ARG, "forcing is not supported for ChFF");
     if (plan->g.nx < 4) return fail(TG_ERR_SHAPE, "tg_bdf1_step needs Nx >= 4");
     int rc = check_ptrs((const void *const *)V_dev, 3, "V");

@@ -41,5 +41,8 @@ hipError_t launch_reduce(const Grid &g, int mode, const double *const *A, const 
 hipError_t launch_init_field(const Grid &g, double *const *V, int kind, double k0, double kmin, double kmax,
                              uint64_t seed, hipStream_t st);
 hipError_t launch_scale_fields(const Grid &g, double *const *V, const double *target_and_e, hipStream_t st);
+// out[0] = max |R| over the Nx*Ny rows of Nz samples (row stride Nz + 2) of a
+// real field; a NaN sample gives NaN.  out: 8 bytes of device memory.
+hipError_t launch_max_abs_real(const Grid &g, const double *R, double *out, hipStream_t st);
 
 }  // namespace tg

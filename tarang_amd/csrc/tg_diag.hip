@@ -7,8 +7,11 @@
 // tree and added with Neumaier compensation), merges the wavefront bins in a
 // fixed order and writes one partial per bin; a second kernel sums the
 // partials of each bin in block order with Neumaier compensation.
+#include <algorithm>
+
 #include "tg_device.h"
 #include "tg_kernels.h"
+
 #include "tg_diag.h"
 #include "tg_aniso.h"
 
@@ -476,6 +479,37 @@ hipError_t launch_scale_fields(const Grid &g, double *const *V, const double *ta
 {
     const size_t n = (size_t)g.nx * g.ny * g.nzc * 2;
     hipLaunchKernelGGL(k_scale_fields, dim3(4096), dim3(256), 0, st, V[0], V[1], V[2], n, target_and_e);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Maximum of |u| over a real field (Get_local_max_real_space, FluidVF.cc:
+// 499-509).  The maximum of non-negative doubles is the maximum of their bit
+// patterns as unsigned integers (NaN above every number), so each wave
+// reduces with shuffles and adds its maximum with one integer atomic max: the
+// result does not depend on the order.
+// ---------------------------------------------------------------------------
+__global__ void k_zero_u64(unsigned long long *p) { *p = 0ull; }
+
+__device__ __forceinline__ double max_nan(double a, double b) { return (b > a || b != b) ? b : a; }
+
+__global__ void k_max_abs_real(const double *R, size_t nrows, int nz, int stride, unsigned long long *out)
+{
+    double m = 0.0;
+    const size_t n = nrows * (size_t)nz;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        m = max_nan(m, fabs(R[(i / nz) * stride + i % nz]));
+    for (int o = 32; o > 0; o >>= 1) m = max_nan(m, __shfl_xor(m, o));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(m));
+}
+
+hipError_t launch_max_abs_real(const Grid &g, const double *R, double *out, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, st, (unsigned long long *)out);
+    const size_t nrows = (size_t)g.nx * g.ny;
+    const int blocks = (int)std::min<size_t>(4096, (nrows * g.nz + 255) / 256);
+    hipLaunchKernelGGL(k_max_abs_real, dim3(blocks), dim3(256), 0, st, R, nrows, g.nz, g.nz + 2,
+                       (unsigned long long *)out);
     return hipGetLastError();
 }
 

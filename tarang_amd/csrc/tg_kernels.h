@@ -71,6 +71,8 @@ struct RkParams {
                           // form (9 products u_j w_i), nonlinear term only
     int write_rhs;        // stage 0: apply force/projection/negation (Compute_rhs)
     double dt, fh, f1;    // fh = -nu*0.5*dt, f1 = -nu*1.0*dt
+    double fhh, f1h;      // the same with nu_hyper (hyperviscosity)
+    int hq;               // hyper_exponent q (0: none)
     double scale;         // 1/(Nx Ny Nz)
     int *nonfinite;       // stage 4: set to 1 if the new state holds a
                           // non-finite value (null: not checked)

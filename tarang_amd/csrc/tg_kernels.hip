@@ -592,14 +592,14 @@ __global__ void __launch_bounds__(CB *(L / 8), 2) k_x_nlin(XnlArgs a)
                 const double Kx = kwave(lx, g.nx) * g.kf[0];
                 const double kxs = Kx * Kx;
                 const double ks = (kxs + Ky * Ky) + Kz * Kz;
-                const double Eh = exp(a.rk.fh * ks);
+                const double Eh = visc_exp(a.rk.fh, a.rk.fhh, a.rk.hq, ks);
                 const cpx<double> r = cvt<double>(N[K][s]);
                 const cpx<double> v0 = dma_tile_read<double, L, CB>(X, lx, c);
                 cpx<double> vn, ac;
                 if (stage == 1) {
                     cpx<double> d0 = v0;
                     if (dealias_zero(g, lx, ly, lz)) d0 = mk<double>(0, 0);
-                    const double E1 = exp(a.rk.f1 * ks);
+                    const double E1 = visc_exp(a.rk.f1, a.rk.f1h, a.rk.hq, ks);
                     ac = c_sixth * (E1 * r);
                     vn = Eh * (d0 + c_half * r);
                 } else if (stage == 2) {
@@ -610,7 +610,7 @@ __global__ void __launch_bounds__(CB *(L / 8), 2) k_x_nlin(XnlArgs a)
                     vn = Eh * (Eh * v0 + c_one * r);
                 } else {
                     ac = Yd[li] + c_sixth * r;
-                    const double E1 = exp(a.rk.f1 * ks);
+                    const double E1 = visc_exp(a.rk.f1, a.rk.f1h, a.rk.hq, ks);
                     vn = E1 * v0 + ac;
                 }
                 Yd[li] = mk<double>(stage == 4 ? vn.x : ac.x, stage == 4 ? vn.y : ac.y);
@@ -657,14 +657,14 @@ __global__ void __launch_bounds__(CB *(L / 8), 2) k_x_nlin(XnlArgs a)
             const double Kx = kwave(lx, g.nx) * g.kf[0];
             const double kxs = Kx * Kx;
             const double ks = (kxs + Ky * Ky) + Kz * Kz;
-            const double Eh = exp(a.rk.fh * ks);
+            const double Eh = visc_exp(a.rk.fh, a.rk.fhh, a.rk.hq, ks);
             const cpx<double> r = cvt<double>(nc[s]);
             const cpx<double> v0 = ldd[li];
             cpx<double> vn, ac;
             if (stage == 1) {
                 cpx<double> d0 = v0;
                 if (dealias_zero(g, lx, ly, lz)) d0 = mk<double>(0, 0);
-                const double E1 = exp(a.rk.f1 * ks);
+                const double E1 = visc_exp(a.rk.f1, a.rk.f1h, a.rk.hq, ks);
                 ac = c_sixth * (E1 * r);
                 vn = Eh * (d0 + c_half * r);
             } else if (stage == 2) {
@@ -675,7 +675,7 @@ __global__ void __launch_bounds__(CB *(L / 8), 2) k_x_nlin(XnlArgs a)
                 vn = Eh * (Eh * v0 + c_one * r);
             } else {
                 ac = ldb[li] + c_sixth * r;
-                const double E1 = exp(a.rk.f1 * ks);
+                const double E1 = visc_exp(a.rk.f1, a.rk.f1h, a.rk.hq, ks);
                 vn = E1 * v0 + ac;
             }
             // accumulator (stages 1-3) or the new state (stage 4) back to B

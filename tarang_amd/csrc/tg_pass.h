@@ -45,6 +45,19 @@ __device__ __forceinline__ bool dealias_zero(const Grid &g, int lx, int ly, int 
     return false;
 }
 
+// Viscous factor exp(exponent) of Array_exp_ksqr: exponent f K^2, or with
+// hyperviscosity (f + fh K^(q-2)) K^2 (universal_basic.cc:265-291), K^(q-2) =
+// (K^2)^((q-2)/2) with integer division (q = 4: K^2), by repeated
+// multiplication; q = 0: none.  ks = K^2.
+__device__ __forceinline__ double visc_exp(double f, double fh, int q, double ks)
+{
+    if (q == 0) return exp(f * ks);
+    const int n = q == 4 ? 1 : (q - 2) / 2;
+    double kp = 1.0;
+    for (int i = 0; i < n; i++) kp *= ks;
+    return exp((f + fh * kp) * ks);
+}
+
 template <typename T>
 __host__ __device__ constexpr int col_block(int L, int elem_bytes)
 {

@@ -270,6 +270,76 @@ def test_rk4_single_shear_mode_decays_exactly():
     assert np.abs(V[0]).max() < 1e-16 and np.abs(V[2]).max() < 1e-16
 
 
+@pytest.mark.parametrize("q", [2, 3, 4, 6, 8])
+def test_rk4_single_shear_mode_hyperviscous_decay(q):
+    """With hyperviscosity (FluidVF hyper_dissipation, Array_exp_ksqr with
+    hyper_factor) a single shear mode decays as exp(-(nu K^2 + nu_h K^2
+    (K^2)^p) t), p = (q-2)/2 with integer division (q = 4: p = 1); a
+    second mode at another |K| checks the K dependence."""
+    o = to.Oracle(16, dealias=to.DEALIAS_REF_CORNER)
+    X, Y, Z = xyz(o)
+    u = [0 * X, 0.3 * np.cos(2 * X + Z) + 0.1 * np.cos(3 * Z), 0 * X]
+    V = [o.forward(real_of(o, f)) for f in u]
+    V0 = [v.copy() for v in V]
+    nu, nuh, dt, n = 0.05, 2e-3, 1e-2, 20
+    fl = to.Fluid(nu, dt, 0, 0, 0, 0, nuh, q)
+    for _ in range(n):
+        o.rk4_step(fl, V)
+    p = 1 if q == 4 else (q - 2) // 2
+    t = n * dt
+    for (lx, ly, lz), K2 in (((2, 0, 1), 5.0), ((0, 0, 3), 9.0)):
+        decay = np.exp(-(nu + nuh * K2 ** p) * K2 * t)
+        assert abs(V[1][lx, ly, lz] - decay * V0[1][lx, ly, lz]) <= 1e-15 * abs(V0[1][lx, ly, lz]) + 1e-17
+    # nu_h = 0 is the plain scheme, bit for bit
+    W, W0 = [v.copy() for v in V0], [v.copy() for v in V0]
+    o.rk4_step(to.Fluid(nu, dt, 0, 0, 0, 0, 0.0, q), W)
+    o.rk4_step(to.Fluid(nu, dt, 0, 0, 0, 0), W0)
+    assert all(np.array_equal(a, b) for a, b in zip(W, W0))
+
+
+@pytest.mark.parametrize("kf", [(1.0, 1.0, 1.0), (1.0, 0.5, 2.0)])
+def test_get_dt_cfl_closed_form(kf):
+    """FluidVF::Get_dt, dt_option 1: the real-space maxima of single modes
+    are their amplitudes (attained at grid points), Delta_x_i = 2 pi /
+    (kfactor_i N_i), dt = min(Courant / sum umax_i / Delta_x_i, dt_fixed)."""
+    o = to.Oracle((16, 8, 32), kfactor=kf, dealias=to.DEALIAS_REF_CORNER)
+    x = 2 * np.pi * np.arange(16) / 16
+    y = 2 * np.pi * np.arange(8) / 8
+    z = 2 * np.pi * np.arange(32) / 32
+    X, Y, Z = np.meshgrid(x, y, z, indexing="ij")
+    u = [0.5 * np.cos(X), 0.3 * np.cos(2 * X + Z), -0.2 * np.cos(Y + 3 * Z)]
+    V = [o.forward(real_of(o, f)) for f in u]
+    dx = [2 * np.pi / kf[i] / n for i, n in enumerate((16, 8, 32))]
+    ref = 0.4 / (0.5 / dx[0] + 0.3 / dx[1] + 0.2 / dx[2])
+    dt, um = o.get_dt([v.copy() for v in V], 0.4, 1.0)
+    assert np.allclose(um, [0.5, 0.3, 0.2], rtol=1e-14, atol=0)
+    assert abs(dt - ref) <= 1e-14 * ref
+    dt, _ = o.get_dt([v.copy() for v in V], 0.4, 0.5 * ref)      # capped by dt_fixed
+    assert dt == 0.5 * ref
+
+
+def test_get_dt_sff_grid_maxima():
+    """SFF: L_x = pi / kfactor_x; the maxima over the cell-centred grid of
+    fields built in real space (their transforms are pinned elsewhere)."""
+    kf = (np.pi, np.pi / np.sqrt(2.0), np.pi / np.sqrt(2.0))
+    N = (16, 8, 16)
+    o = to.Oracle(N, kfactor=kf, dealias=to.DEALIAS_NONE, strong_reality=False)
+    Lx = np.pi / kf[0]
+    x = (np.arange(16) + 0.5) * Lx / 16
+    y = 2 * np.pi / kf[1] * np.arange(8) / 8
+    z = 2 * np.pi / kf[2] * np.arange(16) / 16
+    X, Y, Z = np.meshgrid(x, y, z, indexing="ij")
+    u = [0.7 * np.sin(kf[0] * X) * np.cos(kf[2] * Z), 0.4 * np.cos(2 * kf[0] * X), 0.25 * np.cos(kf[1] * Y)]
+    par = [to.SIN, to.COS, to.COS]
+    V = [o.sff_forward(par[c], real_of(o, u[c])) for c in range(3)]
+    dt, um = o.get_dt(V, 0.3, 10.0, sff=True)
+    umax = [np.abs(f).max() for f in u]
+    assert np.allclose(um, umax, rtol=1e-13, atol=0)
+    dx = [Lx / 16, 2 * np.pi / kf[1] / 8, 2 * np.pi / kf[2] / 16]
+    ref = 0.3 / (umax[0] / dx[0] + umax[1] / dx[1] + umax[2] / dx[2])
+    assert abs(dt - ref) <= 1e-13 * ref
+
+
 def test_rk4_fourth_order_convergence():
     """Lawson RK4 is fourth order; with the strong reality condition on, the
     per-step zeroing of the kz = Nz/2 plane (FFF_pencil_basic.cc:175-177)

@@ -88,14 +88,16 @@ def test_dot_product_test_detects_a_dropped_weight():
     assert abs(lhs - dot(wrong, v)) > 1e-3 * abs(lhs)
 
 
-def test_rk4_step_adjoint_vs_central_difference():
+@pytest.mark.parametrize("hyper", [(0.0, 0), (2e-4, 4), (1e-5, 6)])
+def test_rk4_step_adjoint_vs_central_difference(hyper):
+    """Without and with hyperviscosity (nu_h, q)."""
     o = to.Oracle(16, dealias=to.DEALIAS_REF_CORNER, strong_reality=False)
-    fl = to.Fluid(0.01, 0.01, 0, 0.0, 0.0, 0.0)
+    fl = to.Fluid(0.01, 0.01, 0, 0.0, 0.0, 0.0, *hyper)
     rng = np.random.default_rng(11)
     V = o.init_field(kind=0, k0=3.0, seed=3)
     L, dV = rnd(o, rng), rnd(o, rng)
     g = o.rk4_step_vjp(fl, V, L)
-    h = 1e-5
+    h = 3e-6
     fd = dot(L, [(a - b) / (2 * h) for a, b in zip(rk4(o, fl, [v + h * d for v, d in zip(V, dV)]),
                                                     rk4(o, fl, [v - h * d for v, d in zip(V, dV)]))])
     assert abs(fd - dot(g, dV)) <= 1e-9 * abs(fd)
