@@ -74,7 +74,7 @@ typedef struct tg_config {
 typedef struct tg_fluid {
     double nu;
     double dt;
-    int force_kind;     /* 0 none, 1 constant energy supply alpha-forcing */
+    int force_kind;     /* 0 none, 1 constant energy supply alpha-forcing, 2 random band forcing */
     double force_eps;   /* energy injection rate */
     double force_r_in;  /* forcing band (r_in, r_out] */
     double force_r_out;
@@ -86,6 +86,21 @@ typedef struct tg_fluid {
      * steps return TG_ERR_ARG for q != 0. */
     double nu_hyper;
     int hyper_exponent;
+    /* force_kind 2, random band forcing (Compute_force_using_random_noise_basic,
+     * compute_force_random.cc:65-130): on every mode of (r_in, r_out],
+     * helical amplitudes sqrt((e_k +- h_k/K) / force_tau) with phases drawn
+     * from force_seed (own counter-based generator), through the Craya basis
+     * to Cartesian components about z (force_amp_phase.cc:119-170,
+     * FFF_pencil_inline.h:234-248); e_k = force_eps / n / modes_in_shell(i),
+     * h_k likewise from force_helicity, i = ceil(K), n = the number of shells
+     * floor(r_in)+1 .. ceil(r_out) (universal_basic.cc:104-143).  Needs
+     * force_tau > 0 and |force_helicity| <= r_in force_eps.  The force is a
+     * function of force_seed: the caller changes the seed every random
+     * interval (the reference reseeds when the time passes random_next,
+     * compute_force_random.cc:143-167). */
+    double force_helicity;
+    double force_tau;
+    unsigned long long force_seed;
 } tg_fluid;
 
 /* Rayleigh-Benard parameters, Pr_option PRLARGE with Uscaling ULARGE

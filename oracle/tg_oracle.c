@@ -505,11 +505,84 @@ void or_compute_nlin_giver(const or_grid *g, double *V[3], double *W[3], double 
  * 62-149) with one supply shell and Const_energy_supply_alpha_beta
  * (force_alpha_beta.cc:56-94, beta = 0).  E_band uses the energy weights of
  * Get_local_energy (FFF_pencil_energy.cc:59-67). */
+static inline double uniform01(uint64_t seed, uint64_t ctr);
+
+/* Universal::Compute_modes_in_shell_array (universal_basic.cc:104-143): the
+ * number of modes of the full box, kx in (-Nx/2, Nx/2] (likewise ky, kz),
+ * with ceil(K) = i, for i = 0..imax (only |k_d| <= imax / kfactor_d can
+ * reach those shells). */
+void or_modes_in_shell(const or_grid *g, int imax, double *count)
+{
+    for (int i = 0; i <= imax; i++) count[i] = 0;
+    const int n[3] = {g->nx, g->ny, g->nz};
+    int lim[3];
+    for (int d = 0; d < 3; d++) {
+        lim[d] = (int)ceil(imax / g->kfactor[d]) + 1;
+        if (lim[d] > n[d] / 2) lim[d] = n[d] / 2;
+    }
+    for (int kx = -lim[0]; kx <= lim[0]; kx++)
+        for (int ky = -lim[1]; ky <= lim[1]; ky++)
+            for (int kz = -lim[2]; kz <= lim[2]; kz++) {
+                if (kx <= -n[0] / 2 || ky <= -n[1] / 2 || kz <= -n[2] / 2) continue;
+                const double a = kx * g->kfactor[0], b = ky * g->kfactor[1], c = kz * g->kfactor[2];
+                const int idx = (int)ceil(sqrt(a * a + b * b + c * c));
+                if (idx <= imax) count[idx] += 1;
+            }
+}
+
+/* Random band forcing: Compute_force_using_random_noise_basic
+ * (compute_force_random.cc:65-130) with Put_or_add_force_vector
+ * (force_amp_phase.cc:119-170), Helical_to_Craya and Craya_to_cartesian
+ * (FFF_pencil_inline.h:234-248) about the z axis (anisotropy_dirn 3).  On
+ * every stored mode with r_in < K <= r_out: e_k = supply_e(i) /
+ * modes_in_shell(i), h_k likewise, i = ceil(K); F+- = sqrt((e_k +- h_k/K) /
+ * tau) exp(i phi+-); U1 = i (F- - F+), U2 = F- + F+; F = (vh1, vh2, vpll)
+ * with vpll = -U2 sin(theta), vh1 = U1 sin(phi) + U2 cos(theta) cos(phi),
+ * vh2 = -U1 cos(phi) + U2 sin(phi) cos(theta).  The supply spectra, which
+ * the reference reads from a file, are here the rates force_eps and
+ * force_helicity split evenly over the shells i = floor(r_in)+1 ..
+ * ceil(r_out); the phases phi+- = 2 pi U(seed, 2 m), 2 pi U(seed, 2 m + 1),
+ * m the mode's index in the (Nx, Ny, Nz/2+1) layout, from the counter-based
+ * generator of the seeded initial field (the reference's Blitz generator
+ * is seeded by my_id * time(0), compute_force_random.cc:143-167). */
+static void random_force(const or_grid *g, const or_fluid *f, double *F[3])
+{
+    const int nzc = g->nz / 2 + 1;
+    const int imax = (int)ceil(f->force_r_out), i0 = (int)floor(f->force_r_in) + 1;
+    const int nsh = imax - i0 + 1;
+    if (nsh < 1) return;
+    double *cnt = (double *)calloc((size_t)imax + 1, sizeof(double));
+    or_modes_in_shell(g, imax, cnt);
+    for (int lx = 0; lx < g->nx; lx++)
+        for (int ly = 0; ly < g->ny; ly++)
+            for (int lz = 0; lz < nzc; lz++) {
+                const double K = or_kmag(g, lx, ly, lz);
+                if (!(K > f->force_r_in && K <= f->force_r_out)) continue;
+                const int i = (int)ceil(K);
+                const double e = f->force_eps / nsh / cnt[i], h = f->force_helicity / nsh / cnt[i];
+                const double ap = sqrt((e + h / K) / f->force_tau), am = sqrt((e - h / K) / f->force_tau);
+                const size_t m = cidx(g, lx, ly, lz);
+                const double php = 2 * M_PI * uniform01(f->force_seed, 2 * (uint64_t)m);
+                const double phm = 2 * M_PI * uniform01(f->force_seed, 2 * (uint64_t)m + 1);
+                const cplx Fp = ap * (cos(php) + I * sin(php)), Fm = am * (cos(phm) + I * sin(phm));
+                const cplx U1 = I * (Fm - Fp), U2 = Fm + Fp;
+                const double th = or_polar_angle(g, lx, ly, lz, 3), ph = or_azimuthal_angle(g, lx, ly, lz, 3);
+                ((cplx *)F[0])[m] = U1 * sin(ph) + U2 * cos(th) * cos(ph);
+                ((cplx *)F[1])[m] = -U1 * cos(ph) + U2 * sin(ph) * cos(th);
+                ((cplx *)F[2])[m] = -U2 * sin(th);
+            }
+    free(cnt);
+}
+
 void or_compute_force(const or_grid *g, const or_fluid *f, double *const V[3], double *F[3])
 {
     const size_t nc = ncplx(g);
     for (int c = 0; c < 3; c++) memset(F[c], 0, sizeof(cplx) * nc);
     if (f->force_kind == 0) return;
+    if (f->force_kind == 2) {
+        random_force(g, f, F);
+        return;
+    }
     const int nzc = g->nz / 2 + 1;
     double eband = 0.0;
     for (int lx = 0; lx < g->nx; lx++)

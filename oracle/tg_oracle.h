@@ -50,6 +50,11 @@ typedef struct or_fluid {
     double force_r_out;
     double nu_hyper;     /* hyper_dissipation_coefficient */
     int hyper_exponent;  /* q: exp((-nu - nu_hyper K^(q-2)) K^2 a dt); 0 = none */
+    /* force_kind 2, random band forcing: helicity supply rate, random
+     * interval tau, the seed of the phases */
+    double force_helicity;
+    double force_tau;
+    unsigned long long force_seed;
 } or_fluid;
 
 /* ---- 1D FFT (unnormalised, in place), sign = -1 forward, +1 backward ---- */
@@ -125,6 +130,8 @@ void or_default_radii(const or_grid *g, int n, double *radii);
 double or_aniso_kpll(const or_grid *g, int lx, int ly, int lz, int dirn);
 double or_aniso_kperp(const or_grid *g, int lx, int ly, int lz, int dirn);
 double or_polar_angle(const or_grid *g, int lx, int ly, int lz, int dirn);
+double or_azimuthal_angle(const or_grid *g, int lx, int ly, int lz, int dirn);
+void or_modes_in_shell(const or_grid *g, int imax, double *count);
 int  or_aniso_max_krho_inside(const or_grid *g, int dirn);
 void or_ring_spectrum(const or_grid *g, double *const A[], int ncomp, int n, int dirn, const double *angles,
                       int nsec, double *out);

@@ -29,7 +29,9 @@ class Grid(ctypes.Structure):
 class Fluid(ctypes.Structure):
     _fields_ = [("nu", ctypes.c_double), ("dt", ctypes.c_double), ("force_kind", ctypes.c_int),
                 ("force_eps", ctypes.c_double), ("force_r_in", ctypes.c_double),
-                ("force_r_out", ctypes.c_double), ("nu_hyper", ctypes.c_double), ("hyper_exponent", ctypes.c_int)]
+                ("force_r_out", ctypes.c_double), ("nu_hyper", ctypes.c_double), ("hyper_exponent", ctypes.c_int),
+                ("force_helicity", ctypes.c_double), ("force_tau", ctypes.c_double),
+                ("force_seed", ctypes.c_ulonglong)]
 
 
 class Rbc(ctypes.Structure):
@@ -99,6 +101,8 @@ def lib():
             "or_sff_inverse": (None, [gp, i, p, p]),
             "or_sff_dealias": (None, [gp, p]),
             "or_get_dt": (d, [gp, i, pp, d, d, p]),
+            "or_modes_in_shell": (None, [gp, i, p]),
+            "or_azimuthal_angle": (d, [gp, i, i, i, i]),
             "or_sff_reality": (None, [gp, p, i]),
             "or_sff_energy": (d, [gp, p, p]),
             "or_sff_kmag": (d, [gp, i, i, i]),
@@ -441,6 +445,14 @@ class Oracle:
         out = np.zeros(self.rshape, np.float64)
         self.L.or_sff_inverse(ctypes.byref(self.g), parity, _cptr(spec), _ptr(out))
         return out
+
+    def modes_in_shell(self, imax):
+        c = np.zeros(imax + 1)
+        self.L.or_modes_in_shell(ctypes.byref(self.g), imax, _ptr(c))
+        return c
+
+    def azimuthal_angle(self, lx, ly, lz, dirn=3):
+        return self.L.or_azimuthal_angle(ctypes.byref(self.g), lx, ly, lz, dirn)
 
     def get_dt(self, V, courant, dt_fixed, sff=False):
         """FluidVF::Get_dt with dt_option 1 (dealiases V in place): (dt, umax[3])."""

@@ -60,6 +60,19 @@ double or_polar_angle(const or_grid *g, int lx, int ly, int lz, int dirn)
     return t >= 0 ? t : t + M_PI;
 }
 
+/* Get_azimuthal_angle (basicfn_inline.h:389-407) of AnisKvect_azimuthal_angle
+ * (FFF_pencil_inline.h:838-845) with AnisKh1, AnisKh2 (:729-757): the angle
+ * of (Kh1, Kh2) in [0, 2 pi), 0 on the axis */
+double or_azimuthal_angle(const or_grid *g, int lx, int ly, int lz, int dirn)
+{
+    const double a = kx_of(g, lx) * g->kfactor[0], b = ky_of(g, ly) * g->kfactor[1], c = lz * g->kfactor[2];
+    const double h1 = dirn == 1 ? b : (dirn == 2 ? c : a), h2 = dirn == 1 ? c : (dirn == 2 ? a : b);
+    const double kperp = sqrt(h1 * h1 + h2 * h2);
+    if (!(fabs(kperp) > OR_MYEPS)) return 0.0;
+    const double t = acos(h1 / kperp);
+    return h2 >= 0 ? t : 2 * M_PI - t;
+}
+
 /* Anis_max_Krho_radius_inside (FFF_pencil_inline.h:784-798) */
 int or_aniso_max_krho_inside(const or_grid *g, int dirn)
 {

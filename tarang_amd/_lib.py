@@ -27,7 +27,9 @@ class TgConfig(ctypes.Structure):
 class TgFluid(ctypes.Structure):
     _fields_ = [("nu", ctypes.c_double), ("dt", ctypes.c_double), ("force_kind", ctypes.c_int),
                 ("force_eps", ctypes.c_double), ("force_r_in", ctypes.c_double),
-                ("force_r_out", ctypes.c_double), ("nu_hyper", ctypes.c_double), ("hyper_exponent", ctypes.c_int)]
+                ("force_r_out", ctypes.c_double), ("nu_hyper", ctypes.c_double), ("hyper_exponent", ctypes.c_int),
+                ("force_helicity", ctypes.c_double), ("force_tau", ctypes.c_double),
+                ("force_seed", ctypes.c_ulonglong)]
 
 
 BASIS_FFF, BASIS_SFF, BASIS_CHFF = 0, 1, 2

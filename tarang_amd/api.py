@@ -210,11 +210,13 @@ class ChFF_Basis(FFF_Basis):
         check("tg_helmholtz", self.L.tg_helmholtz(self.plan, _vp(R), _vp(U), float(lambda_supplement), vp, vm))
 
 
-def _fluid(U, dt, force_kind=0, force_eps=0.0, force_r_in=0.0, force_r_out=0.0):
-    """tg_fluid of a FluidVF: dissipation, dt, forcing band, hyperviscosity."""
+def _fluid(U, dt, force_kind=0, force_eps=0.0, force_r_in=0.0, force_r_out=0.0, force_helicity=0.0,
+           force_tau=0.0, force_seed=0):
+    """tg_fluid of a FluidVF: dissipation, dt, forcing, hyperviscosity."""
     return TgFluid(U.dissipation_coefficient, float(dt), int(force_kind), float(force_eps), float(force_r_in),
                    float(force_r_out), getattr(U, "hyper_dissipation_coefficient", 0.0),
-                   getattr(U, "hyper_dissipation_exponent", 0))
+                   getattr(U, "hyper_dissipation_exponent", 0), float(force_helicity), float(force_tau),
+                   int(force_seed))
 
 
 class Time_advance_incompress_BDF1:
@@ -432,11 +434,21 @@ class Correlation:
 class Time_advance_incompress:
     """RK4 time advance of an incompressible fluid on one GPU."""
 
-    def __init__(self, U, dt, force_kind=0, force_eps=0.0, force_r_in=0.0, force_r_out=0.0):
+    def __init__(self, U, dt, force_kind=0, force_eps=0.0, force_r_in=0.0, force_r_out=0.0, force_helicity=0.0,
+                 force_tau=0.0, force_seed=0):
+        """force_kind 1: constant energy supply (rate force_eps on the band
+        (r_in, r_out]); 2: random band forcing (compute_force_random.cc:
+        65-130) with energy and helicity supply rates force_eps,
+        force_helicity, random interval force_tau: the phases are drawn anew
+        (seed force_seed + n in the n-th interval) once the time has passed
+        the end of the current interval."""
         self.U = U
-        self.fluid = _fluid(U, dt, force_kind, force_eps, force_r_in, force_r_out)
+        self.fluid = _fluid(U, dt, force_kind, force_eps, force_r_in, force_r_out, force_helicity, force_tau,
+                            force_seed)
         self._continue = False
         self._versions = None
+        self.time = 0.0
+        self._seed0, self._interval, self._random_next = int(force_seed), 0, float(force_tau)
 
     def _state_versions(self):
         # torch bumps a tensor's version on every in-place write (set_field,
@@ -456,12 +468,22 @@ class Time_advance_incompress:
         (raises TarangGpuError with code TG_ERR_NAN = -5 when the state turns
         non-finite)."""
         b = self.U.basis
+        if self.fluid.force_kind == 2 and nsteps > 1:
+            for _ in range(nsteps):
+                self.Time_advance_step(1, chain)
+            return
+        if self.fluid.force_kind == 2 and self.time >= self._random_next:
+            # Compute_force_using_random_noise_assign (compute_force_random.cc:143-167)
+            self._interval += 1
+            self._random_next += self.fluid.force_tau
+            self.fluid.force_seed = self._seed0 + self._interval
         cont = chain and self._continue and self._versions == self._state_versions()
         flags = STEP_CONTINUE if cont else 0
         self._continue = False
         check("tg_rk4_step", b.L.tg_rk4_step(b.plan, self.U.ptrs(), ctypes.byref(self.fluid), int(nsteps), flags))
         self._continue = True
         self._versions = self._state_versions()
+        self.time += self.fluid.dt * nsteps
 
     def invalidate(self):
         self._continue = False

@@ -32,6 +32,19 @@ __device__ __forceinline__ double aniso_polar(const Grid &g, int kx, int ky, int
     return t >= 0 ? t : t + pi;
 }
 
+// AnisKvect_azimuthal_angle via Get_azimuthal_angle (FFF_pencil_inline.h:
+// 729-757,838-845, basicfn_inline.h:389-407): the angle of (Kh1, Kh2) in
+// [0, 2 pi), 0 on the axis
+__device__ __forceinline__ double aniso_azimuth(const Grid &g, int kx, int ky, int kz, int dirn)
+{
+    const double a = kx * g.kf[0], b = ky * g.kf[1], c = kz * g.kf[2];
+    const double h1 = dirn == 1 ? b : (dirn == 2 ? c : a), h2 = dirn == 1 ? c : (dirn == 2 ? a : b);
+    const double kperp = sqrt(h1 * h1 + h2 * h2);
+    if (!(fabs(kperp) > kAnisEps)) return 0.0;
+    const double t = acos(h1 / kperp);
+    return h2 >= 0 ? t : 2 * 3.14159265358979323846 - t;
+}
+
 // first i in [0, n] with arr[i] >= x (n when none)
 __device__ __forceinline__ int first_ge(const double *arr, int n, double x)
 {

@@ -71,6 +71,9 @@ struct tg_plan {
     int red_blocks = 0;
     // tg_get_dt: one real field (Nx, Ny, Nz+2) and the three maxima, on first use
     double *rfield = nullptr, *umax_dev = nullptr;
+    // random band forcing: per-mode supplies (e_k, h_k) of shells 0..n-1
+    double *force_eh = nullptr;
+    int force_eh_n = 0;
     // chaining of RK4 steps: valid for the same state pointers and band
     bool unext_valid = false;
     double *unext_V[3] = {};
@@ -222,7 +225,7 @@ void free_plan(tg_plan *p)
     for (void *q : p->mhd)
         if (q) (void)hipFree(q);
     void *aptrs[] = {p->adj.ckpt, p->adj.tape, p->adj.vw, p->adj.lg, p->adj.acc, p->nonfinite_dev, p->rfield,
-                     p->umax_dev};
+                     p->umax_dev, p->force_eh};
     for (void *q : aptrs)
         if (q) (void)hipFree(q);
     void *cptrs[] = {p->ch.tmp, p->ch.D, p->ch.S, p->ch.NL[0], p->ch.NL[1], p->ch.NL[2], p->ch.R[0], p->ch.R[1],
@@ -263,6 +266,24 @@ int check_ptrs(const void *const *v, int n, const char *what)
     return TG_OK;
 }
 
+// Universal::Compute_modes_in_shell_array (universal_basic.cc:104-143): the
+// modes of the full box, k_d in (-N_d/2, N_d/2], with ceil(K) = i, i <= imax
+void modes_in_shell(const Grid &g, int imax, double *count)
+{
+    for (int i = 0; i <= imax; i++) count[i] = 0;
+    const int n[3] = {g.nx, g.ny, g.nz};
+    int lim[3];
+    for (int d = 0; d < 3; d++) lim[d] = std::min((int)std::ceil(imax / g.kf[d]) + 1, n[d] / 2);
+    for (int kx = -lim[0]; kx <= lim[0]; kx++)
+        for (int ky = -lim[1]; ky <= lim[1]; ky++)
+            for (int kz = -lim[2]; kz <= lim[2]; kz++) {
+                if (kx <= -n[0] / 2 || ky <= -n[1] / 2 || kz <= -n[2] / 2) continue;
+                const double a = kx * g.kf[0], b = ky * g.kf[1], c = kz * g.kf[2];
+                const int idx = (int)std::ceil(std::sqrt(a * a + b * b + c * c));
+                if (idx <= imax) count[idx] += 1;
+            }
+}
+
 int ensure_slots(tg_plan *p, const tg_fluid *f)
 {
     Band b{};
@@ -278,11 +299,44 @@ int ensure_slots(tg_plan *p, const tg_fluid *f)
         b.r_in = f->force_r_in;
         b.r_out = f->force_r_out;
         b.eps = f->force_eps;
+    } else if (f && f->force_kind == 2) {
+        // random band forcing: per-mode supplies of the band's shells
+        if (!(f->force_r_out > f->force_r_in) || f->force_r_in < 0)
+            return fail(TG_ERR_ARG, "forcing band must satisfy 0 <= r_in < r_out");
+        if (!(f->force_tau > 0) || !(f->force_eps >= 0))
+            return fail(TG_ERR_ARG, "random forcing needs force_tau > 0 and force_eps >= 0");
+        if (!(std::fabs(f->force_helicity) <= f->force_r_in * f->force_eps))
+            return fail(TG_ERR_ARG, "random forcing needs |force_helicity| <= r_in force_eps");
+        const int imax = (int)std::ceil(f->force_r_out), i0 = (int)std::floor(f->force_r_in) + 1;
+        if (imax > 4096) return fail(TG_ERR_ARG, "forcing band r_out=%g too large", f->force_r_out);
+        const int nsh = imax - i0 + 1;
+        std::vector<double> cnt(imax + 1, 0.0), eh(2 * (size_t)(imax + 1), 0.0);
+        modes_in_shell(p->g, imax, cnt.data());
+        for (int i = i0; i <= imax; i++)
+            if (cnt[i] > 0) {
+                eh[2 * i] = f->force_eps / nsh / cnt[i];
+                eh[2 * i + 1] = f->force_helicity / nsh / cnt[i];
+            }
+        if (p->force_eh_n < imax + 1) {
+            if (p->force_eh) { (void)hipFree(p->force_eh); p->force_eh = nullptr; }
+            int rc = dmalloc(p, (void **)&p->force_eh, eh.size() * sizeof(double));
+            if (rc) return rc;
+            p->force_eh_n = imax + 1;
+        }
+        HIPCHK(hipMemcpyAsync(p->force_eh, eh.data(), eh.size() * sizeof(double), hipMemcpyHostToDevice, p->stream));
+        HIPCHK(hipStreamSynchronize(p->stream));
+        b.kind = 2;
+        b.r_in = f->force_r_in;
+        b.r_out = f->force_r_out;
+        b.eps = f->force_eps;
+        b.shell_eh = p->force_eh;
+        b.tau = f->force_tau;
+        b.seed = f->force_seed;
     } else if (f && f->force_kind != 0) {
         return fail(TG_ERR_ARG, "unknown force_kind %d", f->force_kind);
     }
     p->band = b;
-    if (b.kind == 0) return TG_OK;
+    if (b.kind != 1) return TG_OK;
     if (p->slot_kb != b.kb) {
         // the stage-1 transform prepared for a continuation has no slots here
         p->unext_valid = false;
@@ -396,7 +450,7 @@ int run_nlin(tg_plan *p, double *const V[3], double *const out[3], const tg_flui
     }
     // Compute_nlin dealiases the state in place (compute_nlin.cc:61-62)
     if (p->g.dealias) HIPCHK(launch_dealias(p->g, V, 3, p->stream));
-    HIPCHK(launch_xinv_state<T>(p->g, p->band, V, 3, p->U, p->slots[0], p->band.kind ? 2 : 0, p->twx, p->stream));
+    HIPCHK(launch_xinv_state<T>(p->g, p->band, V, 3, p->U, p->slots[0], p->band.kind == 1 ? 2 : 0, p->twx, p->stream));
     int rc = ns_middle<T>(p);
     if (rc) return rc;
     RkParams rk = rk_params(p, f, 0);
@@ -455,7 +509,7 @@ int run_rk4(tg_plan *p, double *const V[3], const tg_fluid *f, int nsteps, int f
     if (rc) return rc;
     if (!cont) {
         Prof pr(p, TG_KERNEL_X_INV);
-        HIPCHK(launch_xinv_state<T>(p->g, p->band, V, 3, p->U, p->slots[0], 1 | (p->band.kind ? 2 : 0), p->twx,
+        HIPCHK(launch_xinv_state<T>(p->g, p->band, V, 3, p->U, p->slots[0], 1 | (p->band.kind == 1 ? 2 : 0), p->twx,
                                     p->stream));
     }
     for (int step = 0; step < nsteps; step++) {
@@ -471,7 +525,7 @@ int run_rk4(tg_plan *p, double *const V[3], const tg_fluid *f, int nsteps, int f
         if (p->cfg.strong_reality) {
             Prof pr(p, TG_KERNEL_REALITY);
             HIPCHK(launch_reality(p->g, V, 3, p->stream));
-            HIPCHK(launch_xinv_state<T>(p->g, p->band, V, 3, p->U, p->slots[0], 1 | 4 | (p->band.kind ? 2 : 0),
+            HIPCHK(launch_xinv_state<T>(p->g, p->band, V, 3, p->U, p->slots[0], 1 | 4 | (p->band.kind == 1 ? 2 : 0),
                                         p->twx, p->stream));
         }
     }

@@ -14,6 +14,7 @@
 
 #include "tg_diag.h"
 #include "tg_aniso.h"
+#include "tg_rand.h"
 
 namespace tg {
 
@@ -368,19 +369,6 @@ hipError_t launch_reduce(const Grid &g, int mode, const double *const *A, const 
 // ---------------------------------------------------------------------------
 // Seeded initial field (same definition as the oracle's or_init_field).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ uint64_t splitmix64(uint64_t x)
-{
-    x += 0x9E3779B97F4A7C15ULL;
-    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ULL;
-    x = (x ^ (x >> 27)) * 0x94D049BB133111EBULL;
-    return x ^ (x >> 31);
-}
-
-__device__ __forceinline__ double uniform01(uint64_t seed, uint64_t ctr)
-{
-    const uint64_t h = splitmix64(splitmix64(seed) ^ ctr);
-    return (double)(h >> 11) * (1.0 / 9007199254740992.0);
-}
 
 struct InitArgs {
     double *V[3];

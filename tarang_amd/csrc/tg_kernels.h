@@ -18,9 +18,14 @@ struct Grid {
 
 // Forcing band slots: modes with |kx|,|ky| <= kb and 0 <= kz <= kb.
 struct Band {
-    int kind;             // 0 none, 1 constant energy supply alpha-forcing
+    int kind;             // 0 none, 1 constant energy supply alpha-forcing, 2 random band forcing
     int kb;               // slot box half-width
     double r_in, r_out, eps;
+    // kind 2: per-mode supplies (e_k, h_k) of shells 0..ceil(r_out), random
+    // interval and the seed of the phases
+    const double *shell_eh;
+    double tau;
+    unsigned long long seed;
 };
 
 __host__ __device__ inline int band_nslots(int kb) { return (2 * kb + 1) * (2 * kb + 1) * (kb + 1); }

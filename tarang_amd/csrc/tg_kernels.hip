@@ -22,6 +22,7 @@
 #include "tg_kernels.h"
 #include "tg_pass.h"
 #include "tg_aniso.h"
+#include "tg_rand.h"
 
 namespace tg {
 
@@ -376,6 +377,30 @@ __device__ double band_alpha(const Grid &g, const Band &b, const double *slots)
     return e > 0.0 ? b.eps / (2.0 * e) : 0.0;
 }
 
+// Random band forcing of one mode (kind 2; compute_force_random.cc:65-130,
+// force_amp_phase.cc:119-170, FFF_pencil_inline.h:234-248): helical
+// amplitudes sqrt((e_k +- h_k/K) / tau) with phases 2 pi U(seed, 2m),
+// 2 pi U(seed, 2m + 1), m = (lx Ny + ly)(Nz/2+1) + lz; U1 = i (F- - F+),
+// U2 = F- + F+ in the Craya basis; Cartesian components about z.
+__device__ void random_force_mode(const Grid &g, const Band &b, int lx, int ly, int lz, int kx, int ky, double K,
+                                  cpx<double> *F)
+{
+    const int i = (int)ceil(K);
+    const double e = b.shell_eh[2 * i], h = b.shell_eh[2 * i + 1];
+    const double ap = sqrt((e + h / K) / b.tau), am = sqrt((e - h / K) / b.tau);
+    const uint64_t m = ((uint64_t)lx * g.ny + ly) * g.nzc + lz;
+    const double twopi = 6.28318530717958647692;
+    const double php = twopi * uniform01(b.seed, 2 * m), phm = twopi * uniform01(b.seed, 2 * m + 1);
+    const cpx<double> Fp = mk<double>(ap * cos(php), ap * sin(php)), Fm = mk<double>(am * cos(phm), am * sin(phm));
+    const cpx<double> d = Fm - Fp;
+    const cpx<double> U1 = mk<double>(-d.y, d.x), U2 = Fm + Fp;
+    const double th = aniso_polar(g, kx, ky, lz, 3), ph = aniso_azimuth(g, kx, ky, lz, 3);
+    const double st = sin(th), ct = cos(th), sp = sin(ph), cp = cos(ph);
+    F[0] = sp * U1 + (ct * cp) * U2;
+    F[1] = (-cp) * U1 + (sp * ct) * U2;
+    F[2] = (-st) * U2;
+}
+
 // Order in which the product spectra are folded into N, following the
 // reference's diag/offdiag accumulation (compute_diag_offdiag.cc:47-112 for
 // NS, :329-414 for the giver form with P[3i+j] = FT[u_j w_i]).
@@ -520,6 +545,15 @@ __global__ void __launch_bounds__(CB *(L / 8), 2) k_x_nlin(XnlArgs a)
                         n1 = n1 - alpha * sv[1];
                         n2 = n2 - alpha * sv[2];
                     }
+                }
+            } else if (a.b.kind == 2 && cw >= 0) {
+                const double K = kmag(g, kx, ky, lz);
+                if (K > a.b.r_in && K <= a.b.r_out) {
+                    cpx<double> F[3];
+                    random_force_mode(g, a.b, lx, ly, lz, kx, ky, K, F);
+                    n0 = n0 - F[0];
+                    n1 = n1 - F[1];
+                    n2 = n2 - F[2];
                 }
             }
             cpx<double> div = mul_i(Kx, n0);

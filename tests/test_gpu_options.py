@@ -134,3 +134,42 @@ def test_get_dt_sff():
     log(f"test_get_dt_sff: dt {dt:.6e} ref {dt_ref:.6e}")
     assert abs(dt - dt_ref) <= 1e-12 * dt_ref
     assert np.allclose(U.umax, um_ref, rtol=1e-12, atol=0)
+
+
+@pytest.mark.parametrize("prec", [api.PREC_FP64, api.PREC_FP32])
+def test_random_band_forcing(prec):
+    """Random band forcing (force_kind 2): the forced right-hand side and
+    RK4 steps against the oracle; the phases are redrawn (seed + 1) once a
+    random interval has passed."""
+    N = (32, 32, 32)
+    nu, dt, tau, seed = 0.01, 1e-3, 1e-3, 1234
+    frc = dict(force_kind=2, force_eps=0.2, force_r_in=2.0, force_r_out=4.0, force_helicity=0.3, force_tau=tau,
+               force_seed=seed)
+    log(f"test_random_band_forcing prec={prec}")
+    o = to.Oracle(N, dealias=to.DEALIAS_REF_CORNER, strong_reality=True)
+    V = o.init_field(kind=1, k0=4.0, kmin=1.0, kmax=10.0, energy=0.5, seed=71)
+
+    def ofl(s):
+        return to.Fluid(nu, dt, 2, 0.2, 2.0, 4.0, 0.0, 0, 0.3, tau, s)
+
+    b = api.FFF_Basis(N, precision=prec, strong_reality=True)
+    U = api.FluidVF(b, nu)
+    U.set_field([dev(v) for v in V])
+    ta = api.Time_advance_incompress(U, dt, **frc)
+    ta.Compute_force_TO_rhs()
+    Vd = [v.copy() for v in V]
+    for v in Vd:
+        o.dealias(v)
+    rhs_ref = o.compute_nlin([v.copy() for v in Vd])
+    o.compute_rhs(rhs_ref, o.compute_force(ofl(seed), Vd))      # in place
+    e_r = rel_l2_vec([host(n) for n in U.nlin], rhs_ref)
+    log(f"  forced rhs rel L2 {e_r:.2e}")
+    assert e_r < TOL[prec]
+    U.set_field([dev(v) for v in V])
+    ta = api.Time_advance_incompress(U, dt, **frc)
+    for s in range(3):
+        ta.Time_advance_step(1)
+        o.rk4_step(ofl(seed + s), V)      # a new interval each step (tau = dt)
+        e = rel_l2_vec([host(v) for v in U.cvf], V)
+        log(f"  step {s + 1} (seed {ta.fluid.force_seed}): rel L2 {e:.2e}")
+        assert ta.fluid.force_seed == seed + s and e < TOL[prec]

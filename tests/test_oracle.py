@@ -340,6 +340,58 @@ def test_get_dt_sff_grid_maxima():
     assert abs(dt - ref) <= 1e-13 * ref
 
 
+def test_modes_in_shell_brute_force():
+    """Universal::Compute_modes_in_shell_array: the modes of the full box
+    k_d in (-N_d/2, N_d/2] with ceil(K) = i."""
+    o = to.Oracle((16, 8, 16), kfactor=(1.0, 0.5, 2.0))
+    c = o.modes_in_shell(6)
+    kx = np.arange(-7, 9) * 1.0
+    ky = np.arange(-3, 5) * 0.5
+    kz = np.arange(-7, 9) * 2.0
+    K = np.sqrt(kx[:, None, None] ** 2 + ky[None, :, None] ** 2 + kz[None, None, :] ** 2)
+    ref = np.bincount(np.ceil(K).astype(int).ravel())[:7]
+    assert np.array_equal(c, ref)
+
+
+@pytest.mark.parametrize("hel", [0.0, 0.6])
+def test_random_force_structure(hel):
+    """Random band forcing (compute_force_random.cc:65-130): zero outside the
+    band, divergence-free, |F|^2 = 2 (F+^2 + F-^2) = 4 e_k / tau, and
+    helicity fraction Re(F* . i K x F) / (K |F|^2) = (h_k / K) / e_k
+    (the + helical mode is the curl eigenvector of eigenvalue +K); e_k =
+    eps / n_shells / modes_in_shell(ceil K); phases set by the seed."""
+    N = (16, 16, 16)
+    kf = (1.0, 1.0, 1.0)
+    o = to.Oracle(N, kfactor=kf)
+    eps, r_in, r_out, tau = 0.3, 2.0, 4.0, 0.02
+    fl = to.Fluid(0.01, 1e-3, 2, eps, r_in, r_out, 0.0, 0, hel, tau, 17)
+    F = o.compute_force(fl, [o.czeros() for _ in range(3)])
+    cnt = o.modes_in_shell(4)
+    nsh = 4 - 2
+    kx = np.fft.fftfreq(16, 1 / 16)[:, None, None] * np.ones((1, 16, 9))
+    ky = np.fft.fftfreq(16, 1 / 16)[None, :, None] * np.ones((16, 1, 9))
+    kz = np.arange(9)[None, None, :] * np.ones((16, 16, 1))
+    K = np.sqrt(kx ** 2 + ky ** 2 + kz ** 2)
+    band = (K > r_in) & (K <= r_out)
+    F2 = sum(np.abs(f) ** 2 for f in F)
+    assert np.all(F2[~band] == 0) and np.all(F2[band] > 0)
+    e = eps / nsh / cnt[np.ceil(K[band]).astype(int)]
+    h = hel / nsh / cnt[np.ceil(K[band]).astype(int)]
+    assert np.allclose(F2[band], 4 * e / tau, rtol=1e-13, atol=0)
+    div = kx * F[0] + ky * F[1] + kz * F[2]
+    assert np.abs(div).max() <= 1e-13 * np.sqrt(F2.max()) * K.max()
+    curl = [1j * (ky * F[2] - kz * F[1]), 1j * (kz * F[0] - kx * F[2]), 1j * (kx * F[1] - ky * F[0])]
+    H = sum((np.conj(F[c]) * curl[c]).real for c in range(3))
+    frac = H[band] / (K[band] * F2[band])
+    assert np.allclose(frac, (h / K[band]) / e, rtol=0, atol=1e-12)
+    # the phases: deterministic in the seed, different for another seed
+    F_same = o.compute_force(fl, [o.czeros() for _ in range(3)])
+    assert all(np.array_equal(a, b) for a, b in zip(F, F_same))
+    fl2 = to.Fluid(0.01, 1e-3, 2, eps, r_in, r_out, 0.0, 0, hel, tau, 18)
+    F_other = o.compute_force(fl2, [o.czeros() for _ in range(3)])
+    assert np.abs(F_other[0] - F[0]).max() > 0.1 * np.abs(F[0]).max()
+
+
 def test_rk4_fourth_order_convergence():
     """Lawson RK4 is fourth order; with the strong reality condition on, the
     per-step zeroing of the kz = Nz/2 plane (FFF_pencil_basic.cc:175-177)
