@@ -74,6 +74,7 @@ struct tg_plan {
     // random band forcing: per-mode supplies (e_k, h_k) of shells 0..n-1
     double *force_eh = nullptr;
     int force_eh_n = 0;
+    double force_key[4] = {};  // eps, helicity, r_in, r_out of the table
     // chaining of RK4 steps: valid for the same state pointers and band
     bool unext_valid = false;
     double *unext_V[3] = {};
@@ -309,22 +310,29 @@ int ensure_slots(tg_plan *p, const tg_fluid *f)
             return fail(TG_ERR_ARG, "random forcing needs |force_helicity| <= r_in force_eps");
         const int imax = (int)std::ceil(f->force_r_out), i0 = (int)std::floor(f->force_r_in) + 1;
         if (imax > 4096) return fail(TG_ERR_ARG, "forcing band r_out=%g too large", f->force_r_out);
-        const int nsh = imax - i0 + 1;
-        std::vector<double> cnt(imax + 1, 0.0), eh(2 * (size_t)(imax + 1), 0.0);
-        modes_in_shell(p->g, imax, cnt.data());
-        for (int i = i0; i <= imax; i++)
-            if (cnt[i] > 0) {
-                eh[2 * i] = f->force_eps / nsh / cnt[i];
-                eh[2 * i + 1] = f->force_helicity / nsh / cnt[i];
+        const double key[4] = {f->force_eps, f->force_helicity, f->force_r_in, f->force_r_out};
+        if (!p->force_eh || std::memcmp(key, p->force_key, sizeof(key)) != 0) {
+            // the per-mode supplies depend on the rates and the band only (not on the seed)
+            const int nsh = imax - i0 + 1;
+            std::vector<double> cnt(imax + 1, 0.0), eh(2 * (size_t)(imax + 1), 0.0);
+            modes_in_shell(p->g, imax, cnt.data());
+            for (int i = i0; i <= imax; i++)
+                if (cnt[i] > 0) {
+                    eh[2 * i] = f->force_eps / nsh / cnt[i];
+                    eh[2 * i + 1] = f->force_helicity / nsh / cnt[i];
+                }
+            if (p->force_eh_n < imax + 1) {
+                HIPCHK(hipStreamSynchronize(p->stream));
+                if (p->force_eh) { (void)hipFree(p->force_eh); p->force_eh = nullptr; }
+                int rc = dmalloc(p, (void **)&p->force_eh, eh.size() * sizeof(double));
+                if (rc) return rc;
+                p->force_eh_n = imax + 1;
             }
-        if (p->force_eh_n < imax + 1) {
-            if (p->force_eh) { (void)hipFree(p->force_eh); p->force_eh = nullptr; }
-            int rc = dmalloc(p, (void **)&p->force_eh, eh.size() * sizeof(double));
-            if (rc) return rc;
-            p->force_eh_n = imax + 1;
+            HIPCHK(hipMemcpyAsync(p->force_eh, eh.data(), eh.size() * sizeof(double), hipMemcpyHostToDevice,
+                                  p->stream));
+            HIPCHK(hipStreamSynchronize(p->stream));
+            std::memcpy(p->force_key, key, sizeof(key));
         }
-        HIPCHK(hipMemcpyAsync(p->force_eh, eh.data(), eh.size() * sizeof(double), hipMemcpyHostToDevice, p->stream));
-        HIPCHK(hipStreamSynchronize(p->stream));
         b.kind = 2;
         b.r_in = f->force_r_in;
         b.r_out = f->force_r_out;
