@@ -62,7 +62,11 @@ static double *sc_table(int n, int parity)
     double *t = (double *)malloc(sizeof(double) * (size_t)n * n);
     for (int m = 0; m < n; m++)
         for (int j = 0; j < n; j++) {
-            const double a = M_PI * m * (2.0 * j + 1.0) / (2.0 * n);
+            /* the phase pi m (2j+1) / (2n) reduced modulo 2 pi in integers first:
+             * sin and cos of a large rounded argument would carry its rounding
+             * error (|a| eps, 1e-13 at n = 768) into every entry */
+            const long r = ((long)m * (2 * j + 1)) % (4 * (long)n);
+            const double a = M_PI * (double)r / (2.0 * n);
             t[(size_t)m * n + j] = parity == OR_SIN ? sin(a) : cos(a);
         }
     return t;

@@ -208,3 +208,123 @@ def test_fp32_takes_powers_of_two_only():
         api.FFF_Basis((48, 48, 48), precision=api.PREC_FP32)
     with pytest.raises(TarangGpuError):
         api.FFF_Basis((56, 48, 48))      # 7 * 8: no radix-7 kernels
+
+
+BIG_ODD_L = [192, 384, 768, 960]
+BIG_ODD_M = [192, 384, 480]
+
+
+def _mhd_two_steps(N):
+    o = to.Oracle(N, dealias=to.DEALIAS_REF_CORNER, strong_reality=True)
+    V = o.init_field(kind=1, k0=4.0, kmin=1.0, kmax=max(N) / 3, energy=0.5, seed=35)
+    W = o.init_field(kind=1, k0=4.0, kmin=1.0, kmax=max(N) / 3, energy=0.25, seed=36)
+    b = api.FFF_Basis(N, dealias=to.DEALIAS_REF_CORNER, strong_reality=True)
+    U, B = api.FluidVF(b, 5e-3), api.FluidVF(b, 3e-3)
+    U.set_field([dev(v) for v in V])
+    B.set_field([dev(w) for w in W])
+    api.Time_advance_incompress_MHD(U, B, 1e-3).Time_advance_step(2)
+    fl = to.Fluid(5e-3, 1e-3, 0, 0, 0, 0)
+    for _ in range(2):
+        o.rk4_step_mhd(fl, 3e-3, V, W)
+    return rel_l2_vec([host(v) for v in U.cvf] + [host(w) for w in B.cvf], V + W)
+
+
+def _rbc_two_steps(N):
+    kf = (np.pi, np.pi / np.sqrt(2.0), np.pi / np.sqrt(2.0))
+    o = to.Oracle(N, kfactor=kf, dealias=to.DEALIAS_REF_CORNER, strong_reality=False)
+    rng = np.random.default_rng(52)
+    lx = np.arange(N[0])[:, None, None] * kf[0]
+    ly = np.arange(N[1])
+    ky = np.where(ly <= N[1] // 2, ly, ly - N[1])[None, :, None] * kf[1]
+    kz = np.arange(N[2] // 2 + 1)[None, None, :] * kf[2]
+    damp = 0.1 * np.exp(-(np.sqrt(lx ** 2 + ky ** 2 + kz ** 2) / (40.0 * kf[0])) ** 2)
+    V = [damp * (rng.standard_normal(o.cshape) + 1j * rng.standard_normal(o.cshape)) for _ in range(3)]
+    T = damp * (rng.standard_normal(o.cshape) + 1j * rng.standard_normal(o.cshape))
+    V[0][0] = 0
+    T[0] = 0
+    b = api.SFF_Basis(N, kfactor=kf, dealias=api.DEALIAS_REF_CORNER, reality=2)
+    U, S = api.FluidVF(b), api.FluidSF(b)
+    U.set_field([dev(v) for v in V])
+    S.set_field(dev(T))
+    ta = api.Time_advance_incompress_RBC(U, S, 1e7, 1.0, 1e-4)
+    ta.Compute_nlin()
+    Nref, NTref = o.compute_nlin_rbc([v.copy() for v in V], T.copy())
+    e_n = rel_l2_vec([host(n) for n in U.nlin] + [host(S.nlin)], Nref + [NTref])
+    U.set_field([dev(v) for v in V])
+    S.set_field(dev(T))
+    api.Time_advance_incompress_RBC(U, S, 1e7, 1.0, 1e-4).Time_advance_step(2)
+    for _ in range(2):
+        o.rk4_step_rbc(to.Rbc(1e7, 1.0, 1e-4, 2), V, T)
+    e = rel_l2_vec([host(v) for v in U.cvf] + [host(S.F)], V + [T])
+    log(f"  RBC {N}: Compute_nlin {e_n:.2e}, 2 RK4 steps {e:.2e}")
+    return e
+
+
+def _vjp(N):
+    o = to.Oracle(N, dealias=to.DEALIAS_REF_CORNER, strong_reality=False)
+    V = o.init_field(kind=1, kmin=1.0, kmax=max(N) / 3, energy=0.5, seed=37)
+    rng = np.random.default_rng(38)
+    Lam = [0.1 * (rng.standard_normal(o.cshape) + 1j * rng.standard_normal(o.cshape)) for _ in range(3)]
+    ref = o.rhs_vjp(V, Lam)
+    b = api.FFF_Basis(N, dealias=api.DEALIAS_REF_CORNER, strong_reality=False)
+    U = api.FluidVF(b, 0.01)
+    U.set_field([dev(v) for v in V])
+    out = api.Adjoint_incompress(U, 1e-3).Rhs_vjp([dev(x) for x in Lam])
+    return rel_l2_vec([host(x) for x in out], ref)
+
+
+@pytest.mark.parametrize("L", BIG_ODD_L)
+def test_large_odd_x_lines_other_paths(L):
+    """The MHD, RBC (sin/cos) and adjoint x passes at the long radix-3/5 x
+    lines (their own tile widths and staging paths)."""
+    N = (L, 8, 16)
+    em, er, ea = _mhd_two_steps(N), _rbc_two_steps(N), _vjp(N)
+    # The sin/cos x lines reach K = pi L (3016 at L = 960): the x-derivatives
+    # of the nonlinear term multiply the round-off of the high-K modes
+    # (which the corner box does not dealias) by K at every stage, and the
+    # second step amplifies it again; measured GPU-oracle differences of
+    # the state: 1e-15 (L = 384), 1.3e-13 (768), 1.6e-12 (960), largest at
+    # lx > 2L/3 where the state is 1e-15 of its peak, while the transforms
+    # alone agree with scipy to 5e-16 and Compute_nlin to 3e-14.  The bound
+    # scales with K_max^2.
+    tol_rbc = TOL * max(1.0, (np.pi * L / 1000.0) ** 2)
+    log(f"test_large_odd_x_lines_other_paths L={L}: MHD {em:.2e} RBC {er:.2e} (bound {tol_rbc:.1e}) adjoint {ea:.2e}")
+    assert em < TOL and er < tol_rbc and ea < TOL
+
+
+@pytest.mark.parametrize("M", BIG_ODD_M)
+def test_large_odd_z_lines_other_paths(M):
+    """The MHD (giver form), RBC and adjoint z passes at long radix-3/5 z lines."""
+    N = (8, 8, 2 * M)
+    em, er, ea = _mhd_two_steps(N), _rbc_two_steps(N), _vjp(N)
+    log(f"test_large_odd_z_lines_other_paths Nz={2 * M}: MHD {em:.2e} RBC {er:.2e} adjoint {ea:.2e}")
+    assert em < TOL and er < TOL and ea < TOL
+
+
+@pytest.mark.parametrize("L", [384, 768, 960])
+def test_sff_long_lines_against_scipy(L):
+    """sin/cos transforms of long radix-3/5 x lines against scipy.fft's
+    DCT-II / DST-II (the convention pinned in test_oracle_sff.py), for the
+    GPU and for the oracle."""
+    from scipy import fft as sf
+    N = (L, 8, 16)
+    kf = (np.pi, np.pi / np.sqrt(2.0), np.pi / np.sqrt(2.0))
+    o = to.Oracle(N, kfactor=kf, dealias=to.DEALIAS_NONE, strong_reality=False)
+    b = api.SFF_Basis(N, kfactor=kf, dealias=api.DEALIAS_NONE, reality=0)
+    r = np.zeros(o.rshape)
+    r[:, :, :16] = np.random.default_rng(5).standard_normal((L, 8, 16))
+    B = np.fft.fft(np.fft.rfft(r[:, :, :16], axis=2), axis=1) / (8 * 16)
+    errs = []
+    for par, code in ((to.COS, api.SC_COS), (to.SIN, api.SC_SIN)):
+        if par == to.COS:
+            ref = sf.dct(B, type=2, axis=0) / (2 * L)
+        else:
+            Y = sf.dst(B, type=2, axis=0) / (2 * L)
+            ref = np.zeros_like(Y)
+            ref[1:] = Y[:-1]
+        A = b.complex_array()
+        b.Forward_transform(dev(r), A, code)
+        errs += [rel_l2(host(A), ref), rel_l2(o.sff_forward(par, r), ref)]
+    log(f"test_sff_long_lines_against_scipy L={L}: GPU cos {errs[0]:.2e} sin {errs[2]:.2e}; "
+        f"oracle cos {errs[1]:.2e} sin {errs[3]:.2e}")
+    assert max(errs) < 1e-13
