@@ -299,12 +299,44 @@ class Time_advance_incompress {
         fluid.force_r_in = r_in;
         fluid.force_r_out = r_out;
     }
+    // Band-limited random forcing (compute_force_random.cc:65-130): energy and
+    // helicity supply rates on (r_in, r_out], phases redrawn every `tau` time
+    // units (seed + n in the n-th interval, as the reference reseeds)
+    void Set_random_forcing(double eps, double helicity, double r_in, double r_out, double tau,
+                            unsigned long long seed)
+    {
+        fluid.force_kind = 2;
+        fluid.force_eps = eps;
+        fluid.force_helicity = helicity;
+        fluid.force_r_in = r_in;
+        fluid.force_r_out = r_out;
+        fluid.force_tau = tau;
+        fluid.force_seed = seed;
+        seed0_ = seed;
+        interval_ = 0;
+        random_next_ = tau;
+    }
     // Time_advance_step with integration_scheme RK4 (Time_advance.cc:43-65)
     void Time_advance_step(int nsteps = 1)
     {
         auto v = U.V();
+        if (fluid.force_kind == 2) {
+            // one step a call: a new random interval may begin at any step
+            for (int s = 0; s < nsteps; s++) {
+                if (time_ >= random_next_) {
+                    interval_++;
+                    random_next_ += fluid.force_tau;
+                    fluid.force_seed = seed0_ + interval_;
+                }
+                check("tg_rk4_step", tg_rk4_step(U.basis.plan(), v.data(), &fluid, 1, chained_ ? TG_STEP_CONTINUE : 0));
+                chained_ = true;
+                time_ += fluid.dt;
+            }
+            return;
+        }
         check("tg_rk4_step", tg_rk4_step(U.basis.plan(), v.data(), &fluid, nsteps, chained_ ? TG_STEP_CONTINUE : 0));
         chained_ = true;
+        time_ += fluid.dt * nsteps;
     }
     // call after modifying U.cvf outside the time advance
     void Field_modified() { chained_ = false; }
@@ -314,6 +346,8 @@ class Time_advance_incompress {
   private:
     FluidVF &U;
     bool chained_ = false;
+    double time_ = 0.0, random_next_ = 0.0;
+    unsigned long long seed0_ = 0, interval_ = 0;
 };
 
 // RK4 of U (nu) and the magnetic field W (eta) (RK_VF_MHD.cc:126-190)
