@@ -122,11 +122,11 @@ __device__ __forceinline__ void tape_mac(cpx<T> *q, const cpx<T> *v, const cpx<T
     }
 }
 
-// sigma lines in a.in[]: 0 xx, 1 yy, 2 zz, 3 xy, 4 yz, 5 zx.  One
-// accumulator at a time: q_m = sum_j u_j sigma_jm over three c2r, then its
-// r2c (nine c2r per line; the off-diagonal sigma lines are read twice, the
-// second time from cache).  With more accumulators live the register
-// allocation of the line transforms grows past the register file.
+// sigma lines in a.in[]: 0 xx, 1 yy, 2 zz, 3 xy, 4 yz, 5 zx.  The three
+// accumulators q_m = sum_j u_j sigma_jm are live together, so each sigma
+// line takes one c2r: a diagonal line sigma_jj adds u_j sigma_jj to q_j, an
+// off-diagonal line sigma_jm adds u_j sigma_jm to q_m and u_m sigma_jm to
+// q_j.  Six c2r and three r2c per line.
 template <typename T, int M>
 __global__ void __launch_bounds__(256) k_z_adj(ZArgs a)
 {
@@ -142,38 +142,32 @@ __global__ void __launch_bounds__(256) k_z_adj(ZArgs a)
     const cpx<T> *twm = (const cpx<T> *)a.twm, *twn = (const cpx<T> *)a.twn;
     cpx<T> *L = lds + li * LP;
     const size_t off = lsafe * a.g.nzp;
-    constexpr int sig[3][3] = {{0, 3, 5}, {3, 1, 4}, {5, 4, 2}};  // sigma_jm -> line
+    cpx<T> q[3][8];
+#pragma unroll
+    for (int m = 0; m < 3; m++)
+#pragma unroll
+        for (int s = 0; s < 8; s++) q[m][s] = mk<T>(0, 0);
 #pragma unroll 1
-    for (int m = 0; m < 3; m++) {
-        cpx<T> q[8], v[8];
+    for (int t = 0; t < 6; t++) {
+        // sigma_jm of line t: (j, m) = (t, t) for t < 3, (t-3, t-2 mod 3) after
+        const int jt = t < 3 ? t : t - 3, mt = t < 3 ? t : (t == 5 ? 0 : t - 2);
+        const cpx<T> *X = (const cpx<T> *)(t == 0 ? a.in[0] : t == 1 ? a.in[1] : t == 2 ? a.in[2] :
+                                            t == 3 ? a.in[3] : t == 4 ? a.in[4] : a.in[5]);
+        cpx<T> v[8];
+        zline_c2r<T, M>(v, X + off, L, twm, twn, j);
+        line_sync<M / 8>();
+        const cpx<T> *uj = (const cpx<T> *)(jt == 0 ? a.tape[0] : jt == 1 ? a.tape[1] : a.tape[2]) + lsafe * M;
+        const cpx<T> *um = (const cpx<T> *)(mt == 0 ? a.tape[0] : mt == 1 ? a.tape[1] : a.tape[2]) + lsafe * M;
 #pragma unroll
-        for (int s = 0; s < 8; s++) q[s] = mk<T>(0, 0);
-#pragma unroll 1
-        for (int jj = 0; jj < 3; jj++) {
-            const int t = m == 0 ? sig[0][jj] : (m == 1 ? sig[1][jj] : sig[2][jj]);
-            const cpx<T> *X = (const cpx<T> *)(t == 0 ? a.in[0] : t == 1 ? a.in[1] : t == 2 ? a.in[2] :
-                                                t == 3 ? a.in[3] : t == 4 ? a.in[4] : a.in[5]);
-            const cpx<T> *u = (const cpx<T> *)(jj == 0 ? a.tape[0] : jj == 1 ? a.tape[1] : a.tape[2]);
-            if constexpr (sizeof(T) == 4 || M <= 128) {
-                // the sigma line and the tape line are loaded together, before
-                // the transform (longer FP64 lines have no registers to spare
-                // for the tape line across the transform)
-                cpx<T> xm, ut[8];
-                zline_load<T, M>(v, xm, X + off, j);
-#pragma unroll
-                for (int s = 0; s < 8; s++) ut[s] = mk<T>(u[lsafe * M + j + TPL * s].x, u[lsafe * M + j + TPL * s].y);
-                zline_c2r_from<T, M>(v, xm, L, twm, twn, j);
-                line_sync<M / 8>();
-#pragma unroll
-                for (int s = 0; s < 8; s++) q[s] = q[s] + mk<T>(ut[s].x * v[s].x, ut[s].y * v[s].y);
-            } else {
-                zline_c2r<T, M>(v, X + off, L, twm, twn, j);
-                line_sync<M / 8>();
-                tape_mac<T, M>(q, v, u + lsafe * M, j);
-            }
+        for (int k = 0; k < 3; k++) {
+            if (k == mt) tape_mac<T, M>(q[k], v, uj, j);
+            if (t >= 3 && k == jt) tape_mac<T, M>(q[k], v, um, j);
         }
+    }
+#pragma unroll
+    for (int m = 0; m < 3; m++) {
         cpx<T> *Q = (cpx<T> *)(m == 0 ? a.out[0] : m == 1 ? a.out[1] : a.out[2]);
-        zline_r2c_post<T, M>(q, Q + off, L, twm, twn, j, valid);
+        zline_r2c_post<T, M>(q[m], Q + off, L, twm, twn, j, valid);
     }
 }
 
