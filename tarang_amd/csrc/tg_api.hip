@@ -92,9 +92,9 @@ struct tg_plan {
     // work layout), [1..4] force inputs theta, u_x of two stage parities
     // (FP64 boundary layout)
     double *rbc[5] = {};
-    // MHD buffers (FP64, boundary layout), allocated on first use:
-    // [0..2] accW, [3..5] N+, [6..8] N-, [9..11] Z+, [12..14] Z-
-    double *mhd[15] = {};
+    // MHD buffers (FP64), allocated on first use: [0..2] accW (padded work
+    // layout), [3..5] Z+, [6..8] Z- (boundary layout)
+    double *mhd[9] = {};
     // adjoint: checkpoints (FP64 spectral, 3 components each), tape of the
     // real-space stage velocities (every x 4 stages x 3 components, packed
     // z lines in the transform precision), recompute state, cotangent and
@@ -545,9 +545,9 @@ int run_rk4(tg_plan *p, double *const V[3], const tg_fluid *f, int nsteps, int f
 
 int ensure_mhd(tg_plan *p)
 {
-    for (int i = 0; i < 15; i++)
+    for (int i = 0; i < 9; i++)
         if (!p->mhd[i]) {
-            int rc = dmalloc(p, (void **)&p->mhd[i], p->nc * 16);
+            int rc = dmalloc(p, (void **)&p->mhd[i], (i < 3 ? p->ncw : p->nc) * 16);
             if (rc) return rc;
         }
     return TG_OK;
@@ -586,15 +586,17 @@ int run_rk4_mhd(tg_plan *p, double *const V[3], double *const W[3], const tg_flu
     int rc = ensure_mhd(p);
     if (rc) return rc;
     double *accW[3] = {p->mhd[0], p->mhd[1], p->mhd[2]};
-    double *Zp[3] = {p->mhd[9], p->mhd[10], p->mhd[11]};
-    double *Zm[3] = {p->mhd[12], p->mhd[13], p->mhd[14]};
+    double *Zp[3] = {p->mhd[3], p->mhd[4], p->mhd[5]};
+    double *Zm[3] = {p->mhd[6], p->mhd[7], p->mhd[8]};
     p->unext_valid = false;
     rc = nonfinite_begin(p);
     if (rc) return rc;
+    // stage-1 input of the first step; the stage-4 x pass of each step leaves
+    // that of the next
+    HIPCHK(launch_mhd_elsasser(p->g, V, W, Zp, Zm, p->stream));
+    rc = mhd_xinv<T>(p, Zp, Zm);
+    if (rc) return rc;
     for (int step = 0; step < nsteps; step++) {
-        HIPCHK(launch_mhd_elsasser(p->g, V, W, Zp, Zm, p->stream));
-        rc = mhd_xinv<T>(p, Zp, Zm);
-        if (rc) return rc;
         for (int s = 1; s <= 4; s++) {
             { Prof pr(p, TG_KERNEL_Y_FFT); HIPCHK(launch_colfft<T>(p->g, 1, +1, p->U, 6, p->twy, p->stream)); }
             {
@@ -629,6 +631,13 @@ int run_rk4_mhd(tg_plan *p, double *const V[3], double *const W[3], const tg_flu
             Prof pr(p, TG_KERNEL_REALITY);
             HIPCHK(launch_reality(p->g, V, 3, p->stream));
             HIPCHK(launch_reality(p->g, W, 3, p->stream));
+            if (step + 1 < nsteps) {
+                // the next stage-1 input on the two planes the condition rewrote
+                Band nb{};
+                HIPCHK(launch_mhd_elsasser_planes(p->g, V, W, Zp, Zm, p->stream));
+                HIPCHK(launch_xinv_state<T>(p->g, nb, Zm, 3, p->U, nullptr, 4, p->twx, p->stream));
+                HIPCHK(launch_xinv_state<T>(p->g, nb, Zp, 3, p->U + 3, nullptr, 4, p->twx, p->stream));
+            }
         }
     }
     return nonfinite_end(p, "tg_rk4_step_mhd");
@@ -639,8 +648,8 @@ int run_nlin_mhd(tg_plan *p, double *const V[3], double *const W[3], double *con
 {
     int rc = ensure_mhd(p);
     if (rc) return rc;
-    double *Zp[3] = {p->mhd[9], p->mhd[10], p->mhd[11]};
-    double *Zm[3] = {p->mhd[12], p->mhd[13], p->mhd[14]};
+    double *Zp[3] = {p->mhd[3], p->mhd[4], p->mhd[5]};
+    double *Zm[3] = {p->mhd[6], p->mhd[7], p->mhd[8]};
     p->unext_valid = false;
     if (p->g.dealias) {
         HIPCHK(launch_dealias(p->g, V, 3, p->stream));

@@ -13,7 +13,8 @@
 //                   above, RHS_u = -P N_u, RHS_b = -N_b (Tadvance_functions.cc:
 //                   262-270), the Lawson RK4 stage for u (nu) and b (eta)
 //                   (RK_VF_MHD.cc:126-190), and the x-inverse of the next
-//                   stage's Z+- (one read of the products per stage)
+//                   stage's Z+- (one read of the products per stage; stage 4
+//                   leaves the next step's stage-1 input)
 #include <type_traits>
 
 #include "tg_device.h"
@@ -91,6 +92,25 @@ __global__ void k_mhd_elsasser(MhdArgs a)
     }
 }
 
+// The same on the kz = 0 and kz = Nz/2 planes only (after the reality
+// condition has rewritten them).
+__global__ void k_mhd_elsasser_planes(MhdArgs a)
+{
+    const Grid &g = a.g;
+    const int n = g.nx * g.ny;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < 2 * n; t += gridDim.x * blockDim.x) {
+        const int lz = t < n ? 0 : g.nz / 2, q = t < n ? t : t - n;
+        const int lx = q / g.ny, ly = q % g.ny;
+        const size_t i = ((size_t)lx * g.ny + ly) * g.nzc + lz;
+        const bool z = dealias_zero_m(g, lx, ly, lz);
+        for (int c = 0; c < 3; c++) {
+            cpx<double> u = ((cpx<double> *)a.V[c])[i], b = ((cpx<double> *)a.W[c])[i];
+            if (z) u = b = mk<double>(0, 0);
+            ((cpx<double> *)a.Zp[c])[i] = u + b;
+            ((cpx<double> *)a.Zm[c])[i] = u - b;
+        }
+    }
+}
 
 __global__ void k_mhd_combine(MhdArgs a)
 {
@@ -227,7 +247,7 @@ __global__ void __launch_bounds__(CB *(L / 8), 1) k_x_mhd(XmhdArgs a)
         cpx<double> un[8];
         // u
         stage_in<L, CB>(ldd, (const cpx<double> *)a.V[K], cols, ncs);
-        if (stage >= 2) stage_in<L, CB>(ldb, (const cpx<double> *)a.accV[K], cols, ncs);
+        if (stage >= 2) stage_in<L, CB>(ldb, (const cpx<double> *)a.accV[K], colw, ncw);
         lds_barrier();
 #pragma unroll
         for (int s = 0; s < 8; s++) {
@@ -244,11 +264,12 @@ __global__ void __launch_bounds__(CB *(L / 8), 1) k_x_mhd(XmhdArgs a)
             un[s] = vn;
         }
         lds_barrier();
-        stage_out<L, CB>(ldb, (cpx<double> *)(stage == 4 ? a.V[K] : a.accV[K]), cols, ncs);
+        if (stage == 4) stage_out<L, CB>(ldb, (cpx<double> *)a.V[K], cols, ncs);
+        else stage_out<L, CB>(ldb, (cpx<double> *)a.accV[K], colw, ncw);
         lds_barrier();
         // b
         stage_in<L, CB>(ldd, (const cpx<double> *)a.W[K], cols, ncs);
-        if (stage >= 2) stage_in<L, CB>(ldb, (const cpx<double> *)a.accW[K], cols, ncs);
+        if (stage >= 2) stage_in<L, CB>(ldb, (const cpx<double> *)a.accW[K], colw, ncw);
         lds_barrier();
         cpx<T> zp[8], zm[8];
 #pragma unroll
@@ -270,10 +291,11 @@ __global__ void __launch_bounds__(CB *(L / 8), 1) k_x_mhd(XmhdArgs a)
             zm[s] = cvt<T>(u - bn);
         }
         lds_barrier();
-        stage_out<L, CB>(ldb, (cpx<double> *)(stage == 4 ? a.W[K] : a.accW[K]), cols, ncs);
+        if (stage == 4) stage_out<L, CB>(ldb, (cpx<double> *)a.W[K], cols, ncs);
+        else stage_out<L, CB>(ldb, (cpx<double> *)a.accW[K], colw, ncw);
         lds_barrier();
-        if (stage == 4) return;  // the next step starts from the state (after its reality condition)
-        // x-inverse of the next stage input: Z+ into U[3+K], Z- into U[K]
+        // x-inverse of the next stage input (stage 4: of the next step's
+        // first stage, D(u) +- D(b) of the new state): Z+ into U[3+K], Z- into U[K]
         fft_line<L, +1>(zp, ldt + c * LP, tw, j);
 #pragma unroll
         for (int s = 0; s < 8; s++) ldt[c * LP + lpad(j + TPL * s)] = zp[s];
@@ -336,6 +358,19 @@ hipError_t launch_mhd_elsasser(const Grid &g, double *const *V, double *const *W
         a.Zm[c] = Zm[c];
     }
     hipLaunchKernelGGL(k_mhd_elsasser, dim3(4096), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_mhd_elsasser_planes(const Grid &g, double *const *V, double *const *W, double *const *Zp,
+                                      double *const *Zm, hipStream_t st)
+{
+    MhdArgs a = mhd_args(g, V, W);
+    for (int c = 0; c < 3; c++) {
+        a.Zp[c] = Zp[c];
+        a.Zm[c] = Zm[c];
+    }
+    const int n = 2 * g.nx * g.ny;
+    hipLaunchKernelGGL(k_mhd_elsasser_planes, dim3((n + 255) / 256), dim3(256), 0, st, a);
     return hipGetLastError();
 }
 
