@@ -63,3 +63,33 @@ def test_mhd_rk4_and_diagnostics(prec, tol):
         u2b, b2b = et.Compute_flux_MHD(U, B)
         log(f"  mhd flux u2b rel {rel_l2(u2b, u2b_ref):.3e} b2b rel {rel_l2(b2b, b2b_ref):.3e}")
         assert rel_l2(u2b, u2b_ref) < 1e-11 and rel_l2(b2b, b2b_ref) < 1e-11
+
+
+@pytest.mark.parametrize("strong", [True, False])
+@pytest.mark.parametrize("prec,tol", [(api.PREC_FP64, FP64_TOL), (api.PREC_FP32, 2e-5)])
+def test_mhd_steps_in_one_call(strong, prec, tol):
+    """Three steps in one call: the stage-4 x pass leaves the next step's
+    Elsasser input, and under the strong reality condition its kz = 0 and
+    Nz/2 planes are re-formed after the condition.  The initial kz = 0 planes
+    are perturbed off Hermitian symmetry so that the condition changes them
+    after the first step."""
+    N = (32, 16, 32)
+    log(f"test_mhd_steps_in_one_call strong={strong} prec={prec}")
+    o = to.Oracle(N, strong_reality=strong)
+    V, W = _fields(o, 25, 26)
+    rng = np.random.default_rng(27)
+    for F in (V, W):
+        for c in range(3):
+            F[c][:, :, 0] += 1e-3 * (rng.standard_normal(F[c].shape[:2]) + 1j * rng.standard_normal(F[c].shape[:2]))
+    nu, eta, dt = 5e-3, 3e-3, 2e-3
+    b = api.FFF_Basis(N, precision=prec, strong_reality=strong)
+    U, B = api.FluidVF(b, nu), api.FluidVF(b, eta)
+    U.set_field([dev(v) for v in V])
+    B.set_field([dev(w) for w in W])
+    fl = to.Fluid(nu, dt, 0, 0, 0, 0)
+    for _ in range(3):
+        o.rk4_step_mhd(fl, eta, V, W)
+    api.Time_advance_incompress_MHD(U, B, dt).Time_advance_step(3)
+    e = rel_l2_vec([host(v) for v in U.cvf] + [host(w) for w in B.cvf], V + W)
+    log(f"  after 3 steps: rel L2 {e:.3e}")
+    assert e < tol
