@@ -136,6 +136,38 @@ def test_rk4_steps(case):
         assert e < FP64_TOL
 
 
+@pytest.mark.parametrize("mode", ["one_call", "chained"])
+@pytest.mark.parametrize("prec,tol", [(api.PREC_FP64, FP64_TOL), (api.PREC_FP32, 2e-5)])
+def test_rk4_steps_reality_planes(mode, prec, tol):
+    """Three forced steps, in one call or in three chained calls: the stage-4
+    x pass leaves the next step's stage-1 input, and after the strong reality
+    condition its kz = 0 and Nz/2 columns (and the forcing band's slots) are
+    re-transformed.  The initial kz = 0 planes are perturbed off Hermitian
+    symmetry so that the condition changes them."""
+    N, nu, dt, force = (32, 16, 32), 0.005, 2e-3, (1, 0.1, 1.0, 3.0)
+    log(f"test_rk4_steps_reality_planes mode={mode} prec={prec}")
+    o = to.Oracle(N, dealias=to.DEALIAS_REF_CORNER, strong_reality=True)
+    V = o.init_field(kind=1, k0=4.0, kmin=1.0, kmax=10.0, energy=0.5, seed=81)
+    rng = np.random.default_rng(82)
+    for c in range(3):
+        V[c][:, :, 0] += 1e-3 * (rng.standard_normal(V[c].shape[:2]) + 1j * rng.standard_normal(V[c].shape[:2]))
+    b = api.FFF_Basis(N, precision=prec, dealias=api.DEALIAS_REF_CORNER, strong_reality=True)
+    U = api.FluidVF(b, nu)
+    U.set_field([dev(v) for v in V])
+    ta = api.Time_advance_incompress(U, dt, *force)
+    fl = to.Fluid(nu, dt, *force)
+    for _ in range(3):
+        o.rk4_step(fl, V)
+    if mode == "one_call":
+        ta.Time_advance_step(3)
+    else:
+        for _ in range(3):
+            ta.Time_advance_step(1)
+    e = rel_l2_vec([host(v) for v in U.cvf], V)
+    log(f"  after 3 steps: rel L2 {e:.3e}")
+    assert e < tol
+
+
 def test_rk4_fp32_vs_fp64():
     N = (32, 32, 32)
     o = to.Oracle(N)
