@@ -23,14 +23,16 @@ struct AdjParams {
     double scale;       // 1/(Nx Ny Nz)
 };
 
-// x-inverse of the six symmetrised fields (-i K_m M_j - i K_j M_m)/c with
-// M = P G (pairs xx, yy, zz, xy, yz, zx), c = 1 on kz = 0 and kz = Nz/2, 2
-// elsewhere.  G: FP64 boundary layout; S: six T work fields.
+// x-inverse of the symmetrised fields (-i K_m M_j - i K_j M_m)/c with
+// M = P G, c = 1 on kz = 0 and kz = Nz/2, 2 elsewhere: the pairs xx, yy, xy,
+// yz, zx (the field is trace-free, zz = -xx - yy, as K.M = 0).  G: FP64
+// boundary layout; S: five T work fields.
 template <typename T>
 hipError_t launch_xinv_sym(const Grid &g, double *const *G, void *const *S, const void *twx, hipStream_t st);
 
-// z pass: c2r of the six symmetrised lines, q_m = sum_j u_j sigma_jm against
-// the taped velocity lines, r2c of q (three T work fields).
+// z pass: c2r of the five symmetrised lines, q_m = sum_j u_j sigma_jm against
+// the taped velocity lines (sigma_zz = -sigma_xx - sigma_yy), r2c of q (three
+// T work fields).
 template <typename T>
 hipError_t launch_z_adj(const Grid &g, void *const *S, void *const *Q, void *const *tape, const void *twm,
                         const void *twn, hipStream_t st);

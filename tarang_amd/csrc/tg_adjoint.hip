@@ -8,10 +8,13 @@
 // F^T = N^-3 I diag(1/c); (i K)^T = -i K; P and D are symmetric.  So
 //   f'(V)^T G = -D diag(c) F[q],  q_m = sum_j u_j sigma_jm,
 //   sigma_jm = I(diag(1/c)(-i K_m M_j - i K_j M_m)),  M = P G,
-// which needs the stage's u (taped by the forward sweep), six inverse and
+// which needs the stage's u (taped by the forward sweep), five inverse and
 // three forward transforms: x-inverse of the symmetrised fields (built from
 // M in registers), y-inverse, z c2r + products with the tape + r2c,
-// y-forward, x-forward with the stage epilogue.
+// y-forward, x-forward with the stage epilogue.  sigma is trace-free
+// (sigma_jj summed is -2i K.M/c = 0, M being projected), so sigma_zz =
+// -sigma_xx - sigma_yy is not transformed: the z pass subtracts u_z sigma_xx
+// and u_z sigma_yy from q_z instead.
 #include "tg_adjoint.h"
 #include "tg_device.h"
 #include "tg_pass.h"
@@ -20,14 +23,15 @@ namespace tg {
 
 struct XsymArgs {
     const double *G[3];
-    void *S[6];
+    void *S[5];
     const void *tw;
     Grid g;
 };
 
 // From the cotangent G of a tile (put(k) stages component k of G into the
-// LDS tile in the stage_in layout): M = P G, the six symmetrised fields
-// (-i K_m M_j - i K_j M_m)/c, their x-inverse, stored into S.
+// LDS tile in the stage_in layout): M = P G, the five symmetrised fields
+// xx, yy, xy, yz, zx of (-i K_m M_j - i K_j M_m)/c (zz = -xx - yy), their
+// x-inverse, stored into S.
 template <typename T, int L, int CB, typename Put>
 __device__ __forceinline__ void xsym_tile(const Grid &g, char *raw, const int *colw, const cpx<T> *tw,
                                           void *const *S, Put put)
@@ -66,10 +70,10 @@ __device__ __forceinline__ void xsym_tile(const Grid &g, char *raw, const int *c
         }
     }
     const double w = (lz == 0 || lz == g.nz / 2) ? 1.0 : 0.5;
-    constexpr int pa[6] = {0, 1, 2, 0, 1, 2};
-    constexpr int pb[6] = {0, 1, 2, 1, 2, 0};
+    constexpr int pa[5] = {0, 1, 0, 1, 2};
+    constexpr int pb[5] = {0, 1, 1, 2, 0};
 #pragma unroll
-    for (int t = 0; t < 6; t++) {
+    for (int t = 0; t < 5; t++) {
         cpx<T> v[8];
 #pragma unroll
         for (int s = 0; s < 8; s++) {
@@ -110,23 +114,26 @@ __global__ void __launch_bounds__(CB *(L / 8)) k_xinv_sym(XsymArgs a)
                         [&](int k) { stage_put<L, CB>((cpx<double> *)raw, gr[k], cols); });
 }
 
-// q += u * v on the packed real points of this thread (u from the tape line).
-template <typename T, int M>
+// q += u * v (SUB: q -= u * v) on the packed real points of this thread (u
+// from the tape line).
+template <typename T, int M, bool SUB = false>
 __device__ __forceinline__ void tape_mac(cpx<T> *q, const cpx<T> *v, const cpx<T> *__restrict__ u, int j)
 {
     constexpr int TPL = M / 8;
 #pragma unroll
     for (int s = 0; s < 8; s++) {
         const cpx<T> A = u[j + TPL * s];
-        q[s] = q[s] + mk<T>(A.x * v[s].x, A.y * v[s].y);
+        const cpx<T> d = mk<T>(A.x * v[s].x, A.y * v[s].y);
+        q[s] = SUB ? q[s] - d : q[s] + d;
     }
 }
 
-// sigma lines in a.in[]: 0 xx, 1 yy, 2 zz, 3 xy, 4 yz, 5 zx.  The three
-// accumulators q_m = sum_j u_j sigma_jm are live together, so each sigma
-// line takes one c2r: a diagonal line sigma_jj adds u_j sigma_jj to q_j, an
+// sigma lines in a.in[]: 0 xx, 1 yy, 2 xy, 3 yz, 4 zx (zz = -xx - yy).  The
+// three accumulators q_m = sum_j u_j sigma_jm are live together, so each
+// sigma line takes one c2r: a diagonal line sigma_jj adds u_j sigma_jj to q_j
+// and, as its share of sigma_zz, subtracts u_z sigma_jj from q_z; an
 // off-diagonal line sigma_jm adds u_j sigma_jm to q_m and u_m sigma_jm to
-// q_j.  Six c2r and three r2c per line.
+// q_j.  Five c2r and three r2c per line.
 template <typename T, int M>
 __global__ void __launch_bounds__(256) k_z_adj(ZArgs a)
 {
@@ -148,21 +155,23 @@ __global__ void __launch_bounds__(256) k_z_adj(ZArgs a)
 #pragma unroll
         for (int s = 0; s < 8; s++) q[m][s] = mk<T>(0, 0);
 #pragma unroll 1
-    for (int t = 0; t < 6; t++) {
-        // sigma_jm of line t: (j, m) = (t, t) for t < 3, (t-3, t-2 mod 3) after
-        const int jt = t < 3 ? t : t - 3, mt = t < 3 ? t : (t == 5 ? 0 : t - 2);
+    for (int t = 0; t < 5; t++) {
+        // sigma_jm of line t: (j, m) = (t, t) for t < 2, (t-2, t-1 mod 3) after
+        const int jt = t < 2 ? t : t - 2, mt = t < 2 ? t : (t == 4 ? 0 : t - 1);
         const cpx<T> *X = (const cpx<T> *)(t == 0 ? a.in[0] : t == 1 ? a.in[1] : t == 2 ? a.in[2] :
-                                            t == 3 ? a.in[3] : t == 4 ? a.in[4] : a.in[5]);
+                                            t == 3 ? a.in[3] : a.in[4]);
         cpx<T> v[8];
         zline_c2r<T, M>(v, X + off, L, twm, twn, j);
         line_sync<M / 8>();
         const cpx<T> *uj = (const cpx<T> *)(jt == 0 ? a.tape[0] : jt == 1 ? a.tape[1] : a.tape[2]) + lsafe * M;
         const cpx<T> *um = (const cpx<T> *)(mt == 0 ? a.tape[0] : mt == 1 ? a.tape[1] : a.tape[2]) + lsafe * M;
+        const cpx<T> *uz = (const cpx<T> *)a.tape[2] + lsafe * M;
 #pragma unroll
         for (int k = 0; k < 3; k++) {
             if (k == mt) tape_mac<T, M>(q[k], v, uj, j);
-            if (t >= 3 && k == jt) tape_mac<T, M>(q[k], v, um, j);
+            if (t >= 2 && k == jt) tape_mac<T, M>(q[k], v, um, j);
         }
+        if (t < 2) tape_mac<T, M, true>(q[2], v, uz, j);
     }
 #pragma unroll
     for (int m = 0; m < 3; m++) {
@@ -174,7 +183,7 @@ __global__ void __launch_bounds__(256) k_z_adj(ZArgs a)
 struct XadjArgs {
     void *Q[3];
     double *lam[3], *acc[3], *lg[3];
-    void *S[6];  // FUSE: the next stage's six symmetrised fields
+    void *S[5];  // FUSE: the next stage's five symmetrised fields
     const void *tw;
     Grid g;
     AdjParams ap;
@@ -318,7 +327,7 @@ hipError_t launch_xinv_sym(const Grid &g, double *const *G, void *const *S, cons
 {
     XsymArgs a{};
     for (int i = 0; i < 3; i++) a.G[i] = G[i];
-    for (int i = 0; i < 6; i++) a.S[i] = S[i];
+    for (int i = 0; i < 5; i++) a.S[i] = S[i];
     a.tw = twx;
     a.g = g;
     TG_DISPATCH_L(T, g.nx, return (xinv_sym_len<T, LL>)(a, st));
@@ -330,7 +339,7 @@ hipError_t launch_z_adj(const Grid &g, void *const *S, void *const *Q, void *con
                         const void *twn, hipStream_t st)
 {
     ZArgs a{};
-    for (int i = 0; i < 6; i++) a.in[i] = S[i];
+    for (int i = 0; i < 5; i++) a.in[i] = S[i];
     for (int i = 0; i < 3; i++) {
         a.out[i] = Q[i];
         a.tape[i] = tape[i];
@@ -349,7 +358,7 @@ hipError_t launch_xfwd_adj(const Grid &g, const AdjParams &ap, void *const *Q, d
 {
     XadjArgs a{};
     if (S_next && ap.mode == ADJ_RHS) return hipErrorInvalidValue;
-    for (int i = 0; i < 6; i++) a.S[i] = S_next ? S_next[i] : nullptr;
+    for (int i = 0; i < 5; i++) a.S[i] = S_next ? S_next[i] : nullptr;
     for (int i = 0; i < 3; i++) {
         a.Q[i] = Q[i];
         a.lam[i] = lam ? lam[i] : nullptr;

@@ -752,10 +752,10 @@ template <typename T>
 int adj_stage(tg_plan *p, double *const G[3], void *const tp[3], const AdjParams &ap, double *const lam[3],
               double *const acc[3], double *const lg[3], bool sym_done = false, bool fuse_next = false)
 {
-    void *S[6] = {p->P[0], p->P[1], p->P[2], p->P[3], p->P[4], p->P[5]};
+    void *S[5] = {p->P[0], p->P[1], p->P[2], p->P[3], p->P[4]};
     void *Q[3] = {p->P[6], p->P[7], p->P[8]};
     if (!sym_done) { Prof pr(p, TG_KERNEL_X_INV_SYM); HIPCHK(launch_xinv_sym<T>(p->g, G, S, p->twx, p->stream)); }
-    { Prof pr(p, TG_KERNEL_Y_FFT); HIPCHK(launch_colfft<T>(p->g, 1, +1, S, 6, p->twy, p->stream)); }
+    { Prof pr(p, TG_KERNEL_Y_FFT); HIPCHK(launch_colfft<T>(p->g, 1, +1, S, 5, p->twy, p->stream)); }
     { Prof pr(p, TG_KERNEL_Z_ADJ); HIPCHK(launch_z_adj<T>(p->g, S, Q, tp, p->twm, p->twn, p->stream)); }
     { Prof pr(p, TG_KERNEL_Y_FFT); HIPCHK(launch_colfft<T>(p->g, 1, -1, Q, 3, p->twy, p->stream)); }
     Prof pr(p, TG_KERNEL_X_FWD_ADJ);
