@@ -94,13 +94,16 @@ def algorithmic_bytes(kernel, N, prec_bytes, nstate=3):
     nc = N[0] * N[1] * (N[2] // 2 + 1)
     Sp = nc * 2 * prec_bytes      # one complex work field in the transform precision
     S64 = nc * 16                 # one complex FP64 state field
-    if kernel == "y_fft":         # 4 stages x (3 inverse + 6 forward fields), read + write
-        return 4 * 9 * 2 * Sp
-    if kernel == "z_products":    # 4 stages x (read 3, write 6)
-        return 4 * 9 * Sp
-    if kernel == "x_nlin_rk":     # 4 stages x (read 6 products, write 3) + per component: V read 4x and
+    # an RK stage transforms 3 stage inputs and 5 products (the trace-shifted
+    # u_x^2 - u_z^2, u_y^2 - u_z^2, u_x u_y, u_y u_z, u_z u_x: the stage only uses the
+    # nonlinear term projected, which removes the gradient the shift adds)
+    if kernel == "y_fft":         # 4 stages x (3 inverse + 5 forward fields), read + write
+        return 4 * 8 * 2 * Sp
+    if kernel == "z_products":    # 4 stages x (read 3, write 5)
+        return 4 * 8 * Sp
+    if kernel == "x_nlin_rk":     # 4 stages x (read 5 products, write 3) + per component: V read 4x and
         # written once, the stage accumulator written 3x and read 3x (both FP64)
-        return 4 * 9 * Sp + nstate * 11 * S64
+        return 4 * 8 * Sp + nstate * 11 * S64
     if kernel == "reality":       # two planes of V (read+write) and the patched columns
         return 2 * 2 * nstate * N[0] * N[1] * 16 + nstate * N[0] * N[1] * 2 * (16 + 2 * prec_bytes)
     if kernel == "x_inv":
@@ -108,7 +111,7 @@ def algorithmic_bytes(kernel, N, prec_bytes, nstate=3):
     return 0
 
 
-def step_bytes_model(N, prec_bytes, nstate=3, n_tr=9):
+def step_bytes_model(N, prec_bytes, nstate=3, n_tr=8):
     """B_step = 8 (n_inv + n_fwd) S_p + 11 n_state S64 (two-sweep model: the FP64
     state read at every stage and written once, the FP64 stage accumulator written
     and read three times each)."""

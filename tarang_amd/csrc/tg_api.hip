@@ -409,11 +409,19 @@ int run_inverse(tg_plan *p, const double *spec_dev, double *real_dev, int sincos
 
 // Velocity x,y-inverse (U[0..2]) -> products -> y-forward (P[0..5])
 template <typename T>
-int ns_middle(tg_plan *p)
+// trace_free: the nonlinear term is only used projected (an RK stage, or
+// Compute_rhs), so the products are the five u_x^2 - u_z^2, u_y^2 - u_z^2,
+// u_x u_y, u_y u_z, u_z u_x: the term less the gradient i K F[u_z^2], which
+// the projection removes exactly, mode by mode.
+int ns_middle(tg_plan *p, bool trace_free)
 {
+    const int np = trace_free ? 5 : 6;
     { Prof pr(p, TG_KERNEL_Y_FFT); HIPCHK(launch_colfft<T>(p->g, 1, +1, p->U, 3, p->twy, p->stream)); }
-    { Prof pr(p, TG_KERNEL_Z_PRODUCTS); HIPCHK(launch_z_products<T>(p->g, p->U, p->P, 0, p->twm, p->twn, p->stream)); }
-    { Prof pr(p, TG_KERNEL_Y_FFT); HIPCHK(launch_colfft<T>(p->g, 1, -1, p->P, 6, p->twy, p->stream)); }
+    {
+        Prof pr(p, TG_KERNEL_Z_PRODUCTS);
+        HIPCHK(launch_z_products<T>(p->g, p->U, p->P, trace_free ? 2 : 0, p->twm, p->twn, p->stream));
+    }
+    { Prof pr(p, TG_KERNEL_Y_FFT); HIPCHK(launch_colfft<T>(p->g, 1, -1, p->P, np, p->twy, p->stream)); }
     return TG_OK;
 }
 
@@ -459,10 +467,11 @@ int run_nlin(tg_plan *p, double *const V[3], double *const out[3], const tg_flui
     // Compute_nlin dealiases the state in place (compute_nlin.cc:61-62)
     if (p->g.dealias) HIPCHK(launch_dealias(p->g, V, 3, p->stream));
     HIPCHK(launch_xinv_state<T>(p->g, p->band, V, 3, p->U, p->slots[0], p->band.kind == 1 ? 2 : 0, p->twx, p->stream));
-    int rc = ns_middle<T>(p);
+    int rc = ns_middle<T>(p, write_rhs != 0);
     if (rc) return rc;
     RkParams rk = rk_params(p, f, 0);
     rk.write_rhs = write_rhs;
+    rk.trace_free = write_rhs != 0;
     HIPCHK(launch_x_nlin<T>(p->g, p->band, rk, p->P, nullptr, nullptr, nullptr, p->slots[0], nullptr, out, p->twx,
                             p->stream));
     p->unext_valid = false;
@@ -522,9 +531,10 @@ int run_rk4(tg_plan *p, double *const V[3], const tg_fluid *f, int nsteps, int f
     }
     for (int step = 0; step < nsteps; step++) {
         for (int s = 1; s <= 4; s++) {
-            rc = ns_middle<T>(p);
+            rc = ns_middle<T>(p, true);
             if (rc) return rc;
             RkParams rk = rk_params(p, f, s);
+            rk.trace_free = 1;
             if (s == 4) rk.nonfinite = p->nonfinite_dev;
             Prof pr(p, TG_KERNEL_X_NLIN_RK);
             HIPCHK(launch_x_nlin<T>(p->g, p->band, rk, p->P, V, p->acc, p->U, p->slots[(s - 1) & 1],
@@ -730,10 +740,11 @@ int record_steps(tg_plan *p, double *const V[3], const tg_fluid *f, int nsteps)
             { Prof pr(p, TG_KERNEL_Y_FFT); HIPCHK(launch_colfft<T>(p->g, 1, +1, p->U, 3, p->twy, p->stream)); }
             {
                 Prof pr(p, TG_KERNEL_Z_PRODUCTS);
-                HIPCHK(launch_z_products<T>(p->g, p->U, p->P, 0, p->twm, p->twn, p->stream, tp));
+                HIPCHK(launch_z_products<T>(p->g, p->U, p->P, 2, p->twm, p->twn, p->stream, tp));
             }
-            { Prof pr(p, TG_KERNEL_Y_FFT); HIPCHK(launch_colfft<T>(p->g, 1, -1, p->P, 6, p->twy, p->stream)); }
+            { Prof pr(p, TG_KERNEL_Y_FFT); HIPCHK(launch_colfft<T>(p->g, 1, -1, p->P, 5, p->twy, p->stream)); }
             RkParams rk = rk_params(p, f, s);
+            rk.trace_free = 1;
             Prof pr(p, TG_KERNEL_X_NLIN_RK);
             HIPCHK(launch_x_nlin<T>(p->g, nb, rk, p->P, V, p->acc, p->U, nullptr, nullptr, nullptr, p->twx,
                                     p->stream, p->x_tile_columns));

@@ -63,8 +63,10 @@ hipError_t launch_z_r2c(const Grid &g, const double *const *R, int ncomp, void *
                         const void *twn, hipStream_t st);
 
 // z pass of the NS nonlinear term: 3 velocity lines in, 6 product lines out
-// (order xx, yy, zz, xy, yz, zx).  giver != 0: 6 lines in (u then w),
-// 9 products out (u_j w_i, order j-major: (x,x),(x,y),(x,z),(y,x)...).
+// (order xx, yy, zz, xy, yz, zx).  giver 2: the five trace-shifted products
+// xx - zz, yy - zz, xy, yz, zx (for a nonlinear term that is only used
+// projected).  giver 1: 6 lines in (u then w), 9 products out (u_j w_i,
+// order j-major: (x,x),(x,y),(x,z),(y,x)...).
 // tape (NS form only): also store the three real velocity lines, each line as
 // Nz/2 packed points (x[2n] + i x[2n+1]) at line*(Nz/2).
 template <typename T>
@@ -81,6 +83,8 @@ struct RkParams {
     double scale;         // 1/(Nx Ny Nz)
     int *nonfinite;       // stage 4: set to 1 if the new state holds a
                           // non-finite value (null: not checked)
+    int trace_free;       // the products are the five trace-shifted ones
+                          // (stages 1-4, or stage 0 with write_rhs)
 };
 
 // Fused x pass: forward x-FFT of the 6 (giver: 9) products, nonlinear term, RHS and the

@@ -269,7 +269,10 @@ __global__ void __launch_bounds__(256) k_z_r2c(ZArgs a)
 
 // z pass of the nonlinear term.  NS: products u_a u_b for the pairs
 // (0,0),(1,1),(2,2),(0,1),(1,2),(2,0) -- the six real-space products of
-// Compute_nlin_diag / Compute_nlin_offdiag (compute_diag_offdiag.cc:47-112).
+// Compute_nlin_diag / Compute_nlin_offdiag (compute_diag_offdiag.cc:47-112);
+// a.ncomp == 5: the trace-shifted five u_x^2 - u_z^2, u_y^2 - u_z^2, u_x u_y,
+// u_y u_z, u_z u_x (the nonlinear term less the gradient i K F[u_z^2], which
+// the pressure projection removes).
 // Giver: out[3*i + j] = u_j w_i (compute_diag_offdiag.cc:329-414).
 // FP32 lines of 256 points (512-point z axis): four waves per SIMD (the
 // allocation fits 128 VGPRs without spills and the pass runs 6% faster at
@@ -311,7 +314,26 @@ __global__ void __launch_bounds__(256, (sizeof(T) == 4 && !GIVER && M == 256) ? 
         cpx<T> *X = (cpx<T> *)a.out[o] + lsafe * a.g.nzp;
         zline_r2c_post<T, M>(p, X, L, twm, twn, j, valid);
     };
-    if (!GIVER) {
+    if (!GIVER && a.ncomp == 5) {
+#pragma unroll
+        for (int o = 0; o < 2; o++) {
+            cpx<T> p[8];
+#pragma unroll
+            for (int s = 0; s < 8; s++)
+                p[s] = mk<T>(u[o][s].x * u[o][s].x - u[2][s].x * u[2][s].x,
+                             u[o][s].y * u[o][s].y - u[2][s].y * u[2][s].y);
+            emit(p, o);
+        }
+        const int pa[3] = {0, 1, 2};
+        const int pb[3] = {1, 2, 0};
+#pragma unroll
+        for (int o = 0; o < 3; o++) {
+            cpx<T> p[8];
+#pragma unroll
+            for (int s = 0; s < 8; s++) p[s] = mk<T>(u[pa[o]][s].x * u[pb[o]][s].x, u[pa[o]][s].y * u[pb[o]][s].y);
+            emit(p, 2 + o);
+        }
+    } else if (!GIVER) {
         const int pa[6] = {0, 1, 2, 0, 1, 2};
         const int pb[6] = {0, 1, 2, 1, 2, 0};
 #pragma unroll
@@ -408,6 +430,8 @@ static __constant__ int kNsOrder[6] = {0, 1, 2, 3, 4, 5};
 // NS products xx, yy, zz, xy, yz, zx: derivative applied for N_0, N_1, N_2
 // (Compute_nlin_diag, then Compute_nlin_offdiag's Add_*deriv calls).
 static __constant__ int kNsSel[6][3] = {{1, 0, 0}, {0, 2, 0}, {0, 0, 3}, {2, 1, 0}, {0, 3, 2}, {3, 0, 1}};
+// The trace-shifted five (xx - zz, yy - zz, xy, yz, zx; projected stages only).
+static __constant__ int kNsSel5[5][3] = {{1, 0, 0}, {0, 2, 0}, {2, 1, 0}, {0, 3, 2}, {3, 0, 1}};
 static __constant__ int kGiverOrder[9] = {0, 4, 8, 3, 1, 7, 5, 2, 6};
 
 template <typename T, int L, int CB, int MODE>
@@ -452,12 +476,15 @@ __global__ void __launch_bounds__(CB *(L / 8), 2) k_x_nlin(XnlArgs a)
     cpx<T> N[3][8];
 #pragma unroll
     for (int s = 0; s < 8; s++) N[0][s] = N[1][s] = N[2][s] = mk<T>(0, 0);
-    constexpr int NPROD = GIVER ? 9 : 6;
+    // five trace-shifted products when the nonlinear term is only used
+    // projected (its gradient part i K F[u_z^2] drops out)
+    const bool tf = !GIVER && a.rk.trace_free;
+    const int nprod = GIVER ? 9 : (tf ? 5 : 6);
     constexpr bool DMA = tile_dma_ok<T, L, CB>();
     const int cw0 = tile * CB;  // first padded work column of the tile
     if constexpr (DMA) dma_tile<T, L, CB>((const cpx<T> *)a.P[GIVER ? kGiverOrder[0] : kNsOrder[0]], cw0, ncw, rawB);
 #pragma unroll 1
-    for (int pi = 0; pi < NPROD; pi++) {
+    for (int pi = 0; pi < nprod; pi++) {
         const int p = GIVER ? kGiverOrder[pi] : kNsOrder[pi];
         int sel0, sel1, sel2;
         if (MODE == 1) {
@@ -470,6 +497,10 @@ __global__ void __launch_bounds__(CB *(L / 8), 2) k_x_nlin(XnlArgs a)
             sel0 = p % 3 == 0 ? p / 3 + 1 : 0;
             sel1 = p % 3 == 1 ? p / 3 + 1 : 0;
             sel2 = p % 3 == 2 ? p / 3 + 1 : 0;
+        } else if (tf) {
+            sel0 = kNsSel5[p][0];
+            sel1 = kNsSel5[p][1];
+            sel2 = kNsSel5[p][2];
         } else {
             sel0 = kNsSel[p][0];
             sel1 = kNsSel[p][1];
@@ -489,7 +520,7 @@ __global__ void __launch_bounds__(CB *(L / 8), 2) k_x_nlin(XnlArgs a)
 #pragma unroll
             for (int s = 0; s < 8; s++) v[s] = dma_tile_read<T, L, CB>(rawB, jj + TPL * s, c);
             lds_barrier();
-            if (pi + 1 < NPROD)
+            if (pi + 1 < nprod)
                 dma_tile<T, L, CB>((const cpx<T> *)a.P[GIVER ? kGiverOrder[pi + 1] : kNsOrder[pi + 1]], cw0, ncw,
                                    rawB);
             else if (a.rk.stage > 0)
@@ -989,10 +1020,13 @@ hipError_t launch_z_products(const Grid &g, void *const *U, void *const *P, int 
                              const void *twn, hipStream_t st, void *const *tape)
 {
     ZArgs a{};
+    // giver 2: the NS form with the five trace-shifted products
+    a.ncomp = giver == 2 ? 5 : 6;
+    giver = giver == 1;
     if (tape && !giver)
         for (int i = 0; i < 3; i++) a.tape[i] = tape[i];
     for (int i = 0; i < (giver ? 6 : 3); i++) a.in[i] = U[i];
-    for (int i = 0; i < (giver ? 9 : 6); i++) a.out[i] = P[i];
+    for (int i = 0; i < (giver ? 9 : a.ncomp); i++) a.out[i] = P[i];
     a.twm = twm;
     a.twn = twn;
     a.g = g;
