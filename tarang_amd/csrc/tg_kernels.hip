@@ -277,7 +277,10 @@ __global__ void __launch_bounds__(256) k_z_r2c(ZArgs a)
 // FP32 lines of 256 points (512-point z axis): four waves per SIMD (the
 // allocation fits 128 VGPRs without spills and the pass runs 6% faster at
 // 512^3 than with the three waves the unconstrained allocation gives).
-template <typename T, int M, bool GIVER>
+// FIVE: the NS form with the five trace-shifted products (an instance of its
+// own: with both NS forms in one kernel the FP32 instance spilled under its
+// four-wave bound).
+template <typename T, int M, bool GIVER, bool FIVE = false>
 __global__ void __launch_bounds__(256, (sizeof(T) == 4 && !GIVER && M == 256) ? 4 : 1) k_z_products(ZArgs a)
 {
     constexpr int TPL = M / 8;
@@ -314,16 +317,9 @@ __global__ void __launch_bounds__(256, (sizeof(T) == 4 && !GIVER && M == 256) ? 
         cpx<T> *X = (cpx<T> *)a.out[o] + lsafe * a.g.nzp;
         zline_r2c_post<T, M>(p, X, L, twm, twn, j, valid);
     };
-    if (!GIVER && a.ncomp == 5) {
-#pragma unroll
-        for (int o = 0; o < 2; o++) {
-            cpx<T> p[8];
-#pragma unroll
-            for (int s = 0; s < 8; s++)
-                p[s] = mk<T>(u[o][s].x * u[o][s].x - u[2][s].x * u[2][s].x,
-                             u[o][s].y * u[o][s].y - u[2][s].y * u[2][s].y);
-            emit(p, o);
-        }
+    if constexpr (!GIVER && FIVE) {
+        // the cross products first, then the two shifted squares (u_z is
+        // last used there)
         const int pa[3] = {0, 1, 2};
         const int pb[3] = {1, 2, 0};
 #pragma unroll
@@ -333,7 +329,16 @@ __global__ void __launch_bounds__(256, (sizeof(T) == 4 && !GIVER && M == 256) ? 
             for (int s = 0; s < 8; s++) p[s] = mk<T>(u[pa[o]][s].x * u[pb[o]][s].x, u[pa[o]][s].y * u[pb[o]][s].y);
             emit(p, 2 + o);
         }
-    } else if (!GIVER) {
+#pragma unroll
+        for (int o = 0; o < 2; o++) {
+            cpx<T> p[8];
+#pragma unroll
+            for (int s = 0; s < 8; s++)
+                p[s] = mk<T>(u[o][s].x * u[o][s].x - u[2][s].x * u[2][s].x,
+                             u[o][s].y * u[o][s].y - u[2][s].y * u[2][s].y);
+            emit(p, o);
+        }
+    } else if constexpr (!GIVER) {
         const int pa[6] = {0, 1, 2, 0, 1, 2};
         const int pb[6] = {0, 1, 2, 1, 2, 0};
 #pragma unroll
@@ -885,6 +890,7 @@ hipError_t z_products_len(const ZArgs &a, int giver, hipStream_t st)
 {
     const dim3 grid = z_grid<MM>(a.g), block(z_lines_per_block<MM>() * (MM / 8));
     if (giver) hipLaunchKernelGGL((k_z_products<T, MM, true>), grid, block, 0, st, a);
+    else if (a.ncomp == 5) hipLaunchKernelGGL((k_z_products<T, MM, false, true>), grid, block, 0, st, a);
     else hipLaunchKernelGGL((k_z_products<T, MM, false>), grid, block, 0, st, a);
     return hipGetLastError();
 }
