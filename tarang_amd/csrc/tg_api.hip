@@ -610,10 +610,11 @@ int run_rk4_mhd(tg_plan *p, double *const V[3], double *const W[3], const tg_flu
         for (int s = 1; s <= 4; s++) {
             { Prof pr(p, TG_KERNEL_Y_FFT); HIPCHK(launch_colfft<T>(p->g, 1, +1, p->U, 6, p->twy, p->stream)); }
             {
+                // the eight trace-shifted products (P_zz enters N_u only as a gradient)
                 Prof pr(p, TG_KERNEL_Z_PRODUCTS);
-                HIPCHK(launch_z_products<T>(p->g, p->U, p->P, 1, p->twm, p->twn, p->stream));
+                HIPCHK(launch_z_products<T>(p->g, p->U, p->P, 3, p->twm, p->twn, p->stream));
             }
-            { Prof pr(p, TG_KERNEL_Y_FFT); HIPCHK(launch_colfft<T>(p->g, 1, -1, p->P, 9, p->twy, p->stream)); }
+            { Prof pr(p, TG_KERNEL_Y_FFT); HIPCHK(launch_colfft<T>(p->g, 1, -1, p->P, 8, p->twy, p->stream)); }
             // one x pass: both nonlinear terms, the stage for u and b, the next Z+- x-inverse
             XmhdArgs xa{};
             for (int i = 0; i < 9; i++) xa.P[i] = p->P[i];

@@ -66,7 +66,9 @@ hipError_t launch_z_r2c(const Grid &g, const double *const *R, int ncomp, void *
 // (order xx, yy, zz, xy, yz, zx).  giver 2: the five trace-shifted products
 // xx - zz, yy - zz, xy, yz, zx (for a nonlinear term that is only used
 // projected).  giver 1: 6 lines in (u then w), 9 products out (u_j w_i,
-// order j-major: (x,x),(x,y),(x,z),(y,x)...).
+// order j-major: (x,x),(x,y),(x,z),(y,x)...); giver 3: the first eight of
+// those with P_xx and P_yy less P_zz (the MHD stage, whose velocity term is
+// only used projected).
 // tape (NS form only): also store the three real velocity lines, each line as
 // Nz/2 packed points (x[2n] + i x[2n+1]) at line*(Nz/2).
 template <typename T>

@@ -279,7 +279,9 @@ __global__ void __launch_bounds__(256) k_z_r2c(ZArgs a)
 // 512^3 than with the three waves the unconstrained allocation gives).
 // FIVE: the NS form with the five trace-shifted products (an instance of its
 // own: with both NS forms in one kernel the FP32 instance spilled under its
-// four-wave bound).
+// four-wave bound); with GIVER, the eight trace-shifted products of the MHD
+// stage: P_xx - P_zz, P_yy - P_zz and the six off-diagonal ones (P_zz is
+// not stored).
 template <typename T, int M, bool GIVER, bool FIVE = false>
 __global__ void __launch_bounds__(256, (sizeof(T) == 4 && !GIVER && M == 256) ? 4 : 1) k_z_products(ZArgs a)
 {
@@ -347,6 +349,34 @@ __global__ void __launch_bounds__(256, (sizeof(T) == 4 && !GIVER && M == 256) ? 
 #pragma unroll
             for (int s = 0; s < 8; s++) p[s] = mk<T>(u[pa[o]][s].x * u[pb[o]][s].x, u[pa[o]][s].y * u[pb[o]][s].y);
             emit(p, o);
+        }
+    } else if constexpr (FIVE) {
+        // w_z first: P_zz = u_z w_z is kept, in this thread's own LDS slots
+        // (registers would cost the FP64 instance half its occupancy), for the
+        // two shifted diagonal products
+        __shared__ cpx<T> dl[LPB * M];
+        cpx<T> *D = dl + li * M;
+#pragma unroll 1
+        for (int ii = 0; ii < 3; ii++) {
+            const int i = ii == 0 ? 2 : ii - 1;
+            cpx<T> w[8];
+            zline_c2r<T, M>(w, (const cpx<T> *)a.in[3 + i] + lsafe * a.g.nzp, L, twm, twn, j);
+            line_sync<M / 8>();
+            if (ii == 0) {
+#pragma unroll
+                for (int s = 0; s < 8; s++) D[j + TPL * s] = mk<T>(u[2][s].x * w[s].x, u[2][s].y * w[s].y);
+            }
+#pragma unroll
+            for (int jj = 0; jj < 3; jj++) {
+                if (jj == i && i == 2) continue;
+                cpx<T> p[8];
+#pragma unroll
+                for (int s = 0; s < 8; s++) {
+                    p[s] = mk<T>(u[jj][s].x * w[s].x, u[jj][s].y * w[s].y);
+                    if (jj == i) p[s] = p[s] - D[j + TPL * s];
+                }
+                emit(p, 3 * i + jj);
+            }
         }
     } else {
 #pragma unroll 1
@@ -889,7 +919,8 @@ template <typename T, int MM>
 hipError_t z_products_len(const ZArgs &a, int giver, hipStream_t st)
 {
     const dim3 grid = z_grid<MM>(a.g), block(z_lines_per_block<MM>() * (MM / 8));
-    if (giver) hipLaunchKernelGGL((k_z_products<T, MM, true>), grid, block, 0, st, a);
+    if (giver && a.ncomp == 8) hipLaunchKernelGGL((k_z_products<T, MM, true, true>), grid, block, 0, st, a);
+    else if (giver) hipLaunchKernelGGL((k_z_products<T, MM, true>), grid, block, 0, st, a);
     else if (a.ncomp == 5) hipLaunchKernelGGL((k_z_products<T, MM, false, true>), grid, block, 0, st, a);
     else hipLaunchKernelGGL((k_z_products<T, MM, false>), grid, block, 0, st, a);
     return hipGetLastError();
@@ -1026,13 +1057,14 @@ hipError_t launch_z_products(const Grid &g, void *const *U, void *const *P, int 
                              const void *twn, hipStream_t st, void *const *tape)
 {
     ZArgs a{};
-    // giver 2: the NS form with the five trace-shifted products
-    a.ncomp = giver == 2 ? 5 : 6;
-    giver = giver == 1;
+    // giver 2: the NS form with the five trace-shifted products; 3: the giver
+    // form with the eight trace-shifted ones (MHD stage)
+    a.ncomp = giver == 2 ? 5 : (giver == 3 ? 8 : (giver == 1 ? 9 : 6));
+    giver = giver == 1 || giver == 3;
     if (tape && !giver)
         for (int i = 0; i < 3; i++) a.tape[i] = tape[i];
     for (int i = 0; i < (giver ? 6 : 3); i++) a.in[i] = U[i];
-    for (int i = 0; i < (giver ? 9 : a.ncomp); i++) a.out[i] = P[i];
+    for (int i = 0; i < a.ncomp; i++) a.out[i] = P[i];
     a.twm = twm;
     a.twn = twn;
     a.g = g;

@@ -13,7 +13,8 @@ hipError_t launch_mhd_elsasser_planes(const Grid &g, double *const *V, double *c
 // NU <- (NU + NW)/2, NW <- (NU - NW)/2 in place (MHD::Elsasser_to_UB_nlin)
 hipError_t launch_mhd_combine(const Grid &g, double *const *NU, double *const *NW, hipStream_t st);
 // Fused x pass of an MHD RK stage (1..4): P[3i+j] = FT[Z-_j Z+_i] (work
-// layout), state V (u), W (b) in the FP64 boundary layout, accumulators
+// layout; the eight trace-shifted products, P_xx and P_yy less P_zz, P[8]
+// not read), state V (u), W (b) in the FP64 boundary layout, accumulators
 // accV, accW FP64 in the padded work layout; every stage writes the x-inverse
 // of the next stage's Z- into U[0..2] and Z+ into U[3..5] (stage 4: of the
 // next step's first stage).

@@ -8,8 +8,8 @@
 //   k_mhd_elsasser  Z+ = D(u) + D(b), Z- = D(u) - D(b)   (stage-1 input)
 //   k_mhd_combine   N_u = (N+ + N-)/2, N_b = (N+ - N-)/2 (MHD.cc:150-163)
 //                   for Compute_nlin
-//   k_x_mhd         the fused x pass of an RK stage: x-forward of the nine
-//                   products, N+ and N- together, their combination as
+//   k_x_mhd         the fused x pass of an RK stage: x-forward of the eight
+//                   trace-shifted products, N+ and N- together, their combination as
 //                   above, RHS_u = -P N_u, RHS_b = -N_b (Tadvance_functions.cc:
 //                   262-270), the Lawson RK4 stage for u (nu) and b (eta)
 //                   (RK_VF_MHD.cc:126-190), and the x-inverse of the next
@@ -132,7 +132,11 @@ __global__ void k_mhd_combine(MhdArgs a)
 // of the padded (ly, lz) space by all Nx, the products arriving by
 // direct-to-LDS loads into tile B while the previous one is transformed.
 // ---------------------------------------------------------------------------
-static __constant__ int kMhdOrder[9] = {0, 4, 8, 3, 1, 7, 5, 2, 6};  // giver-form product order
+// giver-form product order of the eight trace-shifted products (P_xx - P_zz,
+// P_yy - P_zz, the off-diagonal ones; P_zz itself is not formed): the stage
+// uses N_u = (N+ + N-)/2 only projected, and the shift changes N+ and N- by
+// the same gradient -i K P_zz, so N_b = (N+ - N-)/2 is unchanged
+static __constant__ int kMhdOrder[8] = {0, 4, 3, 1, 7, 5, 2, 6};
 
 template <typename T, int L, int CB>
 __global__ void __launch_bounds__(CB *(L / 8), 1) k_x_mhd(XmhdArgs a)
@@ -171,7 +175,7 @@ __global__ void __launch_bounds__(CB *(L / 8), 1) k_x_mhd(XmhdArgs a)
     const int cw0 = tile * CB;
     if constexpr (DMA) dma_tile<T, L, CB>((const cpx<T> *)a.P[kMhdOrder[0]], cw0, ncw, rawB);
 #pragma unroll 1
-    for (int pi = 0; pi < 9; pi++) {
+    for (int pi = 0; pi < 8; pi++) {
         const int p = kMhdOrder[pi];
         const int pii = p / 3, pjj = p % 3;
         int jj = j;
@@ -183,7 +187,7 @@ __global__ void __launch_bounds__(CB *(L / 8), 1) k_x_mhd(XmhdArgs a)
 #pragma unroll
             for (int s = 0; s < 8; s++) v[s] = dma_tile_read<T, L, CB>(rawB, jj + TPL * s, c);
             lds_barrier();
-            if (pi + 1 < 9) dma_tile<T, L, CB>((const cpx<T> *)a.P[kMhdOrder[pi + 1]], cw0, ncw, rawB);
+            if (pi + 1 < 8) dma_tile<T, L, CB>((const cpx<T> *)a.P[kMhdOrder[pi + 1]], cw0, ncw, rawB);
         } else {
             stage_in<L, CB>(ldt, (const cpx<T> *)a.P[p], colw, ncw);
             lds_barrier();
