@@ -926,11 +926,17 @@ RbcParams rbc_params(const tg_plan *p, const tg_rbc *r, int stage)
 
 // y-inverse of U[0..3], z products, y-forward of P[0..8]
 template <typename T>
-int rbc_middle(tg_plan *p)
+// shift: an RK stage, whose velocity products are trace-shifted (xx - zz,
+// yy - zz; P[2], zz, neither formed nor transformed)
+int rbc_middle(tg_plan *p, bool shift)
 {
+    void *F[9];
+    int nf = 0;
+    for (int i = 0; i < 9; i++)
+        if (!(shift && i == 2)) F[nf++] = p->P[i];
     { Prof pr(p, TG_KERNEL_Y_FFT); HIPCHK(launch_colfft<T>(p->g, 1, +1, p->U, 4, p->twy, p->stream)); }
-    { Prof pr(p, TG_KERNEL_Z_RBC); HIPCHK(launch_z_rbc<T>(p->g, p->U, p->P, p->twm, p->twn, p->stream)); }
-    { Prof pr(p, TG_KERNEL_Y_FFT); HIPCHK(launch_colfft<T>(p->g, 1, -1, p->P, 9, p->twy, p->stream)); }
+    { Prof pr(p, TG_KERNEL_Z_RBC); HIPCHK(launch_z_rbc<T>(p->g, p->U, p->P, p->twm, p->twn, p->stream, shift)); }
+    { Prof pr(p, TG_KERNEL_Y_FFT); HIPCHK(launch_colfft<T>(p->g, 1, -1, F, nf, p->twy, p->stream)); }
     return TG_OK;
 }
 
@@ -954,7 +960,7 @@ int run_nlin_rbc(tg_plan *p, double *const V[3], double *T_, double *const out[3
         Prof pr(p, TG_KERNEL_X_SC);
         HIPCHK(launch_xinv_sc<T>(p->g, S, 4, kRbcStateParity, p->U, 0, p->twx, p->twq, p->stream));
     }
-    rc = rbc_middle<T>(p);
+    rc = rbc_middle<T>(p, false);
     if (rc) return rc;
     RbcParams rp = rbc_params(p, r, 0);
     rp.write_rhs = r ? 1 : 0;
@@ -982,7 +988,7 @@ int run_rk4_rbc(tg_plan *p, double *const V[3], double *T_, const tg_rbc *r, int
     }
     for (int step = 0; step < nsteps; step++) {
         for (int s = 1; s <= 4; s++) {
-            rc = rbc_middle<T>(p);
+            rc = rbc_middle<T>(p, true);
             if (rc) return rc;
             // force of stage s: the state (stage 1) or the undealiased stage
             // input written by stage s-1, ping-pong by stage parity
@@ -990,6 +996,7 @@ int run_rk4_rbc(tg_plan *p, double *const V[3], double *T_, const tg_rbc *r, int
             const double *f_in[2] = {s == 1 ? T_ : p->rbc[1 + 2 * rd], s == 1 ? V[0] : p->rbc[2 + 2 * rd]};
             double *f_out[2] = {p->rbc[1 + 2 * wr], p->rbc[2 + 2 * wr]};
             RbcParams rp = rbc_params(p, r, s);
+            rp.trace_free = 1;
             if (s == 4) rp.nonfinite = p->nonfinite_dev;
             Prof pr(p, TG_KERNEL_X_RBC);
             HIPCHK(launch_x_rbc<T>(p->g, rp, p->P, S, acc, p->U, f_in, f_out, nullptr, p->twx, p->twq, p->stream));

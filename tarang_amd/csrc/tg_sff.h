@@ -24,7 +24,7 @@ hipError_t launch_xfwd_sc(const Grid &g, void *const *W, int ncomp, int parity, 
 // products out: xx, yy, zz, xy, yz, zx, x theta, y theta, z theta.
 template <typename T>
 hipError_t launch_z_rbc(const Grid &g, void *const *U, void *const *P, const void *twm, const void *twn,
-                        hipStream_t st);
+                        hipStream_t st, bool shift = false);
 
 struct RbcParams {
     int stage;            // 1..4; 0 = nonlinear terms (or right-hand sides) only
@@ -35,6 +35,8 @@ struct RbcParams {
     double scale;         // 1/(2 Nx Ny Nz)
     int *nonfinite;       // stage 4: set to 1 if the new state holds a
                           // non-finite value (null: not checked)
+    int trace_free;       // velocity products trace-shifted (xx - zz, yy - zz,
+                          // zz not formed): an RK stage
 };
 
 // Fused x pass of an RBC stage: sin/cos x-forward of the 9 products, N and

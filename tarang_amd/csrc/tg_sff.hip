@@ -237,7 +237,11 @@ __device__ __forceinline__ void *sel_ptr(void *const *p, int i, int n)
 
 // Two waves per SIMD also for FP64 512-point lines (a few spilled registers
 // there; 19.3 ms per stage at 512x1024x1024 against 21.0 ms at one wave).
-template <typename T, int M>
+// SHIFT: an RK stage's velocity products are the trace-shifted u_x^2 - u_z^2,
+// u_y^2 - u_z^2 and the off-diagonal ones (out[2], zz, not written): the stage
+// uses the velocity term only projected, and the SFF projection removes the
+// gradient of u_z^2 that the shift adds.
+template <typename T, int M, bool SHIFT = false>
 __global__ void __launch_bounds__(256, 2) k_z_rbc(ZArgs a)
 {
     constexpr int TPL = M / 8;
@@ -266,10 +270,27 @@ __global__ void __launch_bounds__(256, 2) k_z_rbc(ZArgs a)
         for (int s = 0; s < 8; s++) p[s] = mk<T>(x[s].x * y[s].x, x[s].y * y[s].y);
         zline_r2c_post<T, M>(p, (cpx<T> *)a.out[o] + off, L, twm, twn, j, valid);
     };
+    if constexpr (SHIFT) {
+        // the off-diagonal products first, then the shifted squares (u_z is
+        // last used there)
+        emit(u[0], u[1], rbc_pair(0, 1));
+        emit(u[1], u[2], rbc_pair(1, 2));
+        emit(u[0], u[2], rbc_pair(0, 2));
 #pragma unroll
-    for (int ia = 0; ia < 3; ia++)
+        for (int ia = 0; ia < 2; ia++) {
+            cpx<T> p[8];
 #pragma unroll
-        for (int ib = ia; ib < 3; ib++) emit(u[ia], u[ib], rbc_pair(ia, ib));
+            for (int s = 0; s < 8; s++)
+                p[s] = mk<T>(u[ia][s].x * u[ia][s].x - u[2][s].x * u[2][s].x,
+                             u[ia][s].y * u[ia][s].y - u[2][s].y * u[2][s].y);
+            zline_r2c_post<T, M>(p, (cpx<T> *)a.out[ia] + off, L, twm, twn, j, valid);
+        }
+    } else {
+#pragma unroll
+        for (int ia = 0; ia < 3; ia++)
+#pragma unroll
+            for (int ib = ia; ib < 3; ib++) emit(u[ia], u[ib], rbc_pair(ia, ib));
+    }
     cpx<T> t[8];
     zline_c2r<T, M>(t, (const cpx<T> *)a.in[3] + off, L, twm, twn, j);
     line_sync<M / 8>();
@@ -505,12 +526,18 @@ __global__ void __launch_bounds__(CB *(L / 8), 2) k_x_rbc(XrbcArgs a)
 #pragma unroll
     for (int s = 0; s < 8; s++) N[0][s] = N[1][s] = N[2][s] = mk<T>(0, 0);
     if constexpr (DMA) dma_tile<T, L, CB>((const cpx<T> *)a.P[0], cw0, ncw, rawB);
+    // trace-shifted products (an RK stage): xx - zz, yy - zz, xy, yz, zx, the
+    // zz slot skipped
+    const bool shift = rp.trace_free != 0;
+    const int nvp = shift ? 5 : 6;
 #pragma unroll 1
-    for (int p = 0; p < 6; p++) {
+    for (int q = 0; q < nvp; q++) {
+        const int p = shift && q >= 2 ? q + 1 : q;
+        const int pn = q + 1 < nvp ? (shift && q + 1 >= 2 ? q + 2 : q + 1) : -1;
         int jj = j;
         asm volatile("" : "+v"(jj));
         cpx<T> v[8];
-        load_product(v, p, p < 5 ? p + 1 : -1, jj, p == 3 || p == 5);
+        load_product(v, p, pn, jj, p == 3 || p == 5);
         sc_forward_core<L>(v, ldt + c * LP, twl, twql, jj, p == 3 || p == 5);
         const RbcTerm<T> t0(rbc_sel(p, 0), Kyt, Kzt), t1(rbc_sel(p, 1), Kyt, Kzt), t2(rbc_sel(p, 2), Kyt, Kzt);
 #pragma unroll
@@ -635,7 +662,8 @@ hipError_t z_rbc_len(const ZArgs &a, hipStream_t st)
     constexpr int LPB = z_lines_per_block<MM>();
     const size_t nlines = (size_t)a.g.nx * a.g.ny;
     dim3 grid((unsigned)((nlines + LPB - 1) / LPB));
-    hipLaunchKernelGGL((k_z_rbc<T, MM>), grid, dim3(LPB * (MM / 8)), 0, st, a);
+    if (a.ncomp == 8) hipLaunchKernelGGL((k_z_rbc<T, MM, true>), grid, dim3(LPB * (MM / 8)), 0, st, a);
+    else hipLaunchKernelGGL((k_z_rbc<T, MM>), grid, dim3(LPB * (MM / 8)), 0, st, a);
     return hipGetLastError();
 }
 
@@ -698,9 +726,10 @@ hipError_t launch_xfwd_sc(const Grid &g, void *const *W, int ncomp, int parity, 
 
 template <typename T>
 hipError_t launch_z_rbc(const Grid &g, void *const *U, void *const *P, const void *twm, const void *twn,
-                        hipStream_t st)
+                        hipStream_t st, bool shift)
 {
     ZArgs a{};
+    a.ncomp = shift ? 8 : 9;
     for (int i = 0; i < 4; i++) a.in[i] = U[i];
     for (int i = 0; i < 9; i++) a.out[i] = P[i];
     a.twm = twm;
@@ -753,7 +782,7 @@ hipError_t launch_sff_reality(const Grid &g, double *const *V, int ncomp, int mo
     template hipError_t launch_xfwd_sc<T>(const Grid &, void *const *, int, int, double *const *, double,           \
                                           const void *, const void *, hipStream_t);                                 \
     template hipError_t launch_z_rbc<T>(const Grid &, void *const *, void *const *, const void *, const void *,     \
-                                        hipStream_t);                                                               \
+                                        hipStream_t, bool);                                                         \
     template hipError_t launch_x_rbc<T>(const Grid &, const RbcParams &, void *const *, double *const *,            \
                                         double *const *, void *const *, const double *const *, double *const *,     \
                                         double *const *, const void *, const void *, hipStream_t);
