@@ -26,7 +26,7 @@ Besides the headline FP64 number the line reports:
              512 x 1024 x 1024 FP64, Ra = 1e8, Pr = 1 (nu = kappa = 1e-4),
              theta = white noise of amplitude 1e-3 in the first 32 modes per
              axis, u = 0, corner dealias, weak reality; RK4 steps/s, per-kernel
-             times, step roofline (8 (4 + 9) S + 11 * 4 S64 bytes per step) and
+             times, step roofline (8 (4 + 8) S + 11 * 4 S64 bytes per step) and
              the Nusselt number;
   adjoint    the adjoint/forward cost ratio on its own workload: 256^3 FP64,
              200 unforced RK4 steps (no reality condition), checkpoints
@@ -437,7 +437,8 @@ def rbc_run(api, torch, np, prec, N, steps, warmup, Ra, Pr, dt, seed, profile):
 def rbc_rate(api, torch, np, N=(512, 1024, 1024), steps=3, warmup=1, Ra=1e8, Pr=1.0, dt=1e-3, seed=6):
     r = rbc_run(api, torch, np, api.PREC_FP64, N, steps, warmup, Ra, Pr, dt, seed, True)
     ms, S = r["ms"], r["S"]
-    model = 8 * (4 + 9) * S + 11 * 4 * S
+    # a stage transforms 4 inputs and 8 products (the velocity products trace-shifted, zz not formed)
+    model = 8 * (4 + 8) * S + 11 * 4 * S
     print(f"[bench] rbc {N}: {ms:.1f} ms/step, Nu {r['nusselt']:.12g}", file=sys.stderr, flush=True)
     # FP32 transforms from the same fields, the same steps: Nusselt number and energies against FP64
     r32 = rbc_run(api, torch, np, api.PREC_FP32, N, steps, warmup, Ra, Pr, dt, seed, False)
