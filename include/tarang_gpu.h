@@ -342,7 +342,7 @@ enum {
     TG_KERNEL_Z_ADJ = 11,       /* adjoint: z c2r + products with the velocity tape + r2c */
     TG_KERNEL_X_FWD_ADJ = 12,   /* adjoint: x forward + RK4 stage transpose */
     TG_KERNEL_X_SC = 13,        /* SFF sin/cos x passes (transforms, stage-1 input, patches) */
-    TG_KERNEL_Z_RBC = 14,       /* RBC z pass: 4 c2r, 9 products, 9 r2c */
+    TG_KERNEL_Z_RBC = 14,       /* RBC z pass: 4 c2r, 9 products (8 in an RK stage), r2c of each */
     TG_KERNEL_X_RBC = 15,       /* RBC x pass: sin/cos forward + N, N_T + force + projection + RK4 stage */
     TG_KERNEL_COUNT = 16
 };
