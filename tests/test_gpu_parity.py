@@ -92,6 +92,41 @@ def test_compute_rhs_forced():
     assert e < FP64_TOL
 
 
+@pytest.mark.parametrize("prec,tol", [(api.PREC_FP64, 1e-13), (api.PREC_FP32, 1e-5)])
+def test_rhs_trace_shift_is_a_projected_gradient(prec, tol):
+    """The projected right-hand side (five trace-shifted products) equals
+    -P N with N from Compute_nlin (the reference's six products), both on the
+    GPU: the shift changes N by a gradient only, which P removes mode by mode.
+    The projection here is numpy's, from the same wavenumbers."""
+    N = (64, 32, 64)
+    log(f"test_rhs_trace_shift_is_a_projected_gradient prec={prec}")
+    o = to.Oracle(N)
+    V = _init(o, kind=1, seed=14)
+    b = api.FFF_Basis(N, precision=prec)
+    U = api.FluidVF(b, 0.01)
+    U.set_field([dev(v) for v in V])
+    api.Nlin_incompress.Compute_nlin(U)
+    Nl = [host(n) for n in U.nlin]
+    U.set_field([dev(v) for v in V])
+    api.Time_advance_incompress(U, 1e-3).Compute_force_TO_rhs()
+    R = [host(n) for n in U.nlin]
+    # the reference's wavenumbers: +N/2 at the Nyquist index (get_kx/get_ky), which
+    # matters in K.N (numpy's fftfreq takes -N/2 there)
+    kx = np.array([l if l <= N[0] // 2 else l - N[0] for l in range(N[0])], float)[:, None, None]
+    ky = np.array([l if l <= N[1] // 2 else l - N[1] for l in range(N[1])], float)[None, :, None]
+    kz = np.arange(N[2] // 2 + 1)[None, None, :]
+    K = [kx + 0 * ky + 0 * kz, ky + 0 * kx + 0 * kz, kz + 0 * kx + 0 * ky]
+    K2 = K[0] ** 2 + K[1] ** 2 + K[2] ** 2
+    div = sum(K[i] * Nl[i] for i in range(3))
+    with np.errstate(invalid="ignore", divide="ignore"):
+        q = np.where(K2 > 0, div / K2, 0.0)
+    ref = [-(Nl[i] - K[i] * q) for i in range(3)]
+    e = rel_l2_vec(R, ref)
+    shift = rel_l2_vec(R, [-n for n in Nl])
+    log(f"  RHS vs -P N: rel L2 {e:.3e} (vs -N unprojected: {shift:.3e})")
+    assert e < tol and shift > 1e-3
+
+
 def test_compute_nlin_giver():
     N = (32, 32, 32)
     o = to.Oracle(N)
