@@ -203,6 +203,29 @@ def test_rk4_steps_reality_planes(mode, prec, tol):
     assert e < tol
 
 
+def test_rk4_fifty_steps():
+    """Fifty forced RK4 steps at 64^3 (strong reality, corner dealias) in
+    chained calls against the oracle's six-product steps: the trace-shifted
+    stage stays within rounding over a long run (state and energy)."""
+    N, nu, dt, force = (64, 64, 64), 0.005, 2e-3, (1, 0.1, 2.0, 3.0)
+    log("test_rk4_fifty_steps")
+    o = to.Oracle(N, dealias=to.DEALIAS_REF_CORNER, strong_reality=True)
+    V = _init(o, kind=1, seed=17)
+    b = api.FFF_Basis(N, dealias=api.DEALIAS_REF_CORNER, strong_reality=True)
+    U = api.FluidVF(b, nu)
+    U.set_field([dev(v) for v in V])
+    ta = api.Time_advance_incompress(U, dt, *force)
+    fl = to.Fluid(nu, dt, *force)
+    for s in range(50):
+        o.rk4_step(fl, V)
+        ta.Time_advance_step(1)
+        if (s + 1) % 10 == 0:
+            e = rel_l2_vec([host(v) for v in U.cvf], V)
+            E, E_ref = U.Compute_total_energy(), sum(o.energy(v) for v in V)
+            log(f"  step {s + 1}: rel L2 {e:.3e}, energy {abs(E - E_ref) / E_ref:.3e}")
+            assert e < 1e-11 and abs(E - E_ref) < 1e-12 * E_ref
+
+
 def test_rk4_fp32_vs_fp64():
     N = (32, 32, 32)
     o = to.Oracle(N)
