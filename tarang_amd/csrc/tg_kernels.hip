@@ -483,7 +483,10 @@ __global__ void __launch_bounds__(CB *(L / 8), 2) k_x_nlin(XnlArgs a)
     __shared__ __attribute__((aligned(16))) char rawB[CB * LP * sizeof(cpx<double>)];
     __shared__ int colw[CB], cols[CB];
     // twiddles are read from LDS: kept out of registers across the product loop
-    __shared__ cpx<T> twl[L];
+    // (1024-point lines: from global memory, L1-resident, so that two
+    // blocks fit the LDS of a CU)
+    constexpr bool TWL = L < 1024;
+    __shared__ cpx<T> twl[TWL ? L : 1];
     cpx<double> *ldd = (cpx<double> *)rawA;
     cpx<T> *ldt = (cpx<T> *)rawA;
     cpx<double> *ldb = (cpx<double> *)rawB;
@@ -497,8 +500,9 @@ __global__ void __launch_bounds__(CB *(L / 8), 2) k_x_nlin(XnlArgs a)
     const int ly = cw >= 0 ? cw / g.nzp : 0, lz = cw >= 0 ? cw % g.nzp : 0;
     const int ky = kwave(ly, g.ny);
     const double Ky = ky * g.kf[1], Kz = lz * g.kf[2];
-    for (int i = threadIdx.x; i < L; i += CB * TPL) twl[i] = ((const cpx<T> *)a.tw)[i];
-    const cpx<T> *tw = twl;
+    if constexpr (TWL)
+        for (int i = threadIdx.x; i < L; i += CB * TPL) twl[i] = ((const cpx<T> *)a.tw)[i];
+    const cpx<T> *tw = TWL ? (const cpx<T> *)twl : (const cpx<T> *)a.tw;
     const int ncw = g.ny * g.nzp, ncs = g.ny * g.nzc;
     const T sc = (T)a.rk.scale;
     const T Kyt = (T)Ky, Kzt = (T)Kz;

@@ -29,12 +29,16 @@ TOL = {api.PREC_FP64: 1e-12, api.PREC_FP32: 2e-5}
 RBC_KF = (np.pi, np.pi / np.sqrt(2.0), np.pi / np.sqrt(2.0))
 
 
-@pytest.mark.parametrize("N", [(512, 8, 16), (16, 8, 512)])
+@pytest.mark.parametrize("N", [(512, 8, 16), (16, 8, 512), (1024, 8, 16)])
 @pytest.mark.parametrize("dealias", [to.DEALIAS_REF_CORNER, to.DEALIAS_TWOTHIRDS])
 @pytest.mark.parametrize("prec", [api.PREC_FP64, api.PREC_FP32])
 def test_ns_timed_instances(N, dealias, prec):
+    """(1024, 8, 16): the x pass at 1024-point lines (twiddles from global
+    memory, two-column tiles)."""
     log(f"test_ns_timed_instances N={N} dealias={dealias} prec={prec}")
-    tol = TOL[prec]
+    # FP32: the rounding of the transforms is scaled by wavenumbers up to N/2
+    # in the derivatives (the field holds modes up to N/3)
+    tol = TOL[prec] * (max(1, max(N) // 512) if prec == api.PREC_FP32 else 1)
     o = to.Oracle(N, dealias=dealias, strong_reality=True)
     V = o.init_field(kind=1, k0=4.0, kmin=1.0, kmax=max(N) / 3, energy=0.5, seed=31)
     b = api.FFF_Basis(N, precision=prec, dealias=dealias, strong_reality=True)
